@@ -1,0 +1,274 @@
+/*
+ * tagmatch_amd.h -- C ABI of the MI355X (gfx950) TagMatch matching engine.
+ *
+ * Plain C linkage, plain pointers and sizes, int status codes, no exceptions.
+ * This is the device boundary that replaces the reference's CUDA wrapper
+ * class `gpu` (reference gpu.hh:32-77) and the device-facing half of its
+ * back end (reference back_end.hh:19-39).  The C++ facade `class tagmatch`
+ * (include/tagmatch/tagmatch.hh) is built on top of it; other languages bind
+ * these symbols directly (ctypes, cgo, JNI, N-API).
+ *
+ * Data conventions
+ *   filter / query : 6 x uint32 little-endian words; bit p of the 192-bit
+ *                    filter is bit (p % 32) of word (p / 32).  This is the
+ *                    reference's bitvector<192> memory image read as uint32
+ *                    (reference bitvector.hh:104-139, :210-212).
+ *   key            : uint32 (reference key.hh:6).
+ *   subset test    : f is matched by q iff (f[j] & ~q[j]) == 0 for j < 6
+ *                    (reference bitvector.hh:290-296, gpu.cu:48-50).
+ *
+ * Ownership: the caller owns every host buffer passed in and keeps it valid
+ * until the call returns (synchronous calls) or until tm_wait() reports the
+ * submission finished (asynchronous calls).  The library owns all device
+ * memory it allocates.  A tm_fib or tm_ring is used by one host thread at a
+ * time; distinct rings may be driven from distinct threads.
+ */
+#ifndef TAGMATCH_AMD_H_INCLUDED
+#define TAGMATCH_AMD_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes.  The reference prints and calls exit(0) on any CUDA error
+ * (reference gpu.cu:14-24); every entry point here returns one of these. */
+#define TM_OK             0
+#define TM_ERR_INVALID   -1   /* bad argument or size                        */
+#define TM_ERR_HIP       -2   /* HIP runtime error; see tm_last_error()      */
+#define TM_ERR_NOMEM     -3   /* device or pinned allocation failed          */
+#define TM_ERR_OVERFLOW  -4   /* result capacity exceeded; retry larger      */
+#define TM_ERR_STATE     -5   /* call out of sequence (no device, no FIB)    */
+#define TM_ERR_NODEV     -6   /* no HIP device / library built without one   */
+#define TM_ERR_BUSY      -7   /* submission still running (tm_poll)          */
+
+#define TM_FILTER_WORDS   6
+#define TM_BLOCK_FILTERS  64  /* filters per device block (one per lane)    */
+
+/* Thread-local text of the last error. */
+const char * tm_last_error(void);
+const char * tm_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Device lifecycle.
+ * Replaces gpu::initialize / gpu::set_device / gpu::shutdown
+ * (reference gpu.cu:778-794, :914-922) and gpu::mem_info (gpu.cu:796-807). */
+int tm_dev_init(int device);
+int tm_dev_shutdown(void);
+int tm_dev_count(int * count);
+int tm_mem_info(size_t * free_bytes, size_t * total_bytes);
+int tm_dev_synchronize(void);
+/* Name of the active device's architecture (e.g. "gfx950"). */
+int tm_dev_arch(char * buf, size_t len);
+
+/* ------------------------------------------------------------------------ */
+/* Partitioner: the balanced partitioning of a filter table, run on the GPU.
+ * Replaces partitioning::balanced_partitioning + partitioner_gpu
+ * (reference partitioning.cc:382-432, partitioner_gpu.cu:15-158) with a
+ * level-synchronous restatement of the same split rules: bit histograms,
+ * pivot choice and stable pivot splits all run as HIP kernels.
+ *
+ *   words      : n filters (n x 6 words), usually lexicographically sorted.
+ *   maxp       : maximum partition size (reference partitioning.cc:366).
+ *   perm       : out, n entries; filter perm[i] of the input is at position i
+ *                of the partitioned order.  Each partition is a contiguous
+ *                run and keeps the input's relative order.
+ *   part_offsets: out, capacity max_parts + 1; partition p is
+ *                [part_offsets[p], part_offsets[p+1]).
+ *   part_masks : out, capacity max_parts x 6; AND of the partition's filters
+ *                (all-ones for an empty partition, reference
+ *                partitioning.cc:412-423).  A partition of all-zero filters
+ *                has an all-zero mask; the reference cannot terminate on it
+ *                (partitioning.cc:251-271), this engine keeps it as a
+ *                partition that every query selects.
+ *   n_parts    : out, number of partitions.
+ * Returns TM_ERR_OVERFLOW if more than max_parts partitions result. */
+int tm_partition(const uint32_t * words, uint32_t n, uint32_t maxp,
+				 uint32_t * perm, uint32_t * part_offsets, uint32_t * part_masks,
+				 uint32_t max_parts, uint32_t * n_parts);
+
+/* ------------------------------------------------------------------------ */
+/* Device FIB: filters, partition masks and keys resident in HBM.
+ * Replaces back_end::compile_fibs and its per-partition cudaMalloc/copy
+ * (reference back_end.cc:404-494, gpu.cu:825-830); the keys, which the
+ * reference keeps on the host (back_end.cc:109-129), are resident too. */
+typedef struct tm_fib tm_fib;
+
+typedef struct tm_fib_stats {
+	uint64_t n_filters;      /* filters uploaded                           */
+	uint64_t n_blocks;       /* 64-filter device blocks (partitions padded) */
+	uint64_t n_parts;        /* partitions                                 */
+	uint64_t n_keys;         /* key instances (sum of key-set sizes)       */
+	uint64_t n_distinct_keys;/* size of the key dictionary                 */
+	uint64_t device_bytes;   /* HBM held by this FIB                       */
+	uint64_t max_part_size;  /* largest partition, filters                 */
+} tm_fib_stats;
+
+int tm_fib_create(tm_fib ** out);
+int tm_fib_destroy(tm_fib * fib);
+
+/* Upload a partitioned table (replaces any previous content).
+ *   words        : n x 6 words, grouped by partition.
+ *   part_offsets : n_parts + 1 entries, partition p = [off[p], off[p+1]).
+ *   part_masks   : n_parts x 6 words; a partition is sent a query q iff
+ *                  mask & ~q == 0 (reference front_end.cc:465-477).  Must be
+ *                  a subset of every member filter.
+ *   key_offsets  : n + 1 entries; keys of filter i = keys[ko[i] .. ko[i+1]).
+ *   keys         : key values (any uint32).
+ *   filter_ids   : n ids reported by TM_MODE_HITS (NULL: position i).
+ * Within each partition the engine keeps the given order; sorted order makes
+ * the per-block masks (AND of 64 neighbours) most selective. */
+int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
+				  const uint32_t * part_offsets, const uint32_t * part_masks, uint32_t n_parts,
+				  const uint32_t * key_offsets, const uint32_t * keys,
+				  const uint32_t * filter_ids);
+int tm_fib_stats_get(const tm_fib * fib, tm_fib_stats * st);
+
+/* ------------------------------------------------------------------------ */
+/* Matching.
+ *
+ * A ring is one in-flight submission context on its own HIP stream: a query
+ * buffer, the device-side lookup lists and work queue, and output buffers.
+ * Rings replace the reference's stream_handle double buffers
+ * (reference back_end.cc:172-343) and its 256-query __constant__ upload
+ * (gpu.cu:39-40, :809-811).  Several rings may be in flight at once. */
+typedef struct tm_ring tm_ring;
+
+/* Output modes. */
+#define TM_MODE_KEYS_UNIQUE 0  /* per query: sorted, deduplicated keys
+                                  (match_unique, reference
+                                  tagmatch_query.hh:99-119)                  */
+#define TM_MODE_KEYS_ALL    1  /* per query: keys of every matched filter,
+                                  duplicates kept, order unspecified (match,
+                                  reference tagmatch.cc:239-243)             */
+#define TM_MODE_HITS        2  /* per query: ids of matched filters, order
+                                  unspecified                                */
+
+/* Schedules. */
+#define TM_SCHED_PLAIN      0  /* one kernel launch per (partition, batch),
+                                  the reference's dispatch shape
+                                  (back_end.cc:284-327, gpu.cu:878-912)      */
+#define TM_SCHED_PERSISTENT 1  /* one persistent, work-stealing launch over
+                                  every (partition, query tile, block chunk) */
+
+typedef struct tm_match_config {
+	uint32_t mode;           /* TM_MODE_*                                   */
+	uint32_t schedule;       /* TM_SCHED_*                                  */
+	uint32_t batch_size;     /* PLAIN: queries per (partition, batch) launch;
+	                            0 = 256 (reference parameters.hh:22)        */
+	uint32_t chunk_blocks;   /* blocks per work item; 0 = autotuned          */
+	uint32_t waves_per_group;/* waves per workgroup (1..16); 0 = autotuned   */
+	uint32_t groups_per_cu;  /* PERSISTENT: resident workgroups per CU; 0 = auto */
+	uint64_t hit_capacity;   /* HITS / KEYS_ALL: hit slots; 0 = default      */
+	uint32_t use_graph;      /* PERSISTENT: replay steady-state steps from a
+	                            captured hipGraph                            */
+	uint32_t log_units;      /* record per-work-item hit counts and cycles   */
+	uint32_t order_by_cost;  /* PERSISTENT: run work items in descending
+	                            predicted cost (see tm_ring_set_costs)       */
+	uint32_t reserved[7];
+} tm_match_config;
+
+void tm_match_config_default(tm_match_config * cfg);
+
+typedef struct tm_status {
+	uint32_t n_queries;      /* queries in the submission                   */
+	uint32_t n_items;        /* work items (PERSISTENT) or launches (PLAIN)  */
+	uint64_t n_pairs;        /* (query, selected partition) pairs           */
+	uint64_t n_hits;         /* (query, filter) matches                     */
+	uint64_t n_results;      /* values written to the result array          */
+	uint64_t hit_capacity;   /* hit slots available in HITS / KEYS_ALL mode */
+	int32_t  overflow;       /* 1 if hits exceeded hit_capacity             */
+	int32_t  error;          /* TM_* status of the submission               */
+	double   device_ms;      /* stream time from first to last kernel       */
+	double   match_ms;       /* time of the match kernel(s) alone           */
+} tm_status;
+
+/* Create a ring for up to max_queries queries per submission. */
+int tm_ring_create(tm_fib * fib, uint32_t max_queries, const tm_match_config * cfg,
+				   tm_ring ** out);
+int tm_ring_destroy(tm_ring * ring);
+int tm_ring_config(tm_ring * ring, const tm_match_config * cfg);
+
+/* Device pointer of the ring's query buffer (max_queries x 8 words: 6 filter
+ * words + 2 pad), for callers that keep queries resident in HBM. */
+int tm_ring_query_buffer(tm_ring * ring, uint32_t ** dev_words);
+
+/* Submit nq queries (host words, nq x 6): asynchronous H2D copy + lookup +
+ * match + key expansion/dedup on the ring's stream.  Returns immediately. */
+int tm_submit(tm_ring * ring, const uint32_t * query_words, uint32_t nq);
+/* Same, with the queries already in tm_ring_query_buffer(). */
+int tm_submit_resident(tm_ring * ring, uint32_t nq);
+/* TM_OK when the last submission finished, TM_ERR_BUSY while running. */
+int tm_poll(tm_ring * ring);
+/* Block until the last submission finished; fill *st (may be NULL). */
+int tm_wait(tm_ring * ring, tm_status * st);
+/* Copy results of the finished submission to the host:
+ *   offsets : nq + 1 entries; results of query i = values[off[i] .. off[i+1])
+ *   values  : capacity entries (may be NULL to read offsets only).
+ * TM_ERR_OVERFLOW if capacity is too small (offsets are still written). */
+int tm_read_results(tm_ring * ring, uint64_t * offsets, uint32_t * values, uint64_t capacity);
+/* Device pointers of the results of the finished submission. */
+int tm_ring_results_device(tm_ring * ring, const uint64_t ** dev_offsets,
+						   const uint32_t ** dev_values);
+
+/* Per-partition query counts of the finished submission (the on-device
+ * lookup's result; the reference's per-partition queue fill,
+ * front_end.cc:350-376).  counts: n_parts entries. */
+int tm_ring_lookup_counts(tm_ring * ring, uint32_t * counts, uint32_t n_parts);
+
+/* One-shot synchronous match through a temporary ring. */
+int tm_match(tm_fib * fib, const uint32_t * query_words, uint32_t nq,
+			 const tm_match_config * cfg, uint64_t * offsets, uint32_t * values,
+			 uint64_t capacity, tm_status * st);
+
+/* ------------------------------------------------------------------------ */
+/* Host: canonical filter table from (filter, keys) entries, as
+ * tagmatch::add + consolidate build it (reference tagmatch.cc:22-77,
+ * :137-204): identical filters merged with the union of their keys, filters
+ * in the reference's lexicographic order (bitvector.hh:389-397), keys
+ * ascending, entries without keys dropped.  Outputs are malloc'd; release
+ * them with tm_free(). */
+int tm_canonicalize(const uint32_t * words, uint64_t n, const uint64_t * key_off, const uint32_t * keys,
+					uint32_t ** out_words, uint64_t * out_n, uint64_t ** out_key_off,
+					uint32_t ** out_keys);
+void tm_free(void * p);
+
+/* ------------------------------------------------------------------------ */
+/* Seeded synthetic workload generator (host, multi-threaded).  Not part of
+ * the reference interface; it produces the inputs of the tests and bench. */
+#define TM_GEN_MAX_TAGS 64
+#define TM_GEN_MAX_KEYS 16
+
+typedef struct tm_workload_params {
+	uint64_t seed;
+	uint32_t n_tags;         /* vocabulary size                              */
+	double   zipf_s;         /* Zipf exponent of tag popularity              */
+	uint32_t bits_per_tag;   /* Bloom hash functions per tag (k)             */
+	uint32_t ftag_min, ftag_max;  /* tags per filter                         */
+	uint32_t key_min, key_max;    /* keys per filter                         */
+	uint32_t trees, ifaces;       /* key = (tree << 16) | iface              */
+	uint32_t qtag_min, qtag_max;  /* tags per query                          */
+	double   superset_frac;       /* share of queries built over a filter    */
+} tm_workload_params;
+
+typedef struct tm_gen tm_gen;
+
+void tm_workload_default_params(tm_workload_params * p);
+tm_gen * tm_gen_create(const tm_workload_params * p);
+void tm_gen_destroy(tm_gen * g);
+int tm_gen_tag_bits(const tm_gen * g, uint32_t tag, uint8_t * bits);
+/* Filters first .. first+n-1: words n x 6, key_counts n, keys n x key_max. */
+int tm_gen_filters(const tm_gen * g, uint64_t first, uint64_t n, uint32_t * words,
+				   uint32_t * key_counts, uint32_t * keys, int threads);
+/* Queries first .. first+n-1 over a table of n_filters generated filters. */
+int tm_gen_queries(const tm_gen * g, uint64_t n_filters, uint64_t first, uint64_t n,
+				   uint32_t * words, uint8_t * is_superset, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAGMATCH_AMD_H_INCLUDED */

@@ -1,0 +1,419 @@
+"""Python bindings of the MI355X TagMatch engine (ctypes over the C ABI).
+
+The engine is the native library tagmatch_amd/lib/libtagmatch_amd.so
+(HIP kernels for gfx950 + C++ host code); this module only loads it and
+moves numpy arrays across the C ABI declared in include/tagmatch_amd.h.
+There is no Python or CPU fallback: every call fails loudly if the library
+or the GPU is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtagmatch_amd.so")
+
+TM_OK = 0
+TM_ERR_INVALID = -1
+TM_ERR_HIP = -2
+TM_ERR_NOMEM = -3
+TM_ERR_OVERFLOW = -4
+TM_ERR_STATE = -5
+TM_ERR_NODEV = -6
+TM_ERR_BUSY = -7
+
+MODE_KEYS_UNIQUE = 0
+MODE_KEYS_ALL = 1
+MODE_HITS = 2
+SCHED_PLAIN = 0
+SCHED_PERSISTENT = 1
+
+
+class TagMatchError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class WorkloadParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("n_tags", C.c_uint32), ("zipf_s", C.c_double),
+        ("bits_per_tag", C.c_uint32), ("ftag_min", C.c_uint32), ("ftag_max", C.c_uint32),
+        ("key_min", C.c_uint32), ("key_max", C.c_uint32), ("trees", C.c_uint32),
+        ("ifaces", C.c_uint32), ("qtag_min", C.c_uint32), ("qtag_max", C.c_uint32),
+        ("superset_frac", C.c_double),
+    ]
+
+
+class MatchConfig(C.Structure):
+    _fields_ = [
+        ("mode", C.c_uint32), ("schedule", C.c_uint32), ("batch_size", C.c_uint32),
+        ("chunk_blocks", C.c_uint32), ("waves_per_group", C.c_uint32),
+        ("groups_per_cu", C.c_uint32), ("hit_capacity", C.c_uint64), ("use_graph", C.c_uint32),
+        ("log_units", C.c_uint32), ("order_by_cost", C.c_uint32), ("reserved", C.c_uint32 * 7),
+    ]
+
+
+class Status(C.Structure):
+    _fields_ = [
+        ("n_queries", C.c_uint32), ("n_items", C.c_uint32), ("n_pairs", C.c_uint64),
+        ("n_hits", C.c_uint64), ("n_results", C.c_uint64), ("hit_capacity", C.c_uint64),
+        ("overflow", C.c_int32), ("error", C.c_int32), ("device_ms", C.c_double),
+        ("match_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FibStats(C.Structure):
+    _fields_ = [
+        ("n_filters", C.c_uint64), ("n_blocks", C.c_uint64), ("n_parts", C.c_uint64),
+        ("n_keys", C.c_uint64), ("n_distinct_keys", C.c_uint64), ("device_bytes", C.c_uint64),
+        ("max_part_size", C.c_uint64),
+    ]
+
+
+P = C.c_void_p
+U32P = C.POINTER(C.c_uint32)
+U64P = C.POINTER(C.c_uint64)
+
+_SIGS = {
+    "tm_last_error": (C.c_char_p, []),
+    "tm_version": (C.c_char_p, []),
+    "tm_dev_init": (C.c_int, [C.c_int]),
+    "tm_dev_shutdown": (C.c_int, []),
+    "tm_dev_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "tm_mem_info": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "tm_dev_synchronize": (C.c_int, []),
+    "tm_dev_arch": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "tm_partition": (C.c_int, [P, C.c_uint32, C.c_uint32, P, P, P, C.c_uint32, U32P]),
+    "tm_fib_create": (C.c_int, [C.POINTER(P)]),
+    "tm_fib_destroy": (C.c_int, [P]),
+    "tm_fib_upload": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, P, P, P]),
+    "tm_fib_stats_get": (C.c_int, [P, C.POINTER(FibStats)]),
+    "tm_match_config_default": (None, [C.POINTER(MatchConfig)]),
+    "tm_ring_create": (C.c_int, [P, C.c_uint32, C.POINTER(MatchConfig), C.POINTER(P)]),
+    "tm_ring_destroy": (C.c_int, [P]),
+    "tm_ring_config": (C.c_int, [P, C.POINTER(MatchConfig)]),
+    "tm_ring_query_buffer": (C.c_int, [P, C.POINTER(P)]),
+    "tm_submit": (C.c_int, [P, P, C.c_uint32]),
+    "tm_submit_resident": (C.c_int, [P, C.c_uint32]),
+    "tm_poll": (C.c_int, [P]),
+    "tm_wait": (C.c_int, [P, C.POINTER(Status)]),
+    "tm_read_results": (C.c_int, [P, P, P, C.c_uint64]),
+    "tm_ring_results_device": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
+    "tm_ring_lookup_counts": (C.c_int, [P, P, C.c_uint32]),
+    "tm_match": (C.c_int, [P, P, C.c_uint32, C.POINTER(MatchConfig), P, P, C.c_uint64, C.POINTER(Status)]),
+    "tm_canonicalize": (C.c_int, [P, C.c_uint64, P, P, C.POINTER(P), U64P, C.POINTER(P), C.POINTER(P)]),
+    "tm_free": (None, [P]),
+    "tm_workload_default_params": (None, [C.POINTER(WorkloadParams)]),
+    "tm_gen_create": (P, [C.POINTER(WorkloadParams)]),
+    "tm_gen_destroy": (None, [P]),
+    "tm_gen_tag_bits": (C.c_int, [P, C.c_uint32, P]),
+    "tm_gen_filters": (C.c_int, [P, C.c_uint64, C.c_uint64, P, P, P, C.c_int]),
+    "tm_gen_queries": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, P, P, C.c_int]),
+}
+
+# Declared in include/tagmatch_amd.h; tests check every one is exported.
+EXPORTED = sorted(_SIGS)
+
+_lib = None
+
+
+def load():
+    """Load libtagmatch_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TagMatchError(TM_ERR_NODEV, f"{LIB_PATH} is missing: run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else C.c_void_p(0)
+
+
+def check(rc, what=""):
+    if rc != TM_OK:
+        msg = load().tm_last_error().decode(errors="replace")
+        raise TagMatchError(rc, f"{what}: {msg}" if what else msg)
+    return rc
+
+
+def _threads():
+    return int(os.environ.get("TM_HOST_THREADS", "0")) or min(16, os.cpu_count() or 1)
+
+
+# ------------------------------------------------------------------ device
+
+def init(device=0):
+    check(load().tm_dev_init(device), "tm_dev_init")
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = load().tm_dev_count(C.byref(n))
+    return n.value if rc == TM_OK else 0
+
+
+def arch():
+    buf = C.create_string_buffer(64)
+    check(load().tm_dev_arch(buf, 64), "tm_dev_arch")
+    return buf.value.decode()
+
+
+def mem_info():
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    check(load().tm_mem_info(C.byref(f), C.byref(t)), "tm_mem_info")
+    return f.value, t.value
+
+
+# ------------------------------------------------------------------ workload
+
+def default_params(**kw):
+    p = WorkloadParams()
+    load().tm_workload_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Generator:
+    """Seeded workload generator (tm_gen_*)."""
+
+    def __init__(self, params=None, **kw):
+        self.params = params if params is not None else default_params(**kw)
+        self.h = load().tm_gen_create(C.byref(self.params))
+        if not self.h:
+            raise TagMatchError(TM_ERR_INVALID, "tm_gen_create rejected the parameters")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().tm_gen_destroy(self.h)
+            self.h = None
+
+    def filters(self, n, first=0):
+        """Raw generated filters: words (n,6), key_off (n+1,), keys."""
+        kmax = self.params.key_max
+        w = np.zeros((n, 6), np.uint32)
+        kc = np.zeros(n, np.uint32)
+        k = np.zeros((n, kmax), np.uint32)
+        check(load().tm_gen_filters(self.h, first, n, _ptr(w), _ptr(kc), _ptr(k), _threads()), "tm_gen_filters")
+        key_off = np.zeros(n + 1, np.uint64)
+        np.cumsum(kc, out=key_off[1:])
+        mask = np.arange(kmax)[None, :] < kc[:, None]
+        return w, key_off, k[mask].astype(np.uint32)
+
+    def queries(self, n, n_filters, first=0):
+        q = np.zeros((n, 6), np.uint32)
+        s = np.zeros(n, np.uint8)
+        check(load().tm_gen_queries(self.h, n_filters, first, n, _ptr(q), _ptr(s), _threads()), "tm_gen_queries")
+        return q, s
+
+
+def canonicalize(words, key_off, keys):
+    """Merge identical filters, sort lexicographically (tm_canonicalize)."""
+    words = np.ascontiguousarray(words, np.uint32)
+    key_off = np.ascontiguousarray(key_off, np.uint64)
+    keys = np.ascontiguousarray(keys, np.uint32)
+    n = words.shape[0]
+    ow, ok, ov = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    on = C.c_uint64()
+    lib = load()
+    check(lib.tm_canonicalize(_ptr(words), n, _ptr(key_off), _ptr(keys), C.byref(ow), C.byref(on),
+                              C.byref(ok), C.byref(ov)), "tm_canonicalize")
+    m = on.value
+    try:
+        cw = np.ctypeslib.as_array(C.cast(ow, U32P), shape=(max(m, 1) * 6,))[: m * 6].copy().reshape(m, 6)
+        ck = np.ctypeslib.as_array(C.cast(ok, U64P), shape=(m + 1,)).copy()
+        nk = int(ck[-1])
+        cv = np.ctypeslib.as_array(C.cast(ov, U32P), shape=(max(nk, 1),))[:nk].copy()
+    finally:
+        lib.tm_free(ow)
+        lib.tm_free(ok)
+        lib.tm_free(ov)
+    return cw, ck, cv
+
+
+def workload(n_filters, n_queries, seed=None, query_first=0, **kw):
+    """Generate, canonicalize: (words, key_off, keys, queries, superset)."""
+    if seed is not None:
+        kw["seed"] = seed
+    g = Generator(**kw)
+    w, ko, k = g.filters(n_filters)
+    q, s = g.queries(n_queries, n_filters, first=query_first)
+    cw, ck, cv = canonicalize(w, ko, k)
+    return cw, ck, cv, q, s
+
+
+# ------------------------------------------------------------------ partition / FIB
+
+def partition(words, maxp, max_parts=None):
+    """GPU balanced partitioning: (perm, part_offsets, masks)."""
+    words = np.ascontiguousarray(words, np.uint32)
+    n = words.shape[0]
+    cap = max_parts if max_parts is not None else n + 1
+    perm = np.zeros(max(n, 1), np.uint32)
+    off = np.zeros(cap + 2, np.uint32)
+    masks = np.zeros((cap + 1, 6), np.uint32)
+    npart = C.c_uint32(0)
+    check(load().tm_partition(_ptr(words), n, maxp, _ptr(perm), _ptr(off), _ptr(masks), cap, C.byref(npart)),
+          "tm_partition")
+    k = npart.value
+    return perm[:n].copy(), off[: k + 1].copy(), masks[:k].copy()
+
+
+class Fib:
+    """Device FIB (tm_fib)."""
+
+    def __init__(self, words, part_offsets, part_masks, key_off, keys, filter_ids=None):
+        self.h = C.c_void_p()
+        check(load().tm_fib_create(C.byref(self.h)), "tm_fib_create")
+        self.upload(words, part_offsets, part_masks, key_off, keys, filter_ids)
+
+    def upload(self, words, part_offsets, part_masks, key_off, keys, filter_ids=None):
+        words = np.ascontiguousarray(words, np.uint32)
+        po = np.ascontiguousarray(part_offsets, np.uint32)
+        pm = np.ascontiguousarray(part_masks, np.uint32)
+        ko = np.ascontiguousarray(key_off, np.uint32)
+        kv = np.ascontiguousarray(keys, np.uint32)
+        fid = None if filter_ids is None else np.ascontiguousarray(filter_ids, np.uint32)
+        n = words.shape[0]
+        check(load().tm_fib_upload(self.h, _ptr(words), n, _ptr(po), _ptr(pm), len(po) - 1, _ptr(ko), _ptr(kv),
+                                   _ptr(fid)), "tm_fib_upload")
+
+    def stats(self):
+        s = FibStats()
+        check(load().tm_fib_stats_get(self.h, C.byref(s)), "tm_fib_stats_get")
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def close(self):
+        if self.h:
+            load().tm_fib_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_fib(words, key_off, keys, maxp):
+    """Partition a canonical table on the GPU and upload it.
+
+    Returns (fib, perm, part_offsets, masks); filter ids reported in HITS
+    mode are positions in the canonical table."""
+    perm, off, masks = partition(words, maxp)
+    pw = words[perm]
+    cnt = (key_off[1:] - key_off[:-1]).astype(np.int64)[perm]
+    ko = np.zeros(len(perm) + 1, np.uint64)
+    np.cumsum(cnt, out=ko[1:])
+    starts = key_off[:-1][perm].astype(np.int64)
+    idx = np.repeat(starts - ko[:-1].astype(np.int64), cnt) + np.arange(int(ko[-1]), dtype=np.int64)
+    pk = keys[idx] if len(idx) else np.zeros(0, np.uint32)
+    fib = Fib(pw, off, masks, ko.astype(np.uint32), pk, perm.astype(np.uint32))
+    return fib, perm, off, masks
+
+
+def config(mode=MODE_KEYS_UNIQUE, schedule=SCHED_PERSISTENT, **kw):
+    c = MatchConfig()
+    load().tm_match_config_default(C.byref(c))
+    c.mode = mode
+    c.schedule = schedule
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Ring:
+    """One in-flight submission context (tm_ring)."""
+
+    def __init__(self, fib, max_queries, cfg=None, **kw):
+        self.fib = fib
+        self.cfg = cfg if cfg is not None else config(**kw)
+        self.h = C.c_void_p()
+        self.max_queries = max_queries
+        check(load().tm_ring_create(fib.h, max_queries, C.byref(self.cfg), C.byref(self.h)), "tm_ring_create")
+        self.nq = 0
+
+    def configure(self, cfg=None, **kw):
+        self.cfg = cfg if cfg is not None else config(**kw)
+        check(load().tm_ring_config(self.h, C.byref(self.cfg)), "tm_ring_config")
+
+    def submit(self, queries):
+        q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 6)
+        self.nq = q.shape[0]
+        check(load().tm_submit(self.h, _ptr(q), self.nq), "tm_submit")
+
+    def submit_resident(self, nq):
+        self.nq = nq
+        check(load().tm_submit_resident(self.h, nq), "tm_submit_resident")
+
+    def poll(self):
+        rc = load().tm_poll(self.h)
+        if rc == TM_ERR_BUSY:
+            return False
+        check(rc, "tm_poll")
+        return True
+
+    def wait(self):
+        st = Status()
+        check(load().tm_wait(self.h, C.byref(st)), "tm_wait")
+        return st
+
+    def results(self):
+        off = np.zeros(self.nq + 1, np.uint64)
+        check(load().tm_read_results(self.h, _ptr(off), C.c_void_p(0), 0), "tm_read_results")
+        vals = np.zeros(int(off[-1]), np.uint32)
+        check(load().tm_read_results(self.h, _ptr(off), _ptr(vals), vals.size), "tm_read_results")
+        return off, vals
+
+    def lookup_counts(self):
+        n = int(self.fib.stats()["n_parts"])
+        c = np.zeros(max(n, 1), np.uint32)
+        check(load().tm_ring_lookup_counts(self.h, _ptr(c), n), "tm_ring_lookup_counts")
+        return c[:n]
+
+    def query_buffer(self):
+        p = C.c_void_p()
+        check(load().tm_ring_query_buffer(self.h, C.byref(p)), "tm_ring_query_buffer")
+        return p.value
+
+    def close(self):
+        if self.h:
+            load().tm_ring_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def match(fib, queries, mode=MODE_KEYS_UNIQUE, schedule=SCHED_PERSISTENT, **kw):
+    """Synchronous match: returns (offsets, values, status dict)."""
+    q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 6)
+    r = Ring(fib, max(q.shape[0], 1), mode=mode, schedule=schedule, **kw)
+    try:
+        r.submit(q)
+        st = r.wait()
+        off, vals = r.results()
+    finally:
+        r.close()
+    return off, vals, st.as_dict()
+
+
+def split(off, vals):
+    """Per-query lists from (offsets, values)."""
+    return [vals[int(off[i]):int(off[i + 1])] for i in range(len(off) - 1)]

@@ -1,0 +1,76 @@
+// Device-resident FIB (filters, block masks, keys, partitions) and the
+// device view passed to kernels.  Internal to libtagmatch_amd.so.
+#pragma once
+
+#include "hip_util.h"
+
+#include <vector>
+
+namespace tmx {
+
+// Layout in HBM (all arrays 256-byte aligned by hipMalloc):
+//
+//   blk_words [nb][6][64] u32   filter word j of lane l of block b at
+//                               b*384 + j*64 + l: a wave reads each word
+//                               plane with one coalesced 256-byte access.
+//   blk_info  [nb][8]     u32   AND of the block's valid filters (6 words),
+//                               valid lane count, partition id.  Read with
+//                               scalar loads (uniform per wave).
+//   slot_keys [nb][64]    u64   up to 4 dense key ids (16 bits each, 0xFFFF
+//                               = none) inline; if field 0 is 0xFFFF, bits
+//                               16..47 index an overflow list in `ovf`
+//                               (count, ids...).
+//   slot_fid  [nb][64]    u32   filter id reported in TM_MODE_HITS.
+//   key_dict  [K]         u32   sorted distinct keys; dense id = index.
+//   part_info [P][8]      u32   partition mask (6 words), first block,
+//                               block count.
+struct dev_fib_view {
+	const uint32_t * __restrict__ blk_words;
+	const uint32_t * __restrict__ blk_info;
+	const uint64_t * __restrict__ slot_keys;
+	const uint32_t * __restrict__ slot_fid;
+	const uint32_t * __restrict__ ovf;
+	const uint32_t * __restrict__ key_dict;
+	const uint32_t * __restrict__ part_info;
+	uint32_t n_parts;
+	uint32_t n_blocks;
+	uint32_t n_keys;   // K, distinct keys
+	uint32_t kw;       // 32-bit words of a per-query key bitmap: ceil(K/32)
+};
+
+constexpr uint32_t BLOCK_WORDS = 6 * 64;
+constexpr uint32_t KEY_NONE = 0xFFFFu;
+
+} // namespace tmx
+
+struct tm_fib {
+	uint32_t * blk_words = nullptr;
+	uint32_t * blk_info = nullptr;
+	uint64_t * slot_keys = nullptr;
+	uint32_t * slot_fid = nullptr;
+	uint32_t * ovf = nullptr;
+	uint32_t * key_dict = nullptr;
+	uint32_t * part_info = nullptr;
+	uint32_t n_parts = 0, n_blocks = 0, n_keys = 0;
+	uint64_t n_filters = 0, n_key_instances = 0, device_bytes = 0, max_part = 0;
+	// host copies used by the schedulers
+	std::vector<uint32_t> h_part_info;   // P x 8
+	std::vector<uint32_t> h_part_size;   // P, filters per partition
+
+	tmx::dev_fib_view view() const {
+		tmx::dev_fib_view v;
+		v.blk_words = blk_words;
+		v.blk_info = blk_info;
+		v.slot_keys = slot_keys;
+		v.slot_fid = slot_fid;
+		v.ovf = ovf;
+		v.key_dict = key_dict;
+		v.part_info = part_info;
+		v.n_parts = n_parts;
+		v.n_blocks = n_blocks;
+		v.n_keys = n_keys;
+		v.kw = (n_keys + 31) / 32;
+		return v;
+	}
+	void release();
+};

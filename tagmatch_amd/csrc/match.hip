@@ -1,0 +1,1050 @@
+// Query -> partition lookup, the batched subset-match kernel, key expansion
+// and per-query dedup, all on the device; rings, schedules and hipGraph
+// replay on the host side.
+//
+// One submission of nq queries runs as a fixed sequence of kernels on the
+// ring's stream, with no host synchronisation in the persistent schedule:
+//
+//   k_lookup_count / scan / k_lookup_fill
+//        query -> partitions with mask & ~q == 0, grouped per partition
+//        (reference front_end.cc:465-477 do_match, on the device);
+//   k_build_items
+//        cut each partition's query list into batches of 64*TILES queries
+//        and its blocks into chunks: one work item per (batch, chunk)
+//        (reference front_end.cc:350-376 batching of 256 queries);
+//   k_match
+//        per work item: for each 64-filter block, a candidate pass with one
+//        query per lane tests the block's AND mask against 64 queries at
+//        once; for each surviving query, one filter per lane tests
+//        f & ~q == 0 (reference gpu.cu:60-607 phases 2 and 3).  Hits are
+//        expanded to dense key ids and OR-ed into per-query key bitmaps
+//        (staged in LDS), or appended as (query, filter) pairs;
+//   k_bm_count / scan / k_bm_write   (KEYS_UNIQUE)
+//        bitmap -> sorted distinct keys (reference tagmatch_query.hh:99-119
+//        sort + unique, and back_end.cc:329-343 key expansion);
+//   k_hits_count / scan / k_hits_write  (HITS, KEYS_ALL)
+//        group hits per query and expand them to ids or keys.
+#include "fib.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+using namespace tmx;
+
+namespace {
+
+constexpr int WG = 256;          // threads per match workgroup (4 waves)
+constexpr int WAVES = WG / 64;
+constexpr int NQUEUE = 8;        // work queues, one per XCD of an MI355X
+constexpr uint32_t NO_QUERY = 0xFFFFFFFFu;
+constexpr size_t STAGE_LDS_MAX = 64 * 1024;
+
+__device__ inline uint32_t mbcnt64(uint64_t m) {
+	return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// ------------------------------------------------------------- lookup
+
+__global__ void __launch_bounds__(256)
+k_lookup_count(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ part_info,
+			   uint32_t P, uint32_t * __restrict__ pcount) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	const bool valid = i < nq;
+	uint32_t n0 = ~0u, n1 = ~0u, n2 = ~0u, n3 = ~0u, n4 = ~0u, n5 = ~0u;
+	if (valid) {
+		uint4 a = *reinterpret_cast<const uint4 *>(q + (size_t)i * 8);
+		uint2 b = *reinterpret_cast<const uint2 *>(q + (size_t)i * 8 + 4);
+		n0 = ~a.x; n1 = ~a.y; n2 = ~a.z; n3 = ~a.w; n4 = ~b.x; n5 = ~b.y;
+	}
+	for (uint32_t p = 0; p < P; ++p) {
+		const uint32_t * m = part_info + (size_t)p * 8;
+		uint32_t t = (m[0] & n0) | (m[1] & n1) | (m[2] & n2) | (m[3] & n3) | (m[4] & n4) | (m[5] & n5);
+		uint64_t b = __ballot(valid && t == 0);
+		if (b != 0 && (threadIdx.x & 63) == 0) atomicAdd(&pcount[p], (uint32_t)__popcll(b));
+	}
+}
+
+__global__ void __launch_bounds__(256)
+k_lookup_fill(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ part_info,
+			  uint32_t P, unsigned long long * __restrict__ pcur, uint32_t * __restrict__ list,
+			  uint64_t list_cap) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	const uint32_t lane = threadIdx.x & 63;
+	const bool valid = i < nq;
+	uint32_t n0 = ~0u, n1 = ~0u, n2 = ~0u, n3 = ~0u, n4 = ~0u, n5 = ~0u;
+	if (valid) {
+		uint4 a = *reinterpret_cast<const uint4 *>(q + (size_t)i * 8);
+		uint2 b = *reinterpret_cast<const uint2 *>(q + (size_t)i * 8 + 4);
+		n0 = ~a.x; n1 = ~a.y; n2 = ~a.z; n3 = ~a.w; n4 = ~b.x; n5 = ~b.y;
+	}
+	for (uint32_t p = 0; p < P; ++p) {
+		const uint32_t * m = part_info + (size_t)p * 8;
+		uint32_t t = (m[0] & n0) | (m[1] & n1) | (m[2] & n2) | (m[3] & n3) | (m[4] & n4) | (m[5] & n5);
+		const bool sel = valid && t == 0;
+		uint64_t b = __ballot(sel);
+		if (b != 0) {
+			unsigned long long base = 0;
+			if (lane == 0) base = atomicAdd(&pcur[p], (unsigned long long)__popcll(b));
+			base = __shfl(base, 0);
+			if (sel) {
+				unsigned long long pos = base + mbcnt64(b);
+				if (pos < list_cap) list[pos] = i;
+			}
+		}
+	}
+}
+
+// ------------------------------------------------------------- work items
+
+// Single workgroup.  For each schedule rank r (partition porder[r]): clamp
+// its query count to the list capacity, cut it into batches of bq queries
+// and its blocks into chunks of `chunk`; items of rank r are
+// [item_prefix[r], item_prefix[r+1]), ordered chunk-major so that items
+// taken one after another share filter blocks.  The item range is then cut
+// into NQUEUE contiguous queues.
+__global__ void __launch_bounds__(1024)
+k_build_items(const uint32_t * __restrict__ pcount, const uint64_t * __restrict__ pbase,
+			  uint64_t list_cap, const uint32_t * __restrict__ part_info, uint32_t P,
+			  const uint32_t * __restrict__ porder, uint32_t bq, uint32_t chunk,
+			  uint4 * __restrict__ sched, uint64_t * __restrict__ item_prefix,
+			  uint64_t * __restrict__ qbeg, unsigned long long * __restrict__ qnext) {
+	__shared__ uint64_t s_w[16];
+	const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+	uint64_t carry = 0;
+	for (uint32_t base = 0; base < P; base += 1024) {
+		uint32_t r = base + tid;
+		uint64_t items = 0;
+		if (r < P) {
+			uint32_t p = porder ? porder[r] : r;
+			uint64_t c = pcount[p], b = pbase[p];
+			if (b >= list_cap) c = 0;
+			else if (b + c > list_cap) c = list_cap - b;
+			uint32_t nbat = (uint32_t)((c + bq - 1) / bq);
+			uint32_t nblk = part_info[(size_t)p * 8 + 7];
+			uint32_t nch = (nblk + chunk - 1) / chunk;
+			items = (uint64_t)nbat * nch;
+			sched[r] = make_uint4(p, nbat, nch, (uint32_t)c);
+		}
+		uint64_t x = items;
+		for (int o = 1; o < 64; o <<= 1) {
+			uint64_t y = __shfl_up(x, o, 64);
+			if (lane >= o) x += y;
+		}
+		if (lane == 63) s_w[w] = x;
+		__syncthreads();
+		uint64_t pre = 0, tot = 0;
+		for (int k = 0; k < 16; ++k) {
+			if (k < (int)w) pre += s_w[k];
+			tot += s_w[k];
+		}
+		if (r < P) item_prefix[r] = carry + pre + x - items;
+		carry += tot;
+		__syncthreads();
+	}
+	if (tid == 0) item_prefix[P] = carry;
+	if (tid <= NQUEUE) qbeg[tid] = carry * tid / NQUEUE;
+	if (tid < NQUEUE) qnext[tid] = carry * tid / NQUEUE;
+}
+
+// ------------------------------------------------------------- match
+
+struct match_args {
+	dev_fib_view f;
+	const uint32_t * __restrict__ q;          // nq x 8 words
+	const uint32_t * __restrict__ list;       // query indices grouped per partition
+	const uint64_t * __restrict__ pbase;      // list offset per partition
+	const uint4 * __restrict__ sched;         // per rank: partition, batches, chunks, queries
+	const uint64_t * __restrict__ item_prefix;// per rank, n_ranks + 1
+	uint32_t n_ranks;
+	uint32_t chunk;                           // blocks per work item
+	unsigned long long * qnext;               // NQUEUE queue cursors
+	const uint64_t * __restrict__ qbeg;       // NQUEUE + 1 queue bounds
+	int32_t plain_rank;                       // >= 0: one (rank, batch) unit, chunk = blockIdx.x
+	uint32_t plain_batch;
+	uint32_t * gbm;                           // KEYS_UNIQUE: nq x kw bitmap words
+	uint2 * hits;                             // HITS: (query, block slot)
+	uint64_t hit_cap;
+	unsigned long long * counters;            // [0] hit slots taken, [1] hits
+	uint32_t * log;                           // per item: hits, 10 ns ticks
+	uint64_t log_cap;
+};
+
+// OR the dense key ids of a filter into a bitmap (LDS or global).
+__device__ inline void add_keys(uint32_t * bm, uint64_t kv, const uint32_t * __restrict__ ovf) {
+	uint32_t k0 = (uint32_t)(kv & 0xFFFFu);
+	if (k0 != KEY_NONE) {
+		atomicOr(&bm[k0 >> 5], 1u << (k0 & 31));
+		uint32_t k1 = (uint32_t)((kv >> 16) & 0xFFFFu);
+		if (k1 != KEY_NONE) {
+			atomicOr(&bm[k1 >> 5], 1u << (k1 & 31));
+			uint32_t k2 = (uint32_t)((kv >> 32) & 0xFFFFu);
+			if (k2 != KEY_NONE) {
+				atomicOr(&bm[k2 >> 5], 1u << (k2 & 31));
+				uint32_t k3 = (uint32_t)(kv >> 48);
+				if (k3 != KEY_NONE) atomicOr(&bm[k3 >> 5], 1u << (k3 & 31));
+			}
+		}
+	} else {
+		uint32_t off = (uint32_t)(kv >> 16);
+		uint32_t n = ovf[off];
+		for (uint32_t i = 0; i < n; ++i) {
+			uint32_t id = ovf[off + 1 + i];
+			atomicOr(&bm[id >> 5], 1u << (id & 31));
+		}
+	}
+}
+
+// MODE 0: key bitmaps (KEYS_UNIQUE); MODE 1: (query, slot) hit pairs.
+// STAGE: accumulate key bitmaps of the item's queries in LDS, flush once.
+// TILES: 64-query tiles per work item.
+template <int MODE, bool STAGE, int TILES>
+__global__ void __launch_bounds__(WG)
+k_match(match_args a) {
+	constexpr int NQ = TILES * 64;
+	__shared__ uint32_t s_q[NQ * 8];      // per query: 6 inverted words, query index, pad
+	__shared__ uint32_t s_item[5];        // partition, batch, chunk, queries in batch, done
+	__shared__ unsigned long long s_id;
+	__shared__ uint32_t s_hits;
+	extern __shared__ uint32_t s_bm[];    // STAGE: NQ x kw
+	const uint32_t tid = threadIdx.x, lane = tid & 63;
+	const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const uint32_t kw = a.f.kw;
+	const uint32_t home = blockIdx.x % NQUEUE;
+
+	for (int iter = 0;; ++iter) {
+		uint64_t t_start = 0;
+		if (tid == 0) {
+			s_item[4] = 1;
+			s_hits = 0;
+			if (a.plain_rank >= 0) {
+				if (iter == 0) {
+					uint4 sc = a.sched[a.plain_rank];
+					s_item[0] = sc.x;
+					s_item[1] = a.plain_batch;
+					s_item[2] = blockIdx.x;
+					uint32_t rem = sc.w - a.plain_batch * NQ;
+					s_item[3] = rem < (uint32_t)NQ ? rem : (uint32_t)NQ;
+					s_item[4] = 0;
+					s_id = ~0ULL;
+				}
+			} else {
+				unsigned long long item = ~0ULL;
+				for (int k = 0; k < NQUEUE; ++k) {
+					uint32_t x = (home + k) % NQUEUE;
+					if (a.qnext[x] >= a.qbeg[x + 1]) continue;   // drained: skip the atomic
+					unsigned long long i = atomicAdd(&a.qnext[x], 1ULL);
+					if (i < a.qbeg[x + 1]) { item = i; break; }
+				}
+				if (item != ~0ULL) {
+					uint32_t lo = 0, hi = a.n_ranks;   // last rank with prefix <= item
+					while (hi - lo > 1) {
+						uint32_t mid = (lo + hi) / 2;
+						if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
+					}
+					uint4 sc = a.sched[lo];
+					uint64_t local = item - a.item_prefix[lo];
+					uint32_t batch = (uint32_t)(local % sc.y);
+					s_item[0] = sc.x;
+					s_item[1] = batch;
+					s_item[2] = (uint32_t)(local / sc.y);
+					uint32_t rem = sc.w - batch * NQ;
+					s_item[3] = rem < (uint32_t)NQ ? rem : (uint32_t)NQ;
+					s_item[4] = 0;
+					s_id = item;
+				}
+			}
+			t_start = wall_clock64();
+		}
+		__syncthreads();
+		if (s_item[4]) break;
+		const uint32_t p = s_item[0], batch = s_item[1], chunk = s_item[2], nqv = s_item[3];
+
+		// stage the batch's queries (inverted) in LDS
+		const uint64_t lbase = a.pbase[p] + (uint64_t)batch * NQ;
+		for (uint32_t t = tid; t < (uint32_t)NQ; t += WG) {
+			uint32_t * d = &s_q[t * 8];
+			if (t < nqv) {
+				uint32_t qi = a.list[lbase + t];
+				uint4 x = *reinterpret_cast<const uint4 *>(a.q + (size_t)qi * 8);
+				uint2 y = *reinterpret_cast<const uint2 *>(a.q + (size_t)qi * 8 + 4);
+				d[0] = ~x.x; d[1] = ~x.y; d[2] = ~x.z; d[3] = ~x.w; d[4] = ~y.x; d[5] = ~y.y;
+				d[6] = qi;
+			} else {
+				d[0] = d[1] = d[2] = d[3] = d[4] = d[5] = 0;
+				d[6] = NO_QUERY;
+			}
+		}
+		if (STAGE)
+			for (uint32_t w = tid; w < (uint32_t)NQ * kw; w += WG) s_bm[w] = 0;
+		__syncthreads();
+
+		// candidate-pass operands: query (t, lane) of each tile
+		uint32_t qv[TILES][6];
+		uint64_t qok[TILES];
+#pragma unroll
+		for (int t = 0; t < TILES; ++t) {
+			const uint32_t * d = &s_q[(t * 64 + lane) * 8];
+#pragma unroll
+			for (int j = 0; j < 6; ++j) qv[t][j] = d[j];
+			qok[t] = __ballot(d[6] != NO_QUERY);
+		}
+
+		const uint32_t * pi = a.f.part_info + (size_t)p * 8;
+		const uint32_t fb = pi[6], nb = pi[7];
+		const uint32_t b0 = fb + chunk * a.chunk;
+		const uint32_t b1 = min(fb + nb, b0 + a.chunk);
+		uint32_t hit_acc = 0;
+
+		for (uint32_t b = b0 + wave; b < b1; b += WAVES) {
+			const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
+			const uint32_t m0 = bi[0], m1 = bi[1], m2 = bi[2], m3 = bi[3], m4 = bi[4], m5 = bi[5];
+			const uint32_t nv = bi[6];
+			uint64_t cand[TILES];
+			uint64_t any = 0;
+#pragma unroll
+			for (int t = 0; t < TILES; ++t) {
+				uint32_t x = (m0 & qv[t][0]) | (m1 & qv[t][1]) | (m2 & qv[t][2]) |
+							 (m3 & qv[t][3]) | (m4 & qv[t][4]) | (m5 & qv[t][5]);
+				cand[t] = __ballot(x == 0) & qok[t];
+				any |= cand[t];
+			}
+			if (any == 0) continue;
+
+			const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
+			const uint32_t f0 = bw[0], f1 = bw[64], f2 = bw[128], f3 = bw[192], f4 = bw[256], f5 = bw[320];
+			const uint64_t lmask = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
+			uint64_t kv = 0;
+			bool kv_loaded = false;
+
+#pragma unroll
+			for (int t = 0; t < TILES; ++t) {
+				uint64_t c = cand[t];
+				while (c) {
+					const uint32_t j = (uint32_t)__builtin_ctzll(c);
+					c &= c - 1;
+					const uint32_t * qq = &s_q[(t * 64 + j) * 8];
+					const uint4 qa = *reinterpret_cast<const uint4 *>(qq);
+					const uint2 qb = *reinterpret_cast<const uint2 *>(qq + 4);
+					const uint32_t acc = (f0 & qa.x) | (f1 & qa.y) | (f2 & qa.z) |
+										 (f3 & qa.w) | (f4 & qb.x) | (f5 & qb.y);
+					const uint64_t h = __ballot(acc == 0) & lmask;
+					if (h == 0) continue;
+					hit_acc += (uint32_t)__popcll(h);
+					const bool mine = (h >> lane) & 1;
+					if (MODE == 0) {
+						if (!kv_loaded) {
+							kv = a.f.slot_keys[(size_t)b * 64 + lane];
+							kv_loaded = true;
+						}
+						if (mine) {
+							uint32_t * bm = STAGE ? &s_bm[(t * 64 + j) * kw] : &a.gbm[(size_t)qq[6] * kw];
+							add_keys(bm, kv, a.f.ovf);
+						}
+					} else {
+						unsigned long long base = 0;
+						if (lane == 0) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(h));
+						base = __shfl(base, 0);
+						if (mine) {
+							unsigned long long pos = base + mbcnt64(h);
+							if (pos < a.hit_cap) a.hits[pos] = make_uint2(qq[6], b * 64 + lane);
+						}
+					}
+				}
+			}
+		}
+
+		if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
+		__syncthreads();
+		if (STAGE) {
+			for (uint32_t w = tid; w < (uint32_t)NQ * kw; w += WG) {
+				uint32_t v = s_bm[w];
+				if (v) {
+					uint32_t t = w / kw;
+					atomicOr(&a.gbm[(size_t)s_q[t * 8 + 6] * kw + (w - t * kw)], v);
+				}
+			}
+		}
+		if (tid == 0) {
+			if (s_hits) atomicAdd(&a.counters[1], (unsigned long long)s_hits);
+			if (a.log && s_id < a.log_cap) {
+				a.log[s_id * 2] = s_hits;
+				a.log[s_id * 2 + 1] = (uint32_t)(wall_clock64() - t_start);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// ------------------------------------------------------------- finalize
+
+// KEYS_UNIQUE: one wave per query counts its bitmap bits.
+__global__ void __launch_bounds__(256)
+k_bm_count(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t * __restrict__ qcount) {
+	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+	if (qi >= nq) return;
+	uint32_t c = 0;
+	for (uint32_t w = lane; w < kw; w += 64) c += __popc(gbm[(size_t)qi * kw + w]);
+	for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+	if (lane == 0) qcount[qi] = c;
+}
+
+// KEYS_UNIQUE: one wave per query writes its keys in ascending id order,
+// which is ascending key order (the dictionary is sorted), and clears the
+// bitmap for the next submission.
+__global__ void __launch_bounds__(256)
+k_bm_write(uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint64_t * __restrict__ off,
+		   const uint32_t * __restrict__ dict, uint32_t * __restrict__ vals, uint64_t cap) {
+	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+	if (qi >= nq) return;
+	uint64_t pos = off[qi];
+	for (uint32_t wb = 0; wb < kw; wb += 64) {
+		const uint32_t w = wb + lane;
+		uint32_t v = w < kw ? gbm[(size_t)qi * kw + w] : 0;
+		uint32_t c = __popc(v), x = c;
+		for (int o = 1; o < 64; o <<= 1) {
+			uint32_t y = __shfl_up(x, o, 64);
+			if (lane >= (uint32_t)o) x += y;
+		}
+		uint32_t tot = __shfl(x, 63, 64);
+		if (v) {
+			uint64_t p = pos + x - c;
+			gbm[(size_t)qi * kw + w] = 0;
+			while (v) {
+				uint32_t bit = __builtin_ctz(v);
+				v &= v - 1;
+				if (p < cap) vals[p] = dict[w * 32 + bit];
+				++p;
+			}
+		}
+		pos += tot;
+	}
+}
+
+__device__ inline uint32_t slot_key_count(uint64_t kv, const uint32_t * __restrict__ ovf) {
+	if ((kv & 0xFFFFu) == KEY_NONE) return ovf[(uint32_t)(kv >> 16)];
+	uint32_t n = 1;
+	if (((kv >> 16) & 0xFFFFu) != KEY_NONE) ++n; else return n;
+	if (((kv >> 32) & 0xFFFFu) != KEY_NONE) ++n; else return n;
+	if ((kv >> 48) != KEY_NONE) ++n;
+	return n;
+}
+
+__device__ inline uint32_t slot_key_id(uint64_t kv, const uint32_t * __restrict__ ovf, uint32_t i) {
+	if ((kv & 0xFFFFu) == KEY_NONE) return ovf[(uint32_t)(kv >> 16) + 1 + i];
+	return (uint32_t)((kv >> (16 * i)) & 0xFFFFu);
+}
+
+// HITS / KEYS_ALL: values per query.
+template <bool KEYS>
+__global__ void __launch_bounds__(256)
+k_hits_count(const uint2 * __restrict__ hits, const unsigned long long * __restrict__ counters,
+			 uint64_t cap, dev_fib_view f, uint32_t * __restrict__ qcount) {
+	uint64_t n = counters[0];
+	if (n > cap) n = cap;
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		uint2 h = hits[i];
+		uint32_t c = KEYS ? slot_key_count(f.slot_keys[h.y], f.ovf) : 1u;
+		atomicAdd(&qcount[h.x], c);
+	}
+}
+
+template <bool KEYS>
+__global__ void __launch_bounds__(256)
+k_hits_write(const uint2 * __restrict__ hits, const unsigned long long * __restrict__ counters,
+			 uint64_t cap, dev_fib_view f, unsigned long long * __restrict__ qcur,
+			 uint32_t * __restrict__ vals, uint64_t vcap) {
+	uint64_t n = counters[0];
+	if (n > cap) n = cap;
+	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		uint2 h = hits[i];
+		if (KEYS) {
+			uint64_t kv = f.slot_keys[h.y];
+			uint32_t c = slot_key_count(kv, f.ovf);
+			unsigned long long p = atomicAdd(&qcur[h.x], (unsigned long long)c);
+			for (uint32_t k = 0; k < c; ++k)
+				if (p + k < vcap) vals[p + k] = f.key_dict[slot_key_id(kv, f.ovf, k)];
+		} else {
+			unsigned long long p = atomicAdd(&qcur[h.x], 1ULL);
+			if (p < vcap) vals[p] = f.slot_fid[h.y];
+		}
+	}
+}
+
+// Collect the submission's counters in one array for a single D2H copy.
+__global__ void k_gather_stats(const uint64_t * __restrict__ pbase, uint32_t P,
+							   const unsigned long long * __restrict__ counters,
+							   const uint64_t * __restrict__ off, uint32_t nq,
+							   const uint64_t * __restrict__ item_prefix, uint64_t * __restrict__ out) {
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		out[0] = pbase[P];        // (query, partition) pairs
+		out[1] = counters[0];     // hit slots taken
+		out[2] = counters[1];     // hits
+		out[3] = off[nq];         // result values
+		out[4] = item_prefix[P];  // work items
+	}
+}
+
+// ------------------------------------------------------------- kernel table
+
+using match_fn = void (*)(match_args);
+
+template <int MODE, bool STAGE>
+match_fn pick_tiles(int tiles) {
+	switch (tiles) {
+	case 1: return k_match<MODE, STAGE, 1>;
+	case 2: return k_match<MODE, STAGE, 2>;
+	default: return k_match<MODE, STAGE, 4>;
+	}
+}
+
+match_fn pick_match(int mode, bool stage, int tiles) {
+	if (mode == 0) return stage ? pick_tiles<0, true>(tiles) : pick_tiles<0, false>(tiles);
+	return pick_tiles<1, false>(tiles);
+}
+
+} // namespace
+
+// ------------------------------------------------------------- ring
+
+struct tm_ring {
+	tm_fib * fib = nullptr;
+	tm_match_config cfg;
+	uint32_t max_q = 0;
+	uint32_t P = 0, kw = 0;
+	hipStream_t s = nullptr;
+	hipEvent_t e0 = nullptr, em0 = nullptr, em1 = nullptr, e1 = nullptr;
+
+	uint32_t * h_q = nullptr;      // pinned staging, max_q x 8
+	uint32_t * d_q = nullptr;      // max_q x 8
+	uint32_t * d_pcount = nullptr; // P
+	uint64_t * d_pbase = nullptr;  // P + 1
+	unsigned long long * d_pcur = nullptr;
+	uint32_t * d_list = nullptr;
+	uint64_t list_cap = 0;
+	uint4 * d_sched = nullptr;
+	uint64_t * d_iprefix = nullptr;
+	uint64_t * d_qbeg = nullptr;
+	unsigned long long * d_qnext = nullptr;
+	uint32_t * d_porder = nullptr;
+	bool have_order = false;
+	uint32_t * d_gbm = nullptr;
+	uint2 * d_hits = nullptr;
+	uint64_t hit_cap = 0;
+	uint32_t * d_qcount = nullptr;
+	uint64_t * d_off = nullptr;
+	unsigned long long * d_qcur = nullptr;
+	uint32_t * d_vals = nullptr;
+	uint64_t val_cap = 0;
+	unsigned long long * d_cnt = nullptr;
+	uint64_t * d_scan = nullptr;
+	size_t scan_words = 0;
+	uint64_t * d_stat = nullptr;
+	uint64_t * h_stat = nullptr;   // pinned, 8
+	uint32_t * d_log = nullptr;
+	uint64_t log_cap = 0;
+
+	// effective launch parameters
+	int tiles = 4, wgs_per_cu = 2;
+	uint32_t chunk = 256;
+	bool stage = false;
+	size_t stage_lds = 0;
+
+	// graph replay
+	hipGraphExec_t gexec = nullptr;
+	uint32_t graph_nq = 0;
+	bool graph_ok = false;
+
+	uint32_t last_nq = 0;
+	bool pending = false;
+	bool match_events = false;
+	int retries = 0;
+	tm_status st;
+
+	void release();
+};
+
+void tm_ring::release() {
+	if (gexec) (void)hipGraphExecDestroy(gexec);
+	gexec = nullptr;
+	graph_ok = false;
+	if (h_q) (void)hipHostFree(h_q);
+	if (h_stat) (void)hipHostFree(h_stat);
+	h_q = nullptr;
+	h_stat = nullptr;
+	dfree(d_q); dfree(d_pcount); dfree(d_pbase); dfree(d_pcur); dfree(d_list);
+	dfree(d_sched); dfree(d_iprefix); dfree(d_qbeg); dfree(d_qnext); dfree(d_porder);
+	dfree(d_gbm); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
+	dfree(d_vals); dfree(d_cnt); dfree(d_scan); dfree(d_stat); dfree(d_log);
+	if (e0) (void)hipEventDestroy(e0);
+	if (em0) (void)hipEventDestroy(em0);
+	if (em1) (void)hipEventDestroy(em1);
+	if (e1) (void)hipEventDestroy(e1);
+	e0 = em0 = em1 = e1 = nullptr;
+	if (s) (void)hipStreamDestroy(s);
+	s = nullptr;
+}
+
+namespace {
+
+int mode_of(const tm_match_config & c) { return c.mode == TM_MODE_KEYS_UNIQUE ? 0 : 1; }
+
+// Effective launch parameters from the config and the FIB.
+int resolve_params(tm_ring * r) {
+	const tm_match_config & c = r->cfg;
+	if (c.mode > TM_MODE_HITS) return set_error(TM_ERR_INVALID, "unknown mode %u", c.mode);
+	if (c.schedule > TM_SCHED_PERSISTENT) return set_error(TM_ERR_INVALID, "unknown schedule %u", c.schedule);
+	if (c.mode == TM_MODE_KEYS_UNIQUE && r->fib->n_keys > 65536)
+		return set_error(TM_ERR_INVALID,
+						 "KEYS_UNIQUE supports up to 65536 distinct keys (FIB has %u); use KEYS_ALL",
+						 r->fib->n_keys);
+	if (c.schedule == TM_SCHED_PLAIN) {
+		uint32_t bs = c.batch_size ? c.batch_size : 256;
+		if (bs != 64 && bs != 128 && bs != 256)
+			return set_error(TM_ERR_INVALID, "batch_size must be 64, 128 or 256 (got %u)", bs);
+		r->tiles = (int)(bs / 64);
+		r->chunk = WAVES;            // one block per wave, like one filter per thread
+		r->stage = false;
+	} else {
+		uint32_t bs = c.batch_size ? c.batch_size : 256;
+		if (bs != 64 && bs != 128 && bs != 256)
+			return set_error(TM_ERR_INVALID, "batch_size must be 64, 128 or 256 (got %u)", bs);
+		r->tiles = (int)(bs / 64);
+		r->chunk = c.chunk_blocks ? c.chunk_blocks : 256;
+		size_t lds = (size_t)r->tiles * 64 * r->kw * 4;
+		size_t static_lds = (size_t)r->tiles * 64 * 8 * 4 + 64;
+		r->stage = (mode_of(c) == 0) && lds <= STAGE_LDS_MAX && lds + static_lds <= (size_t)dev().max_lds;
+		r->stage_lds = r->stage ? lds : 0;
+		int by_lds = (int)(160 * 1024 / (r->stage_lds + static_lds));
+		int want = c.groups_per_cu ? (int)c.groups_per_cu : 4;
+		r->wgs_per_cu = std::max(1, std::min(want, by_lds));
+	}
+	return TM_OK;
+}
+
+int grow_u32(uint32_t ** p, uint64_t & cap, uint64_t need) {
+	if (need <= cap && *p) return TM_OK;
+	dfree(*p);
+	uint64_t n = std::max<uint64_t>(need, 1);
+	TM_CHECK(dalloc(p, n));
+	cap = n;
+	return TM_OK;
+}
+
+// Enqueue lookup + item building (shared by both schedules).
+void enqueue_front(tm_ring * r, uint32_t nq) {
+	hipStream_t s = r->s;
+	const tm_fib * f = r->fib;
+	const uint32_t P = r->P;
+	fill_u32(r->d_pcount, 0, P, s);
+	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 4, s);
+	uint32_t g = (nq + 255) / 256;
+	if (g == 0) g = 1;
+	hipLaunchKernelGGL(k_lookup_count, dim3(g), dim3(256), 0, s, r->d_q, nq, f->part_info, P, r->d_pcount);
+	scan_u32_to_u64(r->d_pcount, r->d_pbase, P, r->d_scan, s);
+	copy_u64(reinterpret_cast<uint64_t *>(r->d_pcur), r->d_pbase, P, s);
+	hipLaunchKernelGGL(k_lookup_fill, dim3(g), dim3(256), 0, s, r->d_q, nq, f->part_info, P, r->d_pcur,
+					   r->d_list, r->list_cap);
+	hipLaunchKernelGGL(k_build_items, dim3(1), dim3(1024), 0, s, r->d_pcount, r->d_pbase, r->list_cap,
+					   f->part_info, P, r->have_order ? r->d_porder : (const uint32_t *)nullptr,
+					   (uint32_t)(64 * r->tiles), r->chunk, r->d_sched, r->d_iprefix, r->d_qbeg, r->d_qnext);
+}
+
+match_args base_args(tm_ring * r) {
+	match_args a;
+	std::memset(&a, 0, sizeof(a));
+	a.f = r->fib->view();
+	a.q = r->d_q;
+	a.list = r->d_list;
+	a.pbase = r->d_pbase;
+	a.sched = r->d_sched;
+	a.item_prefix = r->d_iprefix;
+	a.n_ranks = r->P;
+	a.chunk = r->chunk;
+	a.qnext = r->d_qnext;
+	a.qbeg = r->d_qbeg;
+	a.plain_rank = -1;
+	a.gbm = r->d_gbm;
+	a.hits = r->d_hits;
+	a.hit_cap = r->hit_cap;
+	a.counters = r->d_cnt;
+	a.log = r->cfg.log_units ? r->d_log : nullptr;
+	a.log_cap = r->log_cap;
+	return a;
+}
+
+void enqueue_finalize(tm_ring * r, uint32_t nq) {
+	hipStream_t s = r->s;
+	const int mode = r->cfg.mode;
+	uint32_t gq = (nq + 3) / 4;
+	if (gq == 0) gq = 1;
+	if (mode == TM_MODE_KEYS_UNIQUE) {
+		hipLaunchKernelGGL(k_bm_count, dim3(gq), dim3(256), 0, s, r->d_gbm, r->kw, nq, r->d_qcount);
+		scan_u32_to_u64(r->d_qcount, r->d_off, nq, r->d_scan, s);
+		hipLaunchKernelGGL(k_bm_write, dim3(gq), dim3(256), 0, s, r->d_gbm, r->kw, nq, r->d_off,
+						   r->fib->key_dict, r->d_vals, r->val_cap);
+	} else {
+		const bool keys = mode == TM_MODE_KEYS_ALL;
+		dev_fib_view v = r->fib->view();
+		fill_u32(r->d_qcount, 0, nq, s);
+		uint32_t gh = 2048;
+		if (keys)
+			hipLaunchKernelGGL(k_hits_count<true>, dim3(gh), dim3(256), 0, s, r->d_hits, r->d_cnt, r->hit_cap, v, r->d_qcount);
+		else
+			hipLaunchKernelGGL(k_hits_count<false>, dim3(gh), dim3(256), 0, s, r->d_hits, r->d_cnt, r->hit_cap, v, r->d_qcount);
+		scan_u32_to_u64(r->d_qcount, r->d_off, nq, r->d_scan, s);
+		copy_u64(reinterpret_cast<uint64_t *>(r->d_qcur), r->d_off, nq, s);
+		if (keys)
+			hipLaunchKernelGGL(k_hits_write<true>, dim3(gh), dim3(256), 0, s, r->d_hits, r->d_cnt, r->hit_cap, v,
+							   r->d_qcur, r->d_vals, r->val_cap);
+		else
+			hipLaunchKernelGGL(k_hits_write<false>, dim3(gh), dim3(256), 0, s, r->d_hits, r->d_cnt, r->hit_cap, v,
+							   r->d_qcur, r->d_vals, r->val_cap);
+	}
+	hipLaunchKernelGGL(k_gather_stats, dim3(1), dim3(64), 0, s, r->d_pbase, r->P, r->d_cnt, r->d_off, nq,
+					   r->d_iprefix, r->d_stat);
+}
+
+// The persistent step: everything between the query upload and the stats.
+int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
+	enqueue_front(r, nq);
+	match_args a = base_args(r);
+	match_fn fn = pick_match(mode_of(r->cfg), r->stage, r->tiles);
+	uint32_t grid = (uint32_t)(dev().num_cu * r->wgs_per_cu);
+	if (events) TM_HIP(hipEventRecord(r->em0, r->s));
+	hipLaunchKernelGGL(fn, dim3(grid), dim3(WG), r->stage_lds, r->s, a);
+	TM_HIP(hipGetLastError());
+	if (events) TM_HIP(hipEventRecord(r->em1, r->s));
+	enqueue_finalize(r, nq);
+	return TM_OK;
+}
+
+// The plain step: one launch per (partition, batch), as the reference
+// launches one kernel per (partition, batch) (back_end.cc:284-327).
+int enqueue_plain(tm_ring * r, uint32_t nq) {
+	enqueue_front(r, nq);
+	std::vector<uint4> sched(r->P);
+	if (r->P) {
+		TM_HIP(hipMemcpyAsync(sched.data(), r->d_sched, sizeof(uint4) * r->P, hipMemcpyDeviceToHost, r->s));
+		TM_HIP(hipStreamSynchronize(r->s));
+	}
+	match_args a = base_args(r);
+	match_fn fn = pick_match(mode_of(r->cfg), false, r->tiles);
+	uint32_t launches = 0;
+	TM_HIP(hipEventRecord(r->em0, r->s));
+	for (uint32_t k = 0; k < r->P; ++k) {
+		const uint4 sc = sched[k];   // partition, batches, chunks, queries
+		if (sc.z == 0) continue;
+		for (uint32_t b = 0; b < sc.y; ++b) {
+			a.plain_rank = (int32_t)k;
+			a.plain_batch = b;
+			hipLaunchKernelGGL(fn, dim3(sc.z), dim3(WG), 0, r->s, a);
+			++launches;
+		}
+	}
+	TM_HIP(hipGetLastError());
+	TM_HIP(hipEventRecord(r->em1, r->s));
+	r->st.n_items = launches;
+	enqueue_finalize(r, nq);
+	return TM_OK;
+}
+
+int ensure_capacity(tm_ring * r) {
+	const uint32_t P = r->P;
+	if (!r->d_pcount) {
+		TM_CHECK(dalloc(&r->d_pcount, P + 1));
+		TM_CHECK(dalloc(&r->d_pbase, P + 1));
+		TM_CHECK(dalloc(&r->d_pcur, P + 1));
+		TM_CHECK(dalloc(&r->d_sched, P + 1));
+		TM_CHECK(dalloc(&r->d_iprefix, P + 1));
+		TM_CHECK(dalloc(&r->d_qbeg, NQUEUE + 1));
+		TM_CHECK(dalloc(&r->d_qnext, NQUEUE));
+		TM_CHECK(dalloc(&r->d_porder, P + 1));
+		TM_CHECK(dalloc(&r->d_cnt, 4));
+		TM_CHECK(dalloc(&r->d_stat, 8));
+		TM_CHECK(dalloc(&r->d_qcount, (size_t)r->max_q + 1));
+		TM_CHECK(dalloc(&r->d_off, (size_t)r->max_q + 1));
+		TM_CHECK(dalloc(&r->d_qcur, (size_t)r->max_q + 1));
+		r->scan_words = std::max(scan_tmp_words(r->max_q), scan_tmp_words(P));
+		TM_CHECK(dalloc(&r->d_scan, r->scan_words));
+	}
+	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->d_gbm) {
+		size_t n = (size_t)r->max_q * r->kw;
+		TM_CHECK(dalloc(&r->d_gbm, n));
+		TM_HIP(hipMemset(r->d_gbm, 0, std::max<size_t>(n, 1) * 4));
+	}
+	if (r->cfg.mode != TM_MODE_KEYS_UNIQUE && !r->d_hits) {
+		uint64_t cap = r->cfg.hit_capacity ? r->cfg.hit_capacity : std::max<uint64_t>(1 << 20, (uint64_t)r->max_q * 64);
+		TM_CHECK(dalloc(&r->d_hits, cap));
+		r->hit_cap = cap;
+	}
+	if (!r->d_list) {
+		uint64_t cap = (uint64_t)r->max_q * std::min<uint32_t>(std::max<uint32_t>(P, 1), 64);
+		TM_CHECK(grow_u32(&r->d_list, r->list_cap, cap));
+	}
+	if (!r->d_vals) {
+		uint64_t per = r->cfg.mode == TM_MODE_KEYS_UNIQUE ? std::min<uint32_t>(r->fib->n_keys, 256) : 64;
+		TM_CHECK(grow_u32(&r->d_vals, r->val_cap, (uint64_t)r->max_q * std::max<uint64_t>(per, 1)));
+	}
+	if (r->cfg.log_units && !r->d_log) {
+		r->log_cap = 1 << 22;
+		TM_CHECK(dalloc(&r->d_log, r->log_cap * 2));
+	}
+	return TM_OK;
+}
+
+int enqueue_step(tm_ring * r, uint32_t nq, bool allow_graph) {
+	TM_HIP(hipEventRecord(r->e0, r->s));
+	const bool persistent = r->cfg.schedule == TM_SCHED_PERSISTENT;
+	if (persistent && allow_graph && r->cfg.use_graph) {
+		if (!r->graph_ok || r->graph_nq != nq) {
+			if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+			r->gexec = nullptr;
+			r->graph_ok = false;
+			hipGraph_t g = nullptr;
+			TM_HIP(hipStreamBeginCapture(r->s, hipStreamCaptureModeThreadLocal));
+			int rc = enqueue_persistent(r, nq, false);
+			hipError_t ce = hipStreamEndCapture(r->s, &g);
+			if (rc != TM_OK) {
+				if (g) (void)hipGraphDestroy(g);
+				return rc;
+			}
+			TM_HIP(ce);
+			hipError_t ie = hipGraphInstantiate(&r->gexec, g, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(g);
+			TM_HIP(ie);
+			r->graph_ok = true;
+			r->graph_nq = nq;
+		}
+		TM_HIP(hipGraphLaunch(r->gexec, r->s));
+		r->match_events = false;
+	} else if (persistent) {
+		TM_CHECK(enqueue_persistent(r, nq, true));
+		r->match_events = true;
+	} else {
+		TM_CHECK(enqueue_plain(r, nq));
+		r->match_events = true;
+	}
+	TM_HIP(hipMemcpyAsync(r->h_stat, r->d_stat, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, r->s));
+	TM_HIP(hipEventRecord(r->e1, r->s));
+	return TM_OK;
+}
+
+int submit_common(tm_ring * r, uint32_t nq) {
+	r->st = tm_status();
+	r->last_nq = nq;
+	r->retries = 0;
+	TM_CHECK(ensure_capacity(r));
+	int rc = enqueue_step(r, nq, true);
+	if (rc != TM_OK) return rc;
+	r->pending = true;
+	return TM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void tm_match_config_default(tm_match_config * c) {
+	std::memset(c, 0, sizeof(*c));
+	c->mode = TM_MODE_KEYS_UNIQUE;
+	c->schedule = TM_SCHED_PERSISTENT;
+	c->batch_size = 256;
+}
+
+int tm_ring_create(tm_fib * fib, uint32_t max_queries, const tm_match_config * cfg, tm_ring ** out) {
+	if (!fib || !out) return set_error(TM_ERR_INVALID, "NULL argument");
+	if (max_queries == 0) return set_error(TM_ERR_INVALID, "max_queries must be > 0");
+	TM_CHECK(require_device());
+	tm_ring * r = new tm_ring();
+	r->fib = fib;
+	if (cfg) r->cfg = *cfg; else tm_match_config_default(&r->cfg);
+	r->max_q = max_queries;
+	r->P = fib->n_parts;
+	r->kw = (fib->n_keys + 31) / 32;
+	int rc = resolve_params(r);
+	if (rc != TM_OK) { delete r; return rc; }
+	hipError_t e = hipStreamCreateWithFlags(&r->s, hipStreamNonBlocking);
+	if (e == hipSuccess) e = hipEventCreate(&r->e0);
+	if (e == hipSuccess) e = hipEventCreate(&r->em0);
+	if (e == hipSuccess) e = hipEventCreate(&r->em1);
+	if (e == hipSuccess) e = hipEventCreate(&r->e1);
+	if (e == hipSuccess) e = hipHostMalloc((void **)&r->h_q, (size_t)max_queries * 32);
+	if (e == hipSuccess) e = hipHostMalloc((void **)&r->h_stat, 8 * sizeof(uint64_t));
+	if (e != hipSuccess) {
+		int code = hip_error(e, "ring setup", __FILE__, __LINE__);
+		r->release();
+		delete r;
+		return code;
+	}
+	rc = dalloc(&r->d_q, (size_t)max_queries * 8);
+	if (rc == TM_OK) rc = ensure_capacity(r);
+	if (rc != TM_OK) { r->release(); delete r; return rc; }
+	*out = r;
+	return TM_OK;
+}
+
+int tm_ring_destroy(tm_ring * r) {
+	if (!r) return TM_OK;
+	if (dev().device >= 0) (void)hipSetDevice(dev().device);
+	if (r->s) (void)hipStreamSynchronize(r->s);
+	r->release();
+	delete r;
+	return TM_OK;
+}
+
+int tm_ring_config(tm_ring * r, const tm_match_config * cfg) {
+	if (!r || !cfg) return set_error(TM_ERR_INVALID, "NULL argument");
+	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
+	TM_CHECK(require_device());
+	tm_match_config old = r->cfg;
+	r->cfg = *cfg;
+	int rc = resolve_params(r);
+	if (rc != TM_OK) { r->cfg = old; (void)resolve_params(r); return rc; }
+	if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+	r->gexec = nullptr;
+	r->graph_ok = false;
+	return ensure_capacity(r);
+}
+
+int tm_ring_query_buffer(tm_ring * r, uint32_t ** dev_words) {
+	if (!r || !dev_words) return set_error(TM_ERR_INVALID, "NULL argument");
+	*dev_words = r->d_q;
+	return TM_OK;
+}
+
+int tm_submit(tm_ring * r, const uint32_t * words, uint32_t nq) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (nq > r->max_q) return set_error(TM_ERR_INVALID, "nq %u exceeds ring capacity %u", nq, r->max_q);
+	if (nq && !words) return set_error(TM_ERR_INVALID, "query words are NULL");
+	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
+	TM_CHECK(require_device());
+	for (uint32_t i = 0; i < nq; ++i) {
+		std::memcpy(r->h_q + (size_t)i * 8, words + (size_t)i * 6, 24);
+		r->h_q[(size_t)i * 8 + 6] = 0;
+		r->h_q[(size_t)i * 8 + 7] = 0;
+	}
+	if (nq) TM_HIP(hipMemcpyAsync(r->d_q, r->h_q, (size_t)nq * 32, hipMemcpyHostToDevice, r->s));
+	return submit_common(r, nq);
+}
+
+int tm_submit_resident(tm_ring * r, uint32_t nq) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (nq > r->max_q) return set_error(TM_ERR_INVALID, "nq %u exceeds ring capacity %u", nq, r->max_q);
+	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
+	TM_CHECK(require_device());
+	return submit_common(r, nq);
+}
+
+int tm_poll(tm_ring * r) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (!r->pending) return TM_OK;
+	hipError_t e = hipEventQuery(r->e1);
+	if (e == hipErrorNotReady) return TM_ERR_BUSY;
+	TM_HIP(e);
+	return TM_OK;
+}
+
+int tm_wait(tm_ring * r, tm_status * st) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	TM_CHECK(require_device());
+	if (r->pending) {
+		for (;;) {
+			TM_HIP(hipEventSynchronize(r->e1));
+			const uint64_t pairs = r->h_stat[0], slots = r->h_stat[1], results = r->h_stat[3];
+			bool again = false;
+			if (pairs > r->list_cap) {
+				TM_CHECK(grow_u32(&r->d_list, r->list_cap, pairs + pairs / 4 + 1024));
+				again = true;
+			}
+			if (r->cfg.mode != TM_MODE_KEYS_UNIQUE && slots > r->hit_cap) {
+				dfree(r->d_hits);
+				uint64_t cap = slots + slots / 4 + 1024;
+				TM_CHECK(dalloc(&r->d_hits, cap));
+				r->hit_cap = cap;
+				again = true;
+			}
+			if (results > r->val_cap) {
+				TM_CHECK(grow_u32(&r->d_vals, r->val_cap, results + results / 4 + 1024));
+				again = true;
+			}
+			if (!again) break;
+			if (++r->retries > 4) {
+				r->pending = false;
+				return set_error(TM_ERR_OVERFLOW, "capacity retry limit reached");
+			}
+			// overflow-retry: capacities changed, so drop any captured graph
+			if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+			r->gexec = nullptr;
+			r->graph_ok = false;
+			TM_CHECK(enqueue_step(r, r->last_nq, false));
+		}
+		r->pending = false;
+		float dms = 0.f, mms = 0.f;
+		(void)hipEventElapsedTime(&dms, r->e0, r->e1);
+		if (r->match_events) (void)hipEventElapsedTime(&mms, r->em0, r->em1);
+		r->st.n_queries = r->last_nq;
+		r->st.n_pairs = r->h_stat[0];
+		r->st.n_hits = r->h_stat[2];
+		r->st.n_results = r->h_stat[3];
+		if (r->cfg.schedule == TM_SCHED_PERSISTENT) r->st.n_items = (uint32_t)r->h_stat[4];
+		r->st.hit_capacity = r->hit_cap;
+		r->st.overflow = r->retries > 0;
+		r->st.error = TM_OK;
+		r->st.device_ms = dms;
+		r->st.match_ms = mms;
+	}
+	if (st) *st = r->st;
+	return TM_OK;
+}
+
+int tm_read_results(tm_ring * r, uint64_t * offsets, uint32_t * values, uint64_t capacity) {
+	if (!r || !offsets) return set_error(TM_ERR_INVALID, "NULL argument");
+	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
+	TM_CHECK(require_device());
+	const uint32_t nq = r->last_nq;
+	if (nq == 0) {
+		offsets[0] = 0;
+		return TM_OK;
+	}
+	TM_HIP(hipMemcpy(offsets, r->d_off, ((size_t)nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+	uint64_t total = offsets[nq];
+	if (!values) return TM_OK;
+	if (total > capacity)
+		return set_error(TM_ERR_OVERFLOW, "%llu result values, capacity %llu", (unsigned long long)total,
+						 (unsigned long long)capacity);
+	if (total) TM_HIP(hipMemcpy(values, r->d_vals, total * sizeof(uint32_t), hipMemcpyDeviceToHost));
+	return TM_OK;
+}
+
+int tm_ring_results_device(tm_ring * r, const uint64_t ** off, const uint32_t ** vals) {
+	if (!r || !off || !vals) return set_error(TM_ERR_INVALID, "NULL argument");
+	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
+	*off = r->d_off;
+	*vals = r->d_vals;
+	return TM_OK;
+}
+
+int tm_ring_lookup_counts(tm_ring * r, uint32_t * counts, uint32_t n_parts) {
+	if (!r || !counts) return set_error(TM_ERR_INVALID, "NULL argument");
+	if (n_parts != r->P) return set_error(TM_ERR_INVALID, "n_parts %u != %u", n_parts, r->P);
+	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
+	TM_CHECK(require_device());
+	if (n_parts) TM_HIP(hipMemcpy(counts, r->d_pcount, (size_t)n_parts * 4, hipMemcpyDeviceToHost));
+	return TM_OK;
+}
+
+int tm_match(tm_fib * fib, const uint32_t * words, uint32_t nq, const tm_match_config * cfg,
+			 uint64_t * offsets, uint32_t * values, uint64_t capacity, tm_status * st) {
+	tm_ring * r = nullptr;
+	TM_CHECK(tm_ring_create(fib, nq ? nq : 1, cfg, &r));
+	int rc = tm_submit(r, words, nq);
+	if (rc == TM_OK) rc = tm_wait(r, st);
+	if (rc == TM_OK) rc = tm_read_results(r, offsets, values, capacity);
+	tm_ring_destroy(r);
+	return rc;
+}
+
+} // extern "C"
