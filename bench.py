@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--schedule", choices=["persistent", "plain"], default="persistent")
     ap.add_argument("--graph", action="store_true", help="replay steps from a hipGraph")
     ap.add_argument("--plain-steps", type=int, default=1, help="steps of the plain single-launch form")
+    ap.add_argument("--plain-queries", type=int, default=65536,
+                    help="queries of the plain form's step (a prefix of the resident queries)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--seed", type=lambda x: int(x, 0), default=0x7461676D61746302)
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
@@ -136,6 +138,8 @@ def main():
         step_ms.append((time.perf_counter() - ts) * 1e3)
         match_ms.append(st.match_ms)
         stats.append(st.as_dict())
+        log(f"step {i}: {step_ms[-1]:.1f} ms (match kernel {st.match_ms:.1f} ms, device {st.device_ms:.1f} ms, "
+            f"{st.n_items} items, {st.n_hits} hits)")
     tm.check(tm.load().tm_dev_synchronize(), "sync")
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
@@ -157,14 +161,16 @@ def main():
     plain = None
     if args.plain_steps > 0 and sched == tm.SCHED_PERSISTENT:
         ring.configure(tm.config(mode=tm.MODE_KEYS_UNIQUE, schedule=tm.SCHED_PLAIN, batch_size=args.batch))
+        npq = min(nq, args.plain_queries)
         tt = []
         for i in range(args.plain_steps):
             ts = time.perf_counter()
-            ring.submit_resident(nq)
+            ring.submit_resident(npq)
             pst = ring.wait()
             tt.append(time.perf_counter() - ts)
-        plain = {"queries_per_s": nq / np.mean(tt), "ms_per_step": 1e3 * np.mean(tt), "launches": pst.n_items,
-                 "match_ms": pst.match_ms}
+            log(f"plain step {i}: {npq} queries in {tt[-1] * 1e3:.1f} ms, {pst.n_items} launches")
+        plain = {"queries_per_s": npq / np.mean(tt), "ms_per_step": 1e3 * np.mean(tt), "launches": pst.n_items,
+                 "queries": npq, "match_ms": pst.match_ms}
     ring.close()
 
     cpu = None

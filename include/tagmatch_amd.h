@@ -166,9 +166,14 @@ typedef struct tm_match_config {
 	uint32_t use_graph;      /* PERSISTENT: replay steady-state steps from a
 	                            captured hipGraph                            */
 	uint32_t log_units;      /* record per-work-item hit counts and cycles   */
-	uint32_t order_by_cost;  /* PERSISTENT: run work items in descending
-	                            predicted cost (see tm_ring_set_costs)       */
-	uint32_t reserved[7];
+	uint32_t order_by_cost;  /* PERSISTENT: visit partitions in the order set
+	                            by tm_ring_set_order                          */
+	uint32_t variant;        /* PERSISTENT inner loop, bit 0: candidate
+	                            query words from scalar loads, four tests per
+	                            iteration (else LDS, two); bit 1: hits go
+	                            through a per-wave key log drained 64 lanes
+	                            at a time; other values: default (2)         */
+	uint32_t reserved[6];
 } tm_match_config;
 
 void tm_match_config_default(tm_match_config * cfg);
@@ -184,6 +189,11 @@ typedef struct tm_status {
 	int32_t  error;          /* TM_* status of the submission               */
 	double   device_ms;      /* stream time from first to last kernel       */
 	double   match_ms;       /* time of the match kernel(s) alone           */
+	uint64_t n_block_visits; /* PERSISTENT: (query group, block) visits with
+	                            at least one candidate query                */
+	uint64_t n_cand_tests;   /* PERSISTENT: 64-filter subset tests run
+	                            (one per candidate query per block)          */
+	uint64_t n_hit_events;   /* PERSISTENT: subset tests with >= 1 hit       */
 } tm_status;
 
 /* Create a ring for up to max_queries queries per submission. */

@@ -51,7 +51,8 @@ class MatchConfig(C.Structure):
         ("mode", C.c_uint32), ("schedule", C.c_uint32), ("batch_size", C.c_uint32),
         ("chunk_blocks", C.c_uint32), ("waves_per_group", C.c_uint32),
         ("groups_per_cu", C.c_uint32), ("hit_capacity", C.c_uint64), ("use_graph", C.c_uint32),
-        ("log_units", C.c_uint32), ("order_by_cost", C.c_uint32), ("reserved", C.c_uint32 * 7),
+        ("log_units", C.c_uint32), ("order_by_cost", C.c_uint32), ("variant", C.c_uint32),
+        ("reserved", C.c_uint32 * 6),
     ]
 
 
@@ -60,7 +61,8 @@ class Status(C.Structure):
         ("n_queries", C.c_uint32), ("n_items", C.c_uint32), ("n_pairs", C.c_uint64),
         ("n_hits", C.c_uint64), ("n_results", C.c_uint64), ("hit_capacity", C.c_uint64),
         ("overflow", C.c_int32), ("error", C.c_int32), ("device_ms", C.c_double),
-        ("match_ms", C.c_double),
+        ("match_ms", C.c_double), ("n_block_visits", C.c_uint64), ("n_cand_tests", C.c_uint64),
+        ("n_hit_events", C.c_uint64),
     ]
 
     def as_dict(self):

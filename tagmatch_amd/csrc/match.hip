@@ -376,6 +376,372 @@ k_match(match_args a) {
 	}
 }
 
+// ------------------------------------------------------------- query-major match
+
+// Persistent schedule.  A work item is a group of NQ = 64*TILES consecutive
+// queries of the submission, owned by one workgroup for the whole match:
+//
+//   for each partition p (schedule order):
+//     lookup, fused: qsel = queries of the group whose bits cover p's mask
+//       (reference front_end.cc:465-477), one 64-lane test per tile;
+//     for each 64-filter block of p (blocks split over the waves):
+//       candidate pass: cand = qsel & (block AND-mask covered by query);
+//       subset test of each candidate query against the 64 filters, two
+//       candidates per iteration;
+//       hits: OR the filters' dense key ids into the group's key bitmaps in
+//       LDS (KEYS_UNIQUE), or stage (query, slot) pairs per wave (HITS).
+//
+// Because a group's results never leave its workgroup until the end, the
+// bitmaps are written out once with plain stores: no global atomics on
+// results.  The reference instead merges partial results on the host under
+// a per-query mutex (back_end.cc:329-343, tagmatch_query.hh:80-93).
+
+struct mq_args {
+	dev_fib_view f;
+	const uint32_t * __restrict__ q;          // nq x 8 words
+	const uint32_t * __restrict__ qinv;       // nq x 8 inverted words (VAR 1)
+	uint32_t nq;
+	const uint32_t * __restrict__ porder;     // partition schedule order (NULL: 0..P-1)
+	unsigned long long * qnext;               // NQUEUE queue cursors
+	const uint64_t * __restrict__ qbeg;       // NQUEUE + 1 queue bounds (items)
+	uint32_t * pcount;                        // per partition: queries selecting it
+	uint32_t * gbm;                           // KEYS_UNIQUE: nq x kw bitmap words
+	uint2 * hits;                             // HITS: (query, block slot)
+	uint64_t hit_cap;
+	unsigned long long * counters;            // [0] hit slots, [1] hits, [2] block visits,
+	                                          // [3] subset tests, [4] tests with hits
+	uint32_t * log;                           // per item: hits, 10 ns ticks
+	uint64_t log_cap;
+};
+
+constexpr int HBUF = 256;                     // staged hits per wave (HITS mode)
+
+// Dense key ids of a hit filter into a bitmap row; `kv` inline form has up to
+// four 16-bit ids, KEY_NONE for an unused field (ORs 0 into word 0).
+template <bool LDS_BM>
+__device__ inline void or_keys(uint32_t * row, uint64_t kv, const uint32_t * __restrict__ ovf) {
+	if ((kv & 0xFFFFu) != KEY_NONE) {
+#pragma unroll
+		for (int i = 0; i < 4; ++i) {
+			const uint32_t id = (uint32_t)(kv >> (16 * i)) & 0xFFFFu;
+			const bool ok = id != KEY_NONE;
+			const uint32_t w = ok ? (id >> 5) : 0u, bit = ok ? (1u << (id & 31)) : 0u;
+			if (LDS_BM) atomicOr(&row[w], bit);
+			else __hip_atomic_fetch_or(&row[w], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		}
+	} else {
+		const uint32_t off = (uint32_t)(kv >> 16), n = ovf[off];
+		for (uint32_t i = 0; i < n; ++i) {
+			const uint32_t id = ovf[off + 1 + i];
+			if (LDS_BM) atomicOr(&row[id >> 5], 1u << (id & 31));
+			else __hip_atomic_fetch_or(&row[id >> 5], 1u << (id & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		}
+	}
+}
+
+// Uniform loads through the constant address space compile to scalar
+// loads (s_load) into SGPRs; used for data that no thread writes during the
+// kernel (FIB block info, partition info, inverted query words).
+typedef const __attribute__((address_space(4))) uint32_t cu32_t;
+
+__device__ inline uint4 sld4(const uint32_t * p) {
+	cu32_t * c = (cu32_t *)(p);
+	return make_uint4(c[0], c[1], c[2], c[3]);
+}
+__device__ inline uint2 sld2(const uint32_t * p) {
+	cu32_t * c = (cu32_t *)(p);
+	return make_uint2(c[0], c[1]);
+}
+__device__ inline uint32_t sld1(const uint32_t * p) { return *(cu32_t *)(p); }
+
+// Per-wave hit sink of the query-major kernel.
+//
+// KEYS_UNIQUE with KLOG: a hit appends (bitmap row, packed key ids) to this
+// wave's 64-entry log in LDS with one plain store per hitting lane; a full
+// log is drained with one entry per lane, so each LDS OR instruction carries
+// up to 64 lanes instead of the two or three lanes of a single test.
+template <int MODE, bool LDS_BM, bool KLOG>
+struct hit_sink {
+	uint32_t * bm_base;       // MODE 0: bitmap rows of the group's queries
+	uint32_t kw;
+	uint2 * hb;               // MODE 1: this wave's staging buffer
+	uint4 * klog;             // KLOG: this wave's key log (64 entries)
+	uint32_t hcnt;
+	const uint32_t * ovf;
+	unsigned long long * counter;
+	uint2 * hits;
+	uint64_t cap;
+
+	__device__ inline void drain(uint32_t lane) {
+		if (lane < hcnt) {
+			const uint4 e = klog[lane];
+			or_keys<LDS_BM>(bm_base + (size_t)e.x * kw, (uint64_t)e.y | ((uint64_t)e.z << 32), ovf);
+		}
+		hcnt = 0;
+	}
+
+	__device__ inline void flush(uint32_t lane) {
+		unsigned long long base = 0;
+		if (lane == 0) base = atomicAdd(counter, (unsigned long long)hcnt);
+		base = __shfl(base, 0);
+		for (uint32_t i = lane; i < hcnt; i += 64)
+			if (base + i < cap) hits[base + i] = hb[i];
+		hcnt = 0;
+	}
+
+	// h: lanes (filters of block b) matched by group query jq (index qi)
+	__device__ inline void add(uint64_t h, uint32_t jq, uint32_t qi, uint32_t b, uint64_t kv, uint32_t lane) {
+		const bool mine = (h >> lane) & 1;
+		if (MODE == 0 && KLOG) {
+			const uint32_t n = (uint32_t)__popcll(h);
+			if (hcnt + n > 64u) drain(lane);
+			if (mine) klog[hcnt + mbcnt64(h)] = make_uint4(jq, (uint32_t)kv, (uint32_t)(kv >> 32), 0u);
+			hcnt += n;
+		} else if (MODE == 0) {
+			if (mine) or_keys<LDS_BM>(bm_base + (size_t)jq * kw, kv, ovf);
+		} else {
+			const uint32_t n = (uint32_t)__popcll(h);
+			if (hcnt + n > (uint32_t)HBUF) flush(lane);
+			if (mine) hb[hcnt + mbcnt64(h)] = make_uint2(qi, b * 64 + lane);
+			hcnt += n;
+		}
+	}
+};
+
+// MODE 0: key bitmaps (LDS_BM: in LDS, else in gbm with workgroup-scope
+// atomics); MODE 1: hit pairs.  VAR 0: candidate query words from LDS, two
+// tests per iteration; VAR 1: from pre-inverted query words in HBM through
+// scalar loads, four independent tests per iteration.
+template <int MODE, bool LDS_BM, int TILES, int NW, int VAR>
+__global__ void __launch_bounds__(NW * 64)
+k_match_q(mq_args a) {
+	constexpr int NQ = TILES * 64;
+	constexpr int NT = NW * 64;
+	constexpr bool KLOG = MODE == 0 && (VAR & 2);
+	__shared__ uint32_t s_q[NQ * 8];
+	__shared__ uint32_t s_item[2];
+	__shared__ uint32_t s_hits;
+	__shared__ uint2 s_hbuf[MODE == 1 ? NW * HBUF : 1];
+	__shared__ uint4 s_klog[KLOG ? NW * 64 : 1];
+	extern __shared__ uint32_t s_bm[];        // LDS_BM: NQ x kw
+	const uint32_t tid = threadIdx.x, lane = tid & 63;
+	const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const uint32_t kw = a.f.kw;
+	const uint32_t home = blockIdx.x % NQUEUE;
+	const uint32_t P = a.f.n_parts;
+
+	for (;;) {
+		uint64_t t_start = 0;
+		if (tid == 0) {
+			unsigned long long item = ~0ULL;
+			for (int k = 0; k < NQUEUE; ++k) {
+				const uint32_t x = (home + k) % NQUEUE;
+				if (a.qnext[x] >= a.qbeg[x + 1]) continue;
+				const unsigned long long i = atomicAdd(&a.qnext[x], 1ULL);
+				if (i < a.qbeg[x + 1]) { item = i; break; }
+			}
+			s_item[0] = (uint32_t)item;
+			s_item[1] = item == ~0ULL;
+			s_hits = 0;
+			t_start = wall_clock64();
+		}
+		__syncthreads();
+		if (s_item[1]) break;
+		const uint32_t item = __builtin_amdgcn_readfirstlane(s_item[0]);
+		const uint32_t q0 = item * NQ;
+
+		for (uint32_t t = tid; t < (uint32_t)NQ; t += NT) {
+			uint32_t * d = &s_q[t * 8];
+			const uint32_t qi = q0 + t;
+			if (qi < a.nq) {
+				const uint4 x = *reinterpret_cast<const uint4 *>(a.q + (size_t)qi * 8);
+				const uint2 y = *reinterpret_cast<const uint2 *>(a.q + (size_t)qi * 8 + 4);
+				d[0] = ~x.x; d[1] = ~x.y; d[2] = ~x.z; d[3] = ~x.w; d[4] = ~y.x; d[5] = ~y.y;
+				d[6] = qi;
+			} else {
+				d[0] = d[1] = d[2] = d[3] = d[4] = d[5] = 0;
+				d[6] = NO_QUERY;
+			}
+		}
+		hit_sink<MODE, LDS_BM, KLOG> sink;
+		sink.klog = &s_klog[KLOG ? wave * 64 : 0];
+		sink.kw = kw;
+		sink.hcnt = 0;
+		sink.ovf = a.f.ovf;
+		sink.counter = &a.counters[0];
+		sink.hits = a.hits;
+		sink.cap = a.hit_cap;
+		sink.hb = &s_hbuf[MODE == 1 ? wave * HBUF : 0];
+		sink.bm_base = nullptr;
+		if (MODE == 0) {
+			if (LDS_BM) {
+				for (uint32_t w = tid; w < (uint32_t)NQ * kw; w += NT) s_bm[w] = 0;
+				sink.bm_base = s_bm;
+			} else {
+				sink.bm_base = a.gbm + (size_t)q0 * kw;
+				const uint32_t nw = (min(a.nq, q0 + NQ) - q0) * kw;
+				for (uint32_t w = tid; w < nw; w += NT) sink.bm_base[w] = 0;
+			}
+		}
+		__syncthreads();
+
+		uint32_t qv[TILES][6];
+		uint64_t qok[TILES];
+#pragma unroll
+		for (int t = 0; t < TILES; ++t) {
+			const uint32_t * d = &s_q[(t * 64 + lane) * 8];
+#pragma unroll
+			for (int j = 0; j < 6; ++j) qv[t][j] = d[j];
+			qok[t] = __ballot(d[6] != NO_QUERY);
+		}
+
+		uint32_t hit_acc = 0, n_visit = 0, n_test = 0, n_hev = 0;
+		for (uint32_t r = 0; r < P; ++r) {
+			const uint32_t p = a.porder ? sld1(a.porder + r) : r;
+			const uint32_t * pi = a.f.part_info + (size_t)p * 8;
+			const uint4 pa = sld4(pi);
+			const uint4 pb = sld4(pi + 4);
+			const uint32_t fb = pb.z, nb = pb.w;
+			uint64_t qsel[TILES];
+			uint64_t anyq = 0;
+			uint32_t nsel = 0;
+#pragma unroll
+			for (int t = 0; t < TILES; ++t) {
+				const uint32_t x = (pa.x & qv[t][0]) | (pa.y & qv[t][1]) | (pa.z & qv[t][2]) |
+								   (pa.w & qv[t][3]) | (pb.x & qv[t][4]) | (pb.y & qv[t][5]);
+				qsel[t] = __ballot(x == 0) & qok[t];
+				anyq |= qsel[t];
+				nsel += (uint32_t)__popcll(qsel[t]);
+			}
+			if (anyq == 0 || nb == 0) continue;
+			if (wave == 0 && lane == 0 && a.pcount) atomicAdd(&a.pcount[p], nsel);
+			for (uint32_t b = fb + wave; b < fb + nb; b += NW) {
+				const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
+				const uint4 ma = sld4(bi);
+				const uint4 mb = sld4(bi + 4);
+				const uint32_t nv = mb.z;
+				uint64_t cand[TILES];
+				uint64_t any = 0;
+#pragma unroll
+				for (int t = 0; t < TILES; ++t) {
+					if (qsel[t] == 0) { cand[t] = 0; continue; }
+					const uint32_t x = (ma.x & qv[t][0]) | (ma.y & qv[t][1]) | (ma.z & qv[t][2]) |
+									   (ma.w & qv[t][3]) | (mb.x & qv[t][4]) | (mb.y & qv[t][5]);
+					cand[t] = __ballot(x == 0) & qsel[t];
+					any |= cand[t];
+				}
+				if (any == 0) continue;
+				++n_visit;
+
+				const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
+				const uint32_t f0 = bw[0], f1 = bw[64], f2 = bw[128], f3 = bw[192], f4 = bw[256], f5 = bw[320];
+				const uint64_t kv = MODE == 0 ? a.f.slot_keys[(size_t)b * 64 + lane] : 0;
+				const uint64_t lmask = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
+
+#pragma unroll
+				for (int t = 0; t < TILES; ++t) {
+					uint64_t c = cand[t];
+					if ((VAR & 1) == 0) {
+						while (c) {
+							const uint32_t j0 = (uint32_t)__builtin_ctzll(c);
+							c &= c - 1;
+							const uint64_t keep1 = c ? ~0ULL : 0ULL;
+							const uint32_t j1 = c ? (uint32_t)__builtin_ctzll(c) : j0;
+							c &= c - 1;
+							const uint32_t * qq0 = &s_q[(t * 64 + j0) * 8];
+							const uint32_t * qq1 = &s_q[(t * 64 + j1) * 8];
+							const uint4 a0 = *reinterpret_cast<const uint4 *>(qq0);
+							const uint2 b0 = *reinterpret_cast<const uint2 *>(qq0 + 4);
+							const uint4 a1 = *reinterpret_cast<const uint4 *>(qq1);
+							const uint2 b1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
+							const uint32_t acc0 = (f0 & a0.x) | (f1 & a0.y) | (f2 & a0.z) | (f3 & a0.w) | (f4 & b0.x) | (f5 & b0.y);
+							const uint32_t acc1 = (f0 & a1.x) | (f1 & a1.y) | (f2 & a1.z) | (f3 & a1.w) | (f4 & b1.x) | (f5 & b1.y);
+							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
+							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
+							n_test += keep1 ? 2 : 1;
+							if (h0) { ++n_hev; hit_acc += (uint32_t)__popcll(h0); sink.add(h0, t * 64 + j0, q0 + t * 64 + j0, b, kv, lane); }
+							if (h1) { ++n_hev; hit_acc += (uint32_t)__popcll(h1); sink.add(h1, t * 64 + j1, q0 + t * 64 + j1, b, kv, lane); }
+						}
+					} else {
+						const uint32_t * qbase = a.qinv + (size_t)(q0 + t * 64) * 8;
+						while (c) {
+							// up to four candidates, tested as independent chains;
+							// missing ones repeat j0 and are masked out afterwards
+							const uint32_t j0 = (uint32_t)__builtin_ctzll(c);
+							c &= c - 1;
+							const uint64_t k1 = c ? ~0ULL : 0ULL;
+							const uint32_t j1 = c ? (uint32_t)__builtin_ctzll(c) : j0;
+							c &= c - 1;
+							const uint64_t k2 = c ? ~0ULL : 0ULL;
+							const uint32_t j2 = c ? (uint32_t)__builtin_ctzll(c) : j0;
+							c &= c - 1;
+							const uint64_t k3 = c ? ~0ULL : 0ULL;
+							const uint32_t j3 = c ? (uint32_t)__builtin_ctzll(c) : j0;
+							c &= c - 1;
+							const uint4 x0 = sld4(qbase + j0 * 8), x1 = sld4(qbase + j1 * 8);
+							const uint4 x2 = sld4(qbase + j2 * 8), x3 = sld4(qbase + j3 * 8);
+							const uint2 y0 = sld2(qbase + j0 * 8 + 4), y1 = sld2(qbase + j1 * 8 + 4);
+							const uint2 y2 = sld2(qbase + j2 * 8 + 4), y3 = sld2(qbase + j3 * 8 + 4);
+							const uint32_t acc0 = (f0 & x0.x) | (f1 & x0.y) | (f2 & x0.z) | (f3 & x0.w) | (f4 & y0.x) | (f5 & y0.y);
+							const uint32_t acc1 = (f0 & x1.x) | (f1 & x1.y) | (f2 & x1.z) | (f3 & x1.w) | (f4 & y1.x) | (f5 & y1.y);
+							const uint32_t acc2 = (f0 & x2.x) | (f1 & x2.y) | (f2 & x2.z) | (f3 & x2.w) | (f4 & y2.x) | (f5 & y2.y);
+							const uint32_t acc3 = (f0 & x3.x) | (f1 & x3.y) | (f2 & x3.z) | (f3 & x3.w) | (f4 & y3.x) | (f5 & y3.y);
+							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
+							const uint64_t h1 = __ballot(acc1 == 0) & lmask & k1;
+							const uint64_t h2 = __ballot(acc2 == 0) & lmask & k2;
+							const uint64_t h3 = __ballot(acc3 == 0) & lmask & k3;
+							n_test += 1 + (k1 ? 1 : 0) + (k2 ? 1 : 0) + (k3 ? 1 : 0);
+							if (h0 | h1 | h2 | h3) {
+								if (h0) { ++n_hev; hit_acc += (uint32_t)__popcll(h0); sink.add(h0, t * 64 + j0, q0 + t * 64 + j0, b, kv, lane); }
+								if (h1) { ++n_hev; hit_acc += (uint32_t)__popcll(h1); sink.add(h1, t * 64 + j1, q0 + t * 64 + j1, b, kv, lane); }
+								if (h2) { ++n_hev; hit_acc += (uint32_t)__popcll(h2); sink.add(h2, t * 64 + j2, q0 + t * 64 + j2, b, kv, lane); }
+								if (h3) { ++n_hev; hit_acc += (uint32_t)__popcll(h3); sink.add(h3, t * 64 + j3, q0 + t * 64 + j3, b, kv, lane); }
+							}
+						}
+					}
+				}
+			}
+		}
+		if (MODE == 1 && sink.hcnt) sink.flush(lane);
+		if (KLOG && sink.hcnt) sink.drain(lane);
+		if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
+		if (lane == 0 && n_visit) {
+			atomicAdd(&a.counters[2], (unsigned long long)n_visit);
+			atomicAdd(&a.counters[3], (unsigned long long)n_test);
+			atomicAdd(&a.counters[4], (unsigned long long)n_hev);
+		}
+		__syncthreads();
+		if (MODE == 0 && LDS_BM) {
+			const uint32_t nw = (min(a.nq, q0 + NQ) - q0) * kw;
+			uint32_t * dst = a.gbm + (size_t)q0 * kw;
+			for (uint32_t w = tid; w < nw; w += NT) dst[w] = s_bm[w];
+		}
+		if (tid == 0) {
+			if (s_hits) atomicAdd(&a.counters[1], (unsigned long long)s_hits);
+			if (a.log && item < a.log_cap) {
+				a.log[(size_t)item * 2] = s_hits;
+				a.log[(size_t)item * 2 + 1] = (uint32_t)(wall_clock64() - t_start);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Inverted query words (6 + 2 pad per query) for the scalar-load variant.
+__global__ void __launch_bounds__(256)
+k_invert_queries(const uint32_t * __restrict__ q, uint32_t nq, uint32_t * __restrict__ qinv) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= nq * 8) return;
+	qinv[i] = (i & 7) < 6 ? ~q[i] : 0u;
+}
+
+// Queue bounds for `items` work items, NQUEUE contiguous ranges.
+__global__ void k_init_queues(uint64_t items, uint64_t * __restrict__ qbeg, unsigned long long * __restrict__ qnext) {
+	const uint32_t t = threadIdx.x;
+	if (t <= NQUEUE) qbeg[t] = items * t / NQUEUE;
+	if (t < NQUEUE) qnext[t] = items * t / NQUEUE;
+}
+
 // ------------------------------------------------------------- finalize
 
 // KEYS_UNIQUE: one wave per query counts its bitmap bits.
@@ -390,10 +756,9 @@ k_bm_count(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t
 }
 
 // KEYS_UNIQUE: one wave per query writes its keys in ascending id order,
-// which is ascending key order (the dictionary is sorted), and clears the
-// bitmap for the next submission.
+// which is ascending key order (the dictionary is sorted).
 __global__ void __launch_bounds__(256)
-k_bm_write(uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint64_t * __restrict__ off,
+k_bm_write(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint64_t * __restrict__ off,
 		   const uint32_t * __restrict__ dict, uint32_t * __restrict__ vals, uint64_t cap) {
 	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
 	if (qi >= nq) return;
@@ -409,7 +774,6 @@ k_bm_write(uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint64_t
 		uint32_t tot = __shfl(x, 63, 64);
 		if (v) {
 			uint64_t p = pos + x - c;
-			gbm[(size_t)qi * kw + w] = 0;
 			while (v) {
 				uint32_t bit = __builtin_ctz(v);
 				v &= v - 1;
@@ -484,6 +848,9 @@ __global__ void k_gather_stats(const uint64_t * __restrict__ pbase, uint32_t P,
 		out[2] = counters[1];     // hits
 		out[3] = off[nq];         // result values
 		out[4] = item_prefix[P];  // work items
+		out[5] = counters[2];     // block visits
+		out[6] = counters[3];     // subset tests
+		out[7] = counters[4];     // tests with hits
 	}
 }
 
@@ -505,6 +872,41 @@ match_fn pick_match(int mode, bool stage, int tiles) {
 	return pick_tiles<1, false>(tiles);
 }
 
+using mq_fn = void (*)(mq_args);
+
+template <int MODE, bool LDS_BM, int TILES, int VAR>
+mq_fn pick_q_waves(int nw) {
+	switch (nw) {
+	case 4: return k_match_q<MODE, LDS_BM, TILES, 4, VAR>;
+	case 16: return k_match_q<MODE, LDS_BM, TILES, 16, VAR>;
+	default: return k_match_q<MODE, LDS_BM, TILES, 8, VAR>;
+	}
+}
+
+template <int MODE, bool LDS_BM, int VAR>
+mq_fn pick_q_tiles(int tiles, int nw) {
+	switch (tiles) {
+	case 1: return pick_q_waves<MODE, LDS_BM, 1, VAR>(nw);
+	case 2: return pick_q_waves<MODE, LDS_BM, 2, VAR>(nw);
+	default: return pick_q_waves<MODE, LDS_BM, 4, VAR>(nw);
+	}
+}
+
+template <int VAR>
+mq_fn pick_q_mode(int mode, bool lds_bm, int tiles, int nw) {
+	if (mode == 0) return lds_bm ? pick_q_tiles<0, true, VAR>(tiles, nw) : pick_q_tiles<0, false, VAR>(tiles, nw);
+	return pick_q_tiles<1, false, VAR>(tiles, nw);
+}
+
+mq_fn pick_match_q(int mode, bool lds_bm, int tiles, int nw, int var) {
+	switch (var) {
+	case 1: return pick_q_mode<1>(mode, lds_bm, tiles, nw);
+	case 2: return pick_q_mode<2>(mode, lds_bm, tiles, nw);
+	case 3: return pick_q_mode<3>(mode, lds_bm, tiles, nw);
+	default: return pick_q_mode<0>(mode, lds_bm, tiles, nw);
+	}
+}
+
 } // namespace
 
 // ------------------------------------------------------------- ring
@@ -519,6 +921,7 @@ struct tm_ring {
 
 	uint32_t * h_q = nullptr;      // pinned staging, max_q x 8
 	uint32_t * d_q = nullptr;      // max_q x 8
+	uint32_t * d_qinv = nullptr;   // max_q x 8, inverted (variant 1)
 	uint32_t * d_pcount = nullptr; // P
 	uint64_t * d_pbase = nullptr;  // P + 1
 	unsigned long long * d_pcur = nullptr;
@@ -547,10 +950,11 @@ struct tm_ring {
 	uint64_t log_cap = 0;
 
 	// effective launch parameters
-	int tiles = 4, wgs_per_cu = 2;
+	int tiles = 4, wgs_per_cu = 2, waves = 8, variant = 1;
 	uint32_t chunk = 256;
-	bool stage = false;
-	size_t stage_lds = 0;
+	bool stage = false;        // PLAIN: LDS-staged key bitmaps
+	bool lds_bm = false;       // PERSISTENT: key bitmaps in LDS
+	size_t stage_lds = 0;      // dynamic LDS bytes of the match launch
 
 	// graph replay
 	hipGraphExec_t gexec = nullptr;
@@ -574,7 +978,7 @@ void tm_ring::release() {
 	if (h_stat) (void)hipHostFree(h_stat);
 	h_q = nullptr;
 	h_stat = nullptr;
-	dfree(d_q); dfree(d_pcount); dfree(d_pbase); dfree(d_pcur); dfree(d_list);
+	dfree(d_q); dfree(d_qinv); dfree(d_pcount); dfree(d_pbase); dfree(d_pcur); dfree(d_list);
 	dfree(d_sched); dfree(d_iprefix); dfree(d_qbeg); dfree(d_qnext); dfree(d_porder);
 	dfree(d_gbm); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
 	dfree(d_vals); dfree(d_cnt); dfree(d_scan); dfree(d_stat); dfree(d_log);
@@ -600,26 +1004,36 @@ int resolve_params(tm_ring * r) {
 		return set_error(TM_ERR_INVALID,
 						 "KEYS_UNIQUE supports up to 65536 distinct keys (FIB has %u); use KEYS_ALL",
 						 r->fib->n_keys);
+	uint32_t bs = c.batch_size ? c.batch_size : 256;
+	if (bs != 64 && bs != 128 && bs != 256)
+		return set_error(TM_ERR_INVALID, "batch_size must be 64, 128 or 256 (got %u)", bs);
+	r->tiles = (int)(bs / 64);
+	const size_t max_lds = (size_t)dev().max_lds;
+	const size_t nq_grp = (size_t)r->tiles * 64;
 	if (c.schedule == TM_SCHED_PLAIN) {
-		uint32_t bs = c.batch_size ? c.batch_size : 256;
-		if (bs != 64 && bs != 128 && bs != 256)
-			return set_error(TM_ERR_INVALID, "batch_size must be 64, 128 or 256 (got %u)", bs);
-		r->tiles = (int)(bs / 64);
-		r->chunk = WAVES;            // one block per wave, like one filter per thread
-		r->stage = false;
-	} else {
-		uint32_t bs = c.batch_size ? c.batch_size : 256;
-		if (bs != 64 && bs != 128 && bs != 256)
-			return set_error(TM_ERR_INVALID, "batch_size must be 64, 128 or 256 (got %u)", bs);
-		r->tiles = (int)(bs / 64);
-		r->chunk = c.chunk_blocks ? c.chunk_blocks : 256;
-		size_t lds = (size_t)r->tiles * 64 * r->kw * 4;
-		size_t static_lds = (size_t)r->tiles * 64 * 8 * 4 + 64;
-		r->stage = (mode_of(c) == 0) && lds <= STAGE_LDS_MAX && lds + static_lds <= (size_t)dev().max_lds;
+		// one launch per (partition, batch); each workgroup covers a chunk of
+		// the partition's blocks and flushes its LDS key bitmaps once
+		r->waves = WAVES;
+		r->chunk = c.chunk_blocks ? c.chunk_blocks : 16;
+		size_t lds = nq_grp * r->kw * 4;
+		size_t static_lds = nq_grp * 32 + 64;
+		r->stage = (mode_of(c) == 0) && lds <= STAGE_LDS_MAX && lds + static_lds <= max_lds;
 		r->stage_lds = r->stage ? lds : 0;
-		int by_lds = (int)(160 * 1024 / (r->stage_lds + static_lds));
-		int want = c.groups_per_cu ? (int)c.groups_per_cu : 4;
-		r->wgs_per_cu = std::max(1, std::min(want, by_lds));
+	} else {
+		r->variant = c.variant <= 3 ? (int)c.variant : 2;
+		int nw = c.waves_per_group ? (int)c.waves_per_group : 8;
+		if (nw != 4 && nw != 8 && nw != 16)
+			return set_error(TM_ERR_INVALID, "waves_per_group must be 4, 8 or 16 (got %d)", nw);
+		r->waves = nw;
+		size_t static_lds = nq_grp * 32 + 64 + (mode_of(c) == 1 ? (size_t)nw * HBUF * 8 : 8) +
+							((mode_of(c) == 0 && (r->variant & 2)) ? (size_t)nw * 64 * 16 : 16);
+		size_t bm = nq_grp * r->kw * 4;
+		r->lds_bm = (mode_of(c) == 0) && bm + static_lds <= max_lds;
+		r->stage_lds = r->lds_bm ? bm : 0;
+		int by_lds = (int)(std::min<size_t>(160 * 1024, max_lds) / (r->stage_lds + static_lds));
+		int by_waves = 32 / nw;
+		int want = c.groups_per_cu ? (int)c.groups_per_cu : 64;
+		r->wgs_per_cu = std::max(1, std::min(want, std::min(by_lds, by_waves)));
 	}
 	return TM_OK;
 }
@@ -639,7 +1053,7 @@ void enqueue_front(tm_ring * r, uint32_t nq) {
 	const tm_fib * f = r->fib;
 	const uint32_t P = r->P;
 	fill_u32(r->d_pcount, 0, P, s);
-	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 4, s);
+	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 8, s);
 	uint32_t g = (nq + 255) / 256;
 	if (g == 0) g = 1;
 	hipLaunchKernelGGL(k_lookup_count, dim3(g), dim3(256), 0, s, r->d_q, nq, f->part_info, P, r->d_pcount);
@@ -709,14 +1123,42 @@ void enqueue_finalize(tm_ring * r, uint32_t nq) {
 
 // The persistent step: everything between the query upload and the stats.
 int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
-	enqueue_front(r, nq);
-	match_args a = base_args(r);
-	match_fn fn = pick_match(mode_of(r->cfg), r->stage, r->tiles);
-	uint32_t grid = (uint32_t)(dev().num_cu * r->wgs_per_cu);
-	if (events) TM_HIP(hipEventRecord(r->em0, r->s));
-	hipLaunchKernelGGL(fn, dim3(grid), dim3(WG), r->stage_lds, r->s, a);
+	hipStream_t s = r->s;
+	const uint32_t P = r->P;
+	fill_u32(r->d_pcount, 0, P, s);
+	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 8, s);
+	const uint64_t nqg = (uint64_t)r->tiles * 64;
+	const uint64_t items = (nq + nqg - 1) / nqg;
+	hipLaunchKernelGGL(k_init_queues, dim3(1), dim3(64), 0, s, items, r->d_qbeg, r->d_qnext);
+	mq_args a;
+	std::memset(&a, 0, sizeof(a));
+	a.f = r->fib->view();
+	a.q = r->d_q;
+	a.nq = nq;
+	a.porder = r->have_order ? r->d_porder : nullptr;
+	a.qnext = r->d_qnext;
+	a.qbeg = r->d_qbeg;
+	a.pcount = r->d_pcount;
+	a.gbm = r->d_gbm;
+	a.hits = r->d_hits;
+	a.hit_cap = r->hit_cap;
+	a.counters = r->d_cnt;
+	a.log = r->cfg.log_units ? r->d_log : nullptr;
+	a.log_cap = r->log_cap;
+	if (r->variant & 1) {
+		a.qinv = r->d_qinv;
+		const uint64_t nw = (uint64_t)nq * 8;
+		hipLaunchKernelGGL(k_invert_queries, dim3((uint32_t)std::max<uint64_t>(1, (nw + 255) / 256)), dim3(256), 0, s,
+						   r->d_q, nq, r->d_qinv);
+	}
+	mq_fn fn = pick_match_q(mode_of(r->cfg), r->lds_bm, r->tiles, r->waves, r->variant);
+	uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)dev().num_cu * r->wgs_per_cu, std::max<uint64_t>(items, 1));
+	if (events) TM_HIP(hipEventRecord(r->em0, s));
+	hipLaunchKernelGGL(fn, dim3(grid), dim3(r->waves * 64), r->stage_lds, s, a);
 	TM_HIP(hipGetLastError());
-	if (events) TM_HIP(hipEventRecord(r->em1, r->s));
+	if (events) TM_HIP(hipEventRecord(r->em1, s));
+	scan_u32_to_u64(r->d_pcount, r->d_pbase, P, r->d_scan, s);
+	fill_u64(r->d_iprefix + P, items, 1, s);
 	enqueue_finalize(r, nq);
 	return TM_OK;
 }
@@ -724,6 +1166,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 // The plain step: one launch per (partition, batch), as the reference
 // launches one kernel per (partition, batch) (back_end.cc:284-327).
 int enqueue_plain(tm_ring * r, uint32_t nq) {
+	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, r->s);
 	enqueue_front(r, nq);
 	std::vector<uint4> sched(r->P);
 	if (r->P) {
@@ -731,7 +1174,7 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 		TM_HIP(hipStreamSynchronize(r->s));
 	}
 	match_args a = base_args(r);
-	match_fn fn = pick_match(mode_of(r->cfg), false, r->tiles);
+	match_fn fn = pick_match(mode_of(r->cfg), r->stage, r->tiles);
 	uint32_t launches = 0;
 	TM_HIP(hipEventRecord(r->em0, r->s));
 	for (uint32_t k = 0; k < r->P; ++k) {
@@ -740,7 +1183,7 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 		for (uint32_t b = 0; b < sc.y; ++b) {
 			a.plain_rank = (int32_t)k;
 			a.plain_batch = b;
-			hipLaunchKernelGGL(fn, dim3(sc.z), dim3(WG), 0, r->s, a);
+			hipLaunchKernelGGL(fn, dim3(sc.z), dim3(WG), r->stage_lds, r->s, a);
 			++launches;
 		}
 	}
@@ -762,7 +1205,7 @@ int ensure_capacity(tm_ring * r) {
 		TM_CHECK(dalloc(&r->d_qbeg, NQUEUE + 1));
 		TM_CHECK(dalloc(&r->d_qnext, NQUEUE));
 		TM_CHECK(dalloc(&r->d_porder, P + 1));
-		TM_CHECK(dalloc(&r->d_cnt, 4));
+		TM_CHECK(dalloc(&r->d_cnt, 8));
 		TM_CHECK(dalloc(&r->d_stat, 8));
 		TM_CHECK(dalloc(&r->d_qcount, (size_t)r->max_q + 1));
 		TM_CHECK(dalloc(&r->d_off, (size_t)r->max_q + 1));
@@ -780,7 +1223,7 @@ int ensure_capacity(tm_ring * r) {
 		TM_CHECK(dalloc(&r->d_hits, cap));
 		r->hit_cap = cap;
 	}
-	if (!r->d_list) {
+	if (!r->d_list && r->cfg.schedule == TM_SCHED_PLAIN) {
 		uint64_t cap = (uint64_t)r->max_q * std::min<uint32_t>(std::max<uint32_t>(P, 1), 64);
 		TM_CHECK(grow_u32(&r->d_list, r->list_cap, cap));
 	}
@@ -880,6 +1323,7 @@ int tm_ring_create(tm_fib * fib, uint32_t max_queries, const tm_match_config * c
 		return code;
 	}
 	rc = dalloc(&r->d_q, (size_t)max_queries * 8);
+	if (rc == TM_OK) rc = dalloc(&r->d_qinv, (size_t)max_queries * 8);
 	if (rc == TM_OK) rc = ensure_capacity(r);
 	if (rc != TM_OK) { r->release(); delete r; return rc; }
 	*out = r;
@@ -955,7 +1399,7 @@ int tm_wait(tm_ring * r, tm_status * st) {
 			TM_HIP(hipEventSynchronize(r->e1));
 			const uint64_t pairs = r->h_stat[0], slots = r->h_stat[1], results = r->h_stat[3];
 			bool again = false;
-			if (pairs > r->list_cap) {
+			if (r->cfg.schedule == TM_SCHED_PLAIN && pairs > r->list_cap) {
 				TM_CHECK(grow_u32(&r->d_list, r->list_cap, pairs + pairs / 4 + 1024));
 				again = true;
 			}
@@ -995,6 +1439,9 @@ int tm_wait(tm_ring * r, tm_status * st) {
 		r->st.error = TM_OK;
 		r->st.device_ms = dms;
 		r->st.match_ms = mms;
+		r->st.n_block_visits = r->h_stat[5];
+		r->st.n_cand_tests = r->h_stat[6];
+		r->st.n_hit_events = r->h_stat[7];
 	}
 	if (st) *st = r->st;
 	return TM_OK;
