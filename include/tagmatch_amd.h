@@ -194,6 +194,10 @@ typedef struct tm_status {
 	uint64_t n_cand_tests;   /* PERSISTENT: 64-filter subset tests run
 	                            (one per candidate query per block)          */
 	uint64_t n_hit_events;   /* PERSISTENT: subset tests with >= 1 hit       */
+	double   pred_hits;      /* cost model: predicted hits of the submission
+	                            (0 without a model)                          */
+	double   model_ms;       /* cost model: host time spent pricing and
+	                            ordering the submission                      */
 } tm_status;
 
 /* Create a ring for up to max_queries queries per submission. */
@@ -233,6 +237,60 @@ int tm_ring_lookup_counts(tm_ring * ring, uint32_t * counts, uint32_t n_parts);
 int tm_match(tm_fib * fib, const uint32_t * query_words, uint32_t nq,
 			 const tm_match_config * cfg, uint64_t * offsets, uint32_t * values,
 			 uint64_t capacity, tm_status * st);
+
+/* ------------------------------------------------------------------------ */
+/* Learned cost model.  No counterpart in the reference, whose scheduler has
+ * no cost estimate (front_end.cc:350-376 flushes batches in arrival order and
+ * back_end.cc:256-269 sizes every result buffer for the worst case).
+ *
+ * Work units are (partition, query batch) pairs.  The plain schedule logs one
+ * row per launch when cfg.log_units is set: TM_UNIT_FEATURES float features,
+ * the unit's hits and its summed workgroup time in 10 ns ticks.  A
+ * TM_UNIT_FEATURES-64-64-2 multilayer perceptron (FP32, ReLU) trained on
+ * those rows with targets log2(1 + hits) / 16 and log2(1 + ticks) / 16
+ * prices the persistent schedule's (query tile, partition) units.  Its
+ * predictions pre-size result buffers and order work items; they never
+ * skip work, and overflow-retry still catches an under-estimate.
+ *
+ * Feature order: log2(1+filters)/20, popcount(mask)/64, log2(1+B)/12, query
+ * popcount mean, std, min, max (/192), fractions of queries with popcount in
+ * [0,80) [80,110) [110,140) [140,192], mean member popcount/64,
+ * popcount(OR of members)/192, mean and max popcount(query & OR)/192,
+ * log2(1 + B * filters)/32. */
+#define TM_UNIT_FEATURES 16
+
+typedef struct tm_mlp tm_mlp;
+
+/* n_in -> n_hidden -> n_hidden -> n_out, He-initialised from seed.  The
+ * parameters are one FP32 array: W1[n_in][h], b1[h], W2[h][h], b2[h],
+ * W3[h][n_out], b3[n_out]. */
+int tm_mlp_create(uint32_t n_in, uint32_t n_hidden, uint32_t n_out, uint64_t seed, tm_mlp ** out);
+int tm_mlp_destroy(tm_mlp * m);
+uint64_t tm_mlp_param_count(const tm_mlp * m);
+int tm_mlp_get_params(const tm_mlp * m, float * params, uint64_t n);
+int tm_mlp_set_params(tm_mlp * m, const float * params, uint64_t n);
+/* y (n x n_out) = network(x (n x n_in)), host buffers. */
+int tm_mlp_forward(tm_mlp * m, const float * x, uint32_t n, float * y);
+/* Mean squared error over all n x n_out outputs and its gradient with
+ * respect to every parameter (grad: tm_mlp_param_count floats, may be NULL). */
+int tm_mlp_loss_grad(tm_mlp * m, const float * x, const float * t, uint32_t n, float * loss, float * grad);
+/* Adam (0.9, 0.999, 1e-8) on shuffled minibatches, all on the GPU; *loss =
+ * mean minibatch loss of the last epoch. */
+int tm_mlp_train(tm_mlp * m, const float * x, const float * t, uint32_t n, uint32_t epochs, uint32_t batch,
+				 float lr, uint64_t seed, float * loss);
+
+/* Unit log of the ring's last plain submission (cfg.log_units): up to
+ * capacity rows of features (TM_UNIT_FEATURES floats), hits, ticks and the
+ * unit itself (3 words: partition, batch index, queries in the batch);
+ * *n_units = rows available.  Any output may be NULL. */
+int tm_ring_unit_log(tm_ring * ring, float * features, uint32_t * hits, uint32_t * ticks,
+					 uint32_t * units, uint32_t capacity, uint32_t * n_units);
+
+/* Use a trained TM_UNIT_FEATURES-64-64-2 model in the ring's PERSISTENT
+ * submissions (NULL: none).  The model must outlive its use by the ring. */
+#define TM_COST_PRESIZE 1u  /* grow result buffers to the predicted size first */
+#define TM_COST_ORDER   2u  /* visit query tiles longest-predicted first        */
+int tm_ring_set_cost_model(tm_ring * ring, tm_mlp * model, uint32_t flags);
 
 /* ------------------------------------------------------------------------ */
 /* Host: canonical filter table from (filter, keys) entries, as

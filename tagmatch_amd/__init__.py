@@ -62,7 +62,7 @@ class Status(C.Structure):
         ("n_hits", C.c_uint64), ("n_results", C.c_uint64), ("hit_capacity", C.c_uint64),
         ("overflow", C.c_int32), ("error", C.c_int32), ("device_ms", C.c_double),
         ("match_ms", C.c_double), ("n_block_visits", C.c_uint64), ("n_cand_tests", C.c_uint64),
-        ("n_hit_events", C.c_uint64),
+        ("n_hit_events", C.c_uint64), ("pred_hits", C.c_double), ("model_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -108,6 +108,17 @@ _SIGS = {
     "tm_ring_results_device": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
     "tm_ring_lookup_counts": (C.c_int, [P, P, C.c_uint32]),
     "tm_match": (C.c_int, [P, P, C.c_uint32, C.POINTER(MatchConfig), P, P, C.c_uint64, C.POINTER(Status)]),
+    "tm_mlp_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(P)]),
+    "tm_mlp_destroy": (C.c_int, [P]),
+    "tm_mlp_param_count": (C.c_uint64, [P]),
+    "tm_mlp_get_params": (C.c_int, [P, P, C.c_uint64]),
+    "tm_mlp_set_params": (C.c_int, [P, P, C.c_uint64]),
+    "tm_mlp_forward": (C.c_int, [P, P, C.c_uint32, P]),
+    "tm_mlp_loss_grad": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(C.c_float), P]),
+    "tm_mlp_train": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                               C.POINTER(C.c_float)]),
+    "tm_ring_unit_log": (C.c_int, [P, P, P, P, P, C.c_uint32, U32P]),
+    "tm_ring_set_cost_model": (C.c_int, [P, P, C.c_uint32]),
     "tm_canonicalize": (C.c_int, [P, C.c_uint64, P, P, C.POINTER(P), U64P, C.POINTER(P), C.POINTER(P)]),
     "tm_free": (None, [P]),
     "tm_workload_default_params": (None, [C.POINTER(WorkloadParams)]),
@@ -386,6 +397,29 @@ class Ring:
         check(load().tm_ring_lookup_counts(self.h, _ptr(c), n), "tm_ring_lookup_counts")
         return c[:n]
 
+    def unit_log(self):
+        """(features [n,16] f32, hits [n] u32, ticks [n] u32, units [n,3]
+        u32 = partition, batch, queries) of the last plain submission made
+        with log_units=1."""
+        n = C.c_uint32()
+        check(load().tm_ring_unit_log(self.h, None, None, None, None, 0, C.byref(n)), "tm_ring_unit_log")
+        k = n.value
+        f = np.zeros((k, UNIT_FEATURES), np.float32)
+        h = np.zeros(k, np.uint32)
+        t = np.zeros(k, np.uint32)
+        u = np.zeros((k, 3), np.uint32)
+        check(load().tm_ring_unit_log(self.h, _ptr(f), _ptr(h), _ptr(t), _ptr(u), k, C.byref(n)),
+              "tm_ring_unit_log")
+        return f, h, t, u
+
+    def set_cost_model(self, mlp, flags=None):
+        """Price PERSISTENT submissions with `mlp` (None: stop)."""
+        if flags is None:
+            flags = COST_PRESIZE | COST_ORDER
+        self.mlp = mlp
+        check(load().tm_ring_set_cost_model(self.h, mlp.h if mlp is not None else None,
+                                            flags if mlp is not None else 0), "tm_ring_set_cost_model")
+
     def query_buffer(self):
         p = C.c_void_p()
         check(load().tm_ring_query_buffer(self.h, C.byref(p)), "tm_ring_query_buffer")
@@ -414,6 +448,86 @@ def match(fib, queries, mode=MODE_KEYS_UNIQUE, schedule=SCHED_PERSISTENT, **kw):
     finally:
         r.close()
     return off, vals, st.as_dict()
+
+
+# ------------------------------------------------------------------ cost model
+
+UNIT_FEATURES = 16
+COST_PRESIZE = 1
+COST_ORDER = 2
+
+
+def encode_targets(hits, ticks):
+    """Training targets of the cost model: log2(1 + x) / 16."""
+    return np.stack([np.log2(1.0 + np.asarray(hits, np.float64)) / 16.0,
+                     np.log2(1.0 + np.asarray(ticks, np.float64)) / 16.0], axis=1).astype(np.float32)
+
+
+def decode_targets(y):
+    """Inverse of encode_targets: (hits, ticks) as float64."""
+    y = np.maximum(np.asarray(y, np.float64), 0.0)
+    return np.exp2(16.0 * y[:, 0]) - 1.0, np.exp2(16.0 * y[:, 1]) - 1.0
+
+
+class MLP:
+    """n_in -> h -> h -> n_out FP32 perceptron on the GPU (tm_mlp)."""
+
+    def __init__(self, n_in=UNIT_FEATURES, n_hidden=64, n_out=2, seed=1):
+        self.shape = (n_in, n_hidden, n_out)
+        self.h = C.c_void_p()
+        check(load().tm_mlp_create(n_in, n_hidden, n_out, seed, C.byref(self.h)), "tm_mlp_create")
+        self.n_params = int(load().tm_mlp_param_count(self.h))
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        check(load().tm_mlp_get_params(self.h, _ptr(p), self.n_params), "tm_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(load().tm_mlp_set_params(self.h, _ptr(p), p.size), "tm_mlp_set_params")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.shape[0])
+        y = np.zeros((x.shape[0], self.shape[2]), np.float32)
+        check(load().tm_mlp_forward(self.h, _ptr(x), x.shape[0], _ptr(y)), "tm_mlp_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.shape[0])
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, self.shape[2])
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        check(load().tm_mlp_loss_grad(self.h, _ptr(x), _ptr(t), x.shape[0], C.byref(loss), _ptr(g)),
+              "tm_mlp_loss_grad")
+        return float(loss.value), g
+
+    def train(self, x, t, epochs=10, batch=1024, lr=1e-3, seed=7):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.shape[0])
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, self.shape[2])
+        loss = C.c_float()
+        check(load().tm_mlp_train(self.h, _ptr(x), _ptr(t), x.shape[0], epochs, batch, lr, seed,
+                                  C.byref(loss)), "tm_mlp_train")
+        return float(loss.value)
+
+    def close(self):
+        if self.h:
+            load().tm_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mape(pred, true, floor=1.0):
+    """Mean absolute percentage error over entries with true >= floor."""
+    pred = np.asarray(pred, np.float64)
+    true = np.asarray(true, np.float64)
+    m = true >= floor
+    return float(np.mean(np.abs(pred[m] - true[m]) / true[m])) if m.any() else 0.0
 
 
 def split(off, vals):

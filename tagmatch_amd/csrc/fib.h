@@ -24,6 +24,9 @@ namespace tmx {
 //   key_dict  [K]         u32   sorted distinct keys; dense id = index.
 //   part_info [P][8]      u32   partition mask (6 words), first block,
 //                               block count.
+//   part_stat [P][8]      u32   OR of the member filters (6 words), member
+//                               count, total popcount of the members (cost
+//                               model features only).
 struct dev_fib_view {
 	const uint32_t * __restrict__ blk_words;
 	const uint32_t * __restrict__ blk_info;
@@ -32,6 +35,7 @@ struct dev_fib_view {
 	const uint32_t * __restrict__ ovf;
 	const uint32_t * __restrict__ key_dict;
 	const uint32_t * __restrict__ part_info;
+	const uint32_t * __restrict__ part_stat;
 	uint32_t n_parts;
 	uint32_t n_blocks;
 	uint32_t n_keys;   // K, distinct keys
@@ -51,6 +55,7 @@ struct tm_fib {
 	uint32_t * ovf = nullptr;
 	uint32_t * key_dict = nullptr;
 	uint32_t * part_info = nullptr;
+	uint32_t * part_stat = nullptr;
 	uint32_t n_parts = 0, n_blocks = 0, n_keys = 0;
 	uint64_t n_filters = 0, n_key_instances = 0, device_bytes = 0, max_part = 0;
 	// host copies used by the schedulers
@@ -66,6 +71,7 @@ struct tm_fib {
 		v.ovf = ovf;
 		v.key_dict = key_dict;
 		v.part_info = part_info;
+		v.part_stat = part_stat;
 		v.n_parts = n_parts;
 		v.n_blocks = n_blocks;
 		v.n_keys = n_keys;

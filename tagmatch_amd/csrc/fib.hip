@@ -30,6 +30,7 @@ void tm_fib::release() {
 	dfree(ovf);
 	dfree(key_dict);
 	dfree(part_info);
+	dfree(part_stat);
 	n_parts = n_blocks = n_keys = 0;
 	n_filters = n_key_instances = device_bytes = max_part = 0;
 	h_part_info.clear();
@@ -184,6 +185,7 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	std::vector<uint32_t> h_fid((size_t)nb * 64, 0xFFFFFFFFu);
 	std::vector<uint32_t> h_part((size_t)n_parts * 8, 0);
 	std::vector<uint32_t> h_psize(n_parts, 0);
+	std::vector<uint32_t> h_pstat((size_t)n_parts * 8, 0);
 	// overflow key lists (count, ids...) for filters with more than 4 keys
 	std::vector<uint32_t> h_ovf;
 	h_ovf.push_back(0);
@@ -211,6 +213,12 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 			}
 			info[6] = lane + 1;
 			info[7] = p;
+			uint32_t * ps = &h_pstat[(size_t)p * 8];
+			for (int j = 0; j < 6; ++j) {
+				ps[j] |= w[j];
+				ps[7] += (uint32_t)__builtin_popcount(w[j]);
+			}
+			ps[6] += 1;
 			h_fid[(size_t)blk * 64 + lane] = filter_ids ? filter_ids[i] : i;
 			uint32_t kb = key_offsets[i], ke = key_offsets[i + 1], kn = ke - kb;
 			if (inline_ok && kn <= 4) {
@@ -240,6 +248,7 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	TM_CHECK(upload(&fib->ovf, h_ovf, bytes));
 	TM_CHECK(upload(&fib->key_dict, dict, bytes));
 	TM_CHECK(upload(&fib->part_info, h_part, bytes));
+	TM_CHECK(upload(&fib->part_stat, h_pstat, bytes));
 	fib->n_parts = n_parts;
 	fib->n_blocks = nb;
 	fib->n_keys = K;

@@ -24,10 +24,14 @@
 //        sort + unique, and back_end.cc:329-343 key expansion);
 //   k_hits_count / scan / k_hits_write  (HITS, KEYS_ALL)
 //        group hits per query and expand them to ids or keys.
+#include "cost_model.h"
 #include "fib.h"
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 using namespace tmx;
@@ -162,6 +166,7 @@ struct match_args {
 	const uint64_t * __restrict__ qbeg;       // NQUEUE + 1 queue bounds
 	int32_t plain_rank;                       // >= 0: one (rank, batch) unit, chunk = blockIdx.x
 	uint32_t plain_batch;
+	uint32_t plain_unit;                      // log row of the (rank, batch) unit
 	uint32_t * gbm;                           // KEYS_UNIQUE: nq x kw bitmap words
 	uint2 * hits;                             // HITS: (query, block slot)
 	uint64_t hit_cap;
@@ -226,7 +231,7 @@ k_match(match_args a) {
 					uint32_t rem = sc.w - a.plain_batch * NQ;
 					s_item[3] = rem < (uint32_t)NQ ? rem : (uint32_t)NQ;
 					s_item[4] = 0;
-					s_id = ~0ULL;
+					s_id = a.plain_unit;
 				}
 			} else {
 				unsigned long long item = ~0ULL;
@@ -368,8 +373,14 @@ k_match(match_args a) {
 		if (tid == 0) {
 			if (s_hits) atomicAdd(&a.counters[1], (unsigned long long)s_hits);
 			if (a.log && s_id < a.log_cap) {
-				a.log[s_id * 2] = s_hits;
-				a.log[s_id * 2 + 1] = (uint32_t)(wall_clock64() - t_start);
+				const uint32_t ticks = (uint32_t)(wall_clock64() - t_start);
+				if (a.plain_rank >= 0) {   // the unit's chunks add up
+					atomicAdd(&a.log[s_id * 2], s_hits);
+					atomicAdd(&a.log[s_id * 2 + 1], ticks);
+				} else {
+					a.log[s_id * 2] = s_hits;
+					a.log[s_id * 2 + 1] = ticks;
+				}
 			}
 		}
 		__syncthreads();
@@ -402,6 +413,7 @@ struct mq_args {
 	const uint32_t * __restrict__ qinv;       // nq x 8 inverted words (VAR 1)
 	uint32_t nq;
 	const uint32_t * __restrict__ porder;     // partition schedule order (NULL: 0..P-1)
+	const uint32_t * __restrict__ iorder;     // query tile of each work item (NULL: identity)
 	unsigned long long * qnext;               // NQUEUE queue cursors
 	const uint64_t * __restrict__ qbeg;       // NQUEUE + 1 queue bounds (items)
 	uint32_t * pcount;                        // per partition: queries selecting it
@@ -547,7 +559,8 @@ k_match_q(mq_args a) {
 		}
 		__syncthreads();
 		if (s_item[1]) break;
-		const uint32_t item = __builtin_amdgcn_readfirstlane(s_item[0]);
+		const uint32_t item0 = __builtin_amdgcn_readfirstlane(s_item[0]);
+		const uint32_t item = a.iorder ? sld1(a.iorder + item0) : item0;   // query tile
 		const uint32_t q0 = item * NQ;
 
 		for (uint32_t t = tid; t < (uint32_t)NQ; t += NT) {
@@ -642,61 +655,61 @@ k_match_q(mq_args a) {
 				for (int t = 0; t < TILES; ++t) {
 					uint64_t c = cand[t];
 					if ((VAR & 1) == 0) {
+						// candidate query words broadcast from LDS, two per iteration
 						while (c) {
 							const uint32_t j0 = (uint32_t)__builtin_ctzll(c);
 							c &= c - 1;
 							const uint64_t keep1 = c ? ~0ULL : 0ULL;
 							const uint32_t j1 = c ? (uint32_t)__builtin_ctzll(c) : j0;
 							c &= c - 1;
+							n_test += keep1 ? 2 : 1;
 							const uint32_t * qq0 = &s_q[(t * 64 + j0) * 8];
 							const uint32_t * qq1 = &s_q[(t * 64 + j1) * 8];
-							const uint4 a0 = *reinterpret_cast<const uint4 *>(qq0);
-							const uint2 b0 = *reinterpret_cast<const uint2 *>(qq0 + 4);
-							const uint4 a1 = *reinterpret_cast<const uint4 *>(qq1);
-							const uint2 b1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
-							const uint32_t acc0 = (f0 & a0.x) | (f1 & a0.y) | (f2 & a0.z) | (f3 & a0.w) | (f4 & b0.x) | (f5 & b0.y);
-							const uint32_t acc1 = (f0 & a1.x) | (f1 & a1.y) | (f2 & a1.z) | (f3 & a1.w) | (f4 & b1.x) | (f5 & b1.y);
+							const uint2 a0 = *reinterpret_cast<const uint2 *>(qq0);
+							const uint2 a1 = *reinterpret_cast<const uint2 *>(qq1);
+							uint32_t acc0 = (f0 & a0.x) | (f1 & a0.y);
+							uint32_t acc1 = (f0 & a1.x) | (f1 & a1.y);
+							if (VAR & 4) {
+								// early exit: words 0-1 already reject every filter
+								// for both candidates (the common case)
+								const uint64_t alive = (__ballot(acc0 == 0) | (__ballot(acc1 == 0) & keep1)) & lmask;
+								if (alive == 0) continue;
+							}
+							const uint2 b0 = *reinterpret_cast<const uint2 *>(qq0 + 2);
+							const uint2 c0 = *reinterpret_cast<const uint2 *>(qq0 + 4);
+							const uint2 b1 = *reinterpret_cast<const uint2 *>(qq1 + 2);
+							const uint2 c1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
+							acc0 |= (f2 & b0.x) | (f3 & b0.y) | (f4 & c0.x) | (f5 & c0.y);
+							acc1 |= (f2 & b1.x) | (f3 & b1.y) | (f4 & c1.x) | (f5 & c1.y);
 							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
 							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
-							n_test += keep1 ? 2 : 1;
 							if (h0) { ++n_hev; hit_acc += (uint32_t)__popcll(h0); sink.add(h0, t * 64 + j0, q0 + t * 64 + j0, b, kv, lane); }
 							if (h1) { ++n_hev; hit_acc += (uint32_t)__popcll(h1); sink.add(h1, t * 64 + j1, q0 + t * 64 + j1, b, kv, lane); }
 						}
 					} else {
+						// candidate query words through scalar loads, two per iteration
 						const uint32_t * qbase = a.qinv + (size_t)(q0 + t * 64) * 8;
 						while (c) {
-							// up to four candidates, tested as independent chains;
-							// missing ones repeat j0 and are masked out afterwards
 							const uint32_t j0 = (uint32_t)__builtin_ctzll(c);
 							c &= c - 1;
-							const uint64_t k1 = c ? ~0ULL : 0ULL;
+							const uint64_t keep1 = c ? ~0ULL : 0ULL;
 							const uint32_t j1 = c ? (uint32_t)__builtin_ctzll(c) : j0;
 							c &= c - 1;
-							const uint64_t k2 = c ? ~0ULL : 0ULL;
-							const uint32_t j2 = c ? (uint32_t)__builtin_ctzll(c) : j0;
-							c &= c - 1;
-							const uint64_t k3 = c ? ~0ULL : 0ULL;
-							const uint32_t j3 = c ? (uint32_t)__builtin_ctzll(c) : j0;
-							c &= c - 1;
-							const uint4 x0 = sld4(qbase + j0 * 8), x1 = sld4(qbase + j1 * 8);
-							const uint4 x2 = sld4(qbase + j2 * 8), x3 = sld4(qbase + j3 * 8);
-							const uint2 y0 = sld2(qbase + j0 * 8 + 4), y1 = sld2(qbase + j1 * 8 + 4);
-							const uint2 y2 = sld2(qbase + j2 * 8 + 4), y3 = sld2(qbase + j3 * 8 + 4);
-							const uint32_t acc0 = (f0 & x0.x) | (f1 & x0.y) | (f2 & x0.z) | (f3 & x0.w) | (f4 & y0.x) | (f5 & y0.y);
-							const uint32_t acc1 = (f0 & x1.x) | (f1 & x1.y) | (f2 & x1.z) | (f3 & x1.w) | (f4 & y1.x) | (f5 & y1.y);
-							const uint32_t acc2 = (f0 & x2.x) | (f1 & x2.y) | (f2 & x2.z) | (f3 & x2.w) | (f4 & y2.x) | (f5 & y2.y);
-							const uint32_t acc3 = (f0 & x3.x) | (f1 & x3.y) | (f2 & x3.z) | (f3 & x3.w) | (f4 & y3.x) | (f5 & y3.y);
-							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
-							const uint64_t h1 = __ballot(acc1 == 0) & lmask & k1;
-							const uint64_t h2 = __ballot(acc2 == 0) & lmask & k2;
-							const uint64_t h3 = __ballot(acc3 == 0) & lmask & k3;
-							n_test += 1 + (k1 ? 1 : 0) + (k2 ? 1 : 0) + (k3 ? 1 : 0);
-							if (h0 | h1 | h2 | h3) {
-								if (h0) { ++n_hev; hit_acc += (uint32_t)__popcll(h0); sink.add(h0, t * 64 + j0, q0 + t * 64 + j0, b, kv, lane); }
-								if (h1) { ++n_hev; hit_acc += (uint32_t)__popcll(h1); sink.add(h1, t * 64 + j1, q0 + t * 64 + j1, b, kv, lane); }
-								if (h2) { ++n_hev; hit_acc += (uint32_t)__popcll(h2); sink.add(h2, t * 64 + j2, q0 + t * 64 + j2, b, kv, lane); }
-								if (h3) { ++n_hev; hit_acc += (uint32_t)__popcll(h3); sink.add(h3, t * 64 + j3, q0 + t * 64 + j3, b, kv, lane); }
+							n_test += keep1 ? 2 : 1;
+							const uint2 a0 = sld2(qbase + j0 * 8), a1 = sld2(qbase + j1 * 8);
+							uint32_t acc0 = (f0 & a0.x) | (f1 & a0.y);
+							uint32_t acc1 = (f0 & a1.x) | (f1 & a1.y);
+							if (VAR & 4) {
+								const uint64_t alive = (__ballot(acc0 == 0) | (__ballot(acc1 == 0) & keep1)) & lmask;
+								if (alive == 0) continue;
 							}
+							const uint4 b0 = sld4(qbase + j0 * 8 + 2), b1 = sld4(qbase + j1 * 8 + 2);
+							acc0 |= (f2 & b0.x) | (f3 & b0.y) | (f4 & b0.z) | (f5 & b0.w);
+							acc1 |= (f2 & b1.x) | (f3 & b1.y) | (f4 & b1.z) | (f5 & b1.w);
+							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
+							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
+							if (h0) { ++n_hev; hit_acc += (uint32_t)__popcll(h0); sink.add(h0, t * 64 + j0, q0 + t * 64 + j0, b, kv, lane); }
+							if (h1) { ++n_hev; hit_acc += (uint32_t)__popcll(h1); sink.add(h1, t * 64 + j1, q0 + t * 64 + j1, b, kv, lane); }
 						}
 					}
 				}
@@ -903,6 +916,10 @@ mq_fn pick_match_q(int mode, bool lds_bm, int tiles, int nw, int var) {
 	case 1: return pick_q_mode<1>(mode, lds_bm, tiles, nw);
 	case 2: return pick_q_mode<2>(mode, lds_bm, tiles, nw);
 	case 3: return pick_q_mode<3>(mode, lds_bm, tiles, nw);
+	case 4: return pick_q_mode<4>(mode, lds_bm, tiles, nw);
+	case 5: return pick_q_mode<5>(mode, lds_bm, tiles, nw);
+	case 6: return pick_q_mode<6>(mode, lds_bm, tiles, nw);
+	case 7: return pick_q_mode<7>(mode, lds_bm, tiles, nw);
 	default: return pick_q_mode<0>(mode, lds_bm, tiles, nw);
 	}
 }
@@ -948,6 +965,22 @@ struct tm_ring {
 	uint64_t * h_stat = nullptr;   // pinned, 8
 	uint32_t * d_log = nullptr;
 	uint64_t log_cap = 0;
+	// plain unit log: per launch (partition, batch, queries) and features
+	uint4 * d_units = nullptr;
+	float * d_ufeat = nullptr;
+	uint64_t units_cap = 0;
+	uint32_t n_units = 0;
+
+	// cost model (persistent schedule)
+	tm_mlp * model = nullptr;
+	uint32_t model_flags = 0;
+	float * d_tfeat = nullptr;     // (tile, partition) feature rows
+	float * d_tpred = nullptr;     // predictions, 2 per row
+	float * d_tcost = nullptr;     // per tile: predicted ticks
+	float * d_thits = nullptr;     // per tile: predicted hits
+	uint32_t * d_iorder = nullptr; // per work item: query tile
+	uint64_t trows_cap = 0, tiles_cap = 0;
+	bool use_iorder = false;
 
 	// effective launch parameters
 	int tiles = 4, wgs_per_cu = 2, waves = 8, variant = 1;
@@ -982,6 +1015,9 @@ void tm_ring::release() {
 	dfree(d_sched); dfree(d_iprefix); dfree(d_qbeg); dfree(d_qnext); dfree(d_porder);
 	dfree(d_gbm); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
 	dfree(d_vals); dfree(d_cnt); dfree(d_scan); dfree(d_stat); dfree(d_log);
+	dfree(d_units); dfree(d_ufeat);
+	dfree(d_tfeat); dfree(d_tpred); dfree(d_tcost); dfree(d_thits); dfree(d_iorder);
+	units_cap = trows_cap = tiles_cap = 0;
 	if (e0) (void)hipEventDestroy(e0);
 	if (em0) (void)hipEventDestroy(em0);
 	if (em1) (void)hipEventDestroy(em1);
@@ -1020,7 +1056,7 @@ int resolve_params(tm_ring * r) {
 		r->stage = (mode_of(c) == 0) && lds <= STAGE_LDS_MAX && lds + static_lds <= max_lds;
 		r->stage_lds = r->stage ? lds : 0;
 	} else {
-		r->variant = c.variant <= 3 ? (int)c.variant : 2;
+		r->variant = c.variant <= 7 ? (int)c.variant : 6;
 		int nw = c.waves_per_group ? (int)c.waves_per_group : 8;
 		if (nw != 4 && nw != 8 && nw != 16)
 			return set_error(TM_ERR_INVALID, "waves_per_group must be 4, 8 or 16 (got %d)", nw);
@@ -1136,6 +1172,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.q = r->d_q;
 	a.nq = nq;
 	a.porder = r->have_order ? r->d_porder : nullptr;
+	a.iorder = r->use_iorder ? r->d_iorder : nullptr;
 	a.qnext = r->d_qnext;
 	a.qbeg = r->d_qbeg;
 	a.pcount = r->d_pcount;
@@ -1175,6 +1212,35 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 	}
 	match_args a = base_args(r);
 	match_fn fn = pick_match(mode_of(r->cfg), r->stage, r->tiles);
+	const uint32_t bq = 64 * r->tiles;
+	r->n_units = 0;
+	if (r->cfg.log_units) {
+		// one log row per launch: (partition, batch, queries in the batch)
+		std::vector<uint4> units;
+		for (uint32_t k = 0; k < r->P; ++k) {
+			const uint4 sc = sched[k];
+			if (sc.z == 0) continue;
+			for (uint32_t b = 0; b < sc.y; ++b)
+				units.push_back(make_uint4(sc.x, b, std::min(bq, sc.w - b * bq), 0));
+		}
+		const uint32_t nu = (uint32_t)std::min<uint64_t>(units.size(), r->log_cap);
+		if (nu > r->units_cap) {
+			dfree(r->d_units);
+			dfree(r->d_ufeat);
+			r->units_cap = 0;
+			TM_CHECK(dalloc(&r->d_units, nu));
+			TM_CHECK(dalloc(&r->d_ufeat, (size_t)nu * N_FEAT));
+			r->units_cap = nu;
+		}
+		if (nu) {
+			TM_HIP(hipMemcpyAsync(r->d_units, units.data(), sizeof(uint4) * nu, hipMemcpyHostToDevice, r->s));
+			fill_u32(r->d_log, 0, (uint64_t)nu * 2, r->s);
+			launch_unit_features_plain(r->fib->view(), r->d_q, r->d_list, r->d_pbase, r->d_units, nu, bq,
+									   r->d_ufeat, r->s);
+			TM_HIP(hipStreamSynchronize(r->s));   // the unit vector is pageable host memory
+		}
+		r->n_units = nu;
+	}
 	uint32_t launches = 0;
 	TM_HIP(hipEventRecord(r->em0, r->s));
 	for (uint32_t k = 0; k < r->P; ++k) {
@@ -1183,6 +1249,7 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 		for (uint32_t b = 0; b < sc.y; ++b) {
 			a.plain_rank = (int32_t)k;
 			a.plain_batch = b;
+			a.plain_unit = launches;
 			hipLaunchKernelGGL(fn, dim3(sc.z), dim3(WG), r->stage_lds, r->s, a);
 			++launches;
 		}
@@ -1275,11 +1342,129 @@ int enqueue_step(tm_ring * r, uint32_t nq, bool allow_graph) {
 	return TM_OK;
 }
 
+void drop_graph(tm_ring * r) {
+	if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+	r->gexec = nullptr;
+	r->graph_ok = false;
+}
+
+// Price the submission's (query tile, partition) units with the cost model:
+// order tiles longest-first (dealt round-robin over the NQUEUE queues so
+// every queue starts with heavy tiles) and grow result buffers to the
+// predicted volume.  Predictions never remove work: an under-estimate is
+// caught by the overflow retry in tm_wait.
+int price_submission(tm_ring * r, uint32_t nq) {
+	const bool want = r->model && r->cfg.schedule == TM_SCHED_PERSISTENT && nq > 0 && r->P > 0;
+	const bool had_order = r->use_iorder;
+	r->use_iorder = false;
+	if (!want) {
+		if (had_order) drop_graph(r);
+		return TM_OK;
+	}
+	const auto t0 = std::chrono::steady_clock::now();
+	hipStream_t s = r->s;
+	const uint32_t nqt = 64 * r->tiles;
+	const uint32_t n_tiles = (nq + nqt - 1) / nqt;
+	const uint64_t rows = (uint64_t)n_tiles * r->P;
+	if (rows > 0xFFFFFFFFull) return set_error(TM_ERR_INVALID, "too many cost model rows");
+	if (rows > r->trows_cap) {
+		dfree(r->d_tfeat);
+		dfree(r->d_tpred);
+		r->trows_cap = 0;
+		TM_CHECK(dalloc(&r->d_tfeat, rows * N_FEAT));
+		TM_CHECK(dalloc(&r->d_tpred, rows * N_TARGET));
+		r->trows_cap = rows;
+	}
+	if (n_tiles > r->tiles_cap) {
+		const bool graph_uses = r->d_iorder != nullptr;
+		dfree(r->d_tcost);
+		dfree(r->d_thits);
+		dfree(r->d_iorder);
+		r->tiles_cap = 0;
+		TM_CHECK(dalloc(&r->d_tcost, n_tiles));
+		TM_CHECK(dalloc(&r->d_thits, n_tiles));
+		TM_CHECK(dalloc(&r->d_iorder, n_tiles));
+		r->tiles_cap = n_tiles;
+		if (graph_uses) drop_graph(r);
+	}
+	launch_unit_features_tiles(r->fib->view(), r->d_q, nq, nqt, n_tiles, r->d_tfeat, s);
+	TM_CHECK(mlp_infer(r->model, r->d_tfeat, (uint32_t)rows, r->d_tpred, s));
+	launch_tile_costs(r->d_tfeat, r->d_tpred, n_tiles, r->P, r->d_tcost, r->d_thits, s);
+	TM_HIP(hipGetLastError());
+	std::vector<float> cost(n_tiles), hits(n_tiles);
+	TM_HIP(hipMemcpyAsync(cost.data(), r->d_tcost, 4ull * n_tiles, hipMemcpyDeviceToHost, s));
+	TM_HIP(hipMemcpyAsync(hits.data(), r->d_thits, 4ull * n_tiles, hipMemcpyDeviceToHost, s));
+	TM_HIP(hipStreamSynchronize(s));
+	double total = 0.0;
+	for (uint32_t t = 0; t < n_tiles; ++t) total += hits[t];
+	r->st.pred_hits = total;
+
+	if (r->model_flags & TM_COST_ORDER) {
+		std::vector<uint32_t> idx(n_tiles);
+		std::iota(idx.begin(), idx.end(), 0u);
+		std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+		// queue x serves items [n*x/NQUEUE, n*(x+1)/NQUEUE) (k_init_queues)
+		std::vector<uint32_t> order(n_tiles);
+		std::vector<uint64_t> fill(NQUEUE), end(NQUEUE);
+		for (int x = 0; x < NQUEUE; ++x) {
+			fill[x] = (uint64_t)n_tiles * x / NQUEUE;
+			end[x] = (uint64_t)n_tiles * (x + 1) / NQUEUE;
+		}
+		int x = 0;
+		for (uint32_t k = 0; k < n_tiles; ++k) {
+			while (fill[x] >= end[x]) x = (x + 1) % NQUEUE;
+			order[fill[x]++] = idx[k];
+			x = (x + 1) % NQUEUE;
+		}
+		TM_HIP(hipMemcpyAsync(r->d_iorder, order.data(), 4ull * n_tiles, hipMemcpyHostToDevice, s));
+		TM_HIP(hipStreamSynchronize(s));   // `order` is pageable and local
+		r->use_iorder = true;
+	}
+	if (r->use_iorder != had_order) drop_graph(r);
+
+	if (r->model_flags & TM_COST_PRESIZE) {
+		const tm_fib * f = r->fib;
+		const double kpf = f->n_filters ? (double)f->n_key_instances / (double)f->n_filters : 1.0;
+		const double margin = 1.25;   // predictions are fitted in log space: allow for spread
+		uint64_t slots = 0, results = 0;
+		if (r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
+			// expected distinct keys of h hits with kpf uniformly drawn keys
+			// each, from the tile's mean hits per query (concave in h, so
+			// the tile mean over-estimates: a safe direction)
+			const double K = std::max<double>(f->n_keys, 1.0);
+			double est = 0.0;
+			for (uint32_t t = 0; t < n_tiles; ++t) {
+				const uint32_t qn = std::min<uint32_t>(nqt, nq - t * nqt);
+				const double h = hits[t] / std::max<uint32_t>(qn, 1);
+				est += qn * K * (1.0 - std::exp(-h * kpf / K));
+			}
+			results = (uint64_t)(est * margin) + 4096;
+		} else {
+			slots = (uint64_t)(total * margin) + 4096;
+			results = r->cfg.mode == TM_MODE_KEYS_ALL ? (uint64_t)(total * kpf * margin) + 4096 : slots;
+			if (slots > r->hit_cap) {
+				dfree(r->d_hits);
+				r->hit_cap = 0;
+				TM_CHECK(dalloc(&r->d_hits, slots));
+				r->hit_cap = slots;
+				drop_graph(r);
+			}
+		}
+		if (results > r->val_cap) {
+			TM_CHECK(grow_u32(&r->d_vals, r->val_cap, results));
+			drop_graph(r);
+		}
+	}
+	r->st.model_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+	return TM_OK;
+}
+
 int submit_common(tm_ring * r, uint32_t nq) {
 	r->st = tm_status();
 	r->last_nq = nq;
 	r->retries = 0;
 	TM_CHECK(ensure_capacity(r));
+	TM_CHECK(price_submission(r, nq));
 	int rc = enqueue_step(r, nq, true);
 	if (rc != TM_OK) return rc;
 	r->pending = true;
@@ -1471,6 +1656,47 @@ int tm_ring_results_device(tm_ring * r, const uint64_t ** off, const uint32_t **
 	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
 	*off = r->d_off;
 	*vals = r->d_vals;
+	return TM_OK;
+}
+
+int tm_ring_unit_log(tm_ring * r, float * features, uint32_t * hits, uint32_t * ticks, uint32_t * units,
+					 uint32_t capacity, uint32_t * n_units) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
+	if (n_units) *n_units = r->n_units;
+	const uint32_t n = std::min(capacity, r->n_units);
+	if (n == 0) return TM_OK;
+	TM_CHECK(require_device());
+	if (features)
+		TM_HIP(hipMemcpy(features, r->d_ufeat, (size_t)n * N_FEAT * sizeof(float), hipMemcpyDeviceToHost));
+	if (units) {
+		std::vector<uint4> u(n);
+		TM_HIP(hipMemcpy(u.data(), r->d_units, (size_t)n * sizeof(uint4), hipMemcpyDeviceToHost));
+		for (uint32_t i = 0; i < n; ++i) {
+			units[(size_t)i * 3] = u[i].x;
+			units[(size_t)i * 3 + 1] = u[i].y;
+			units[(size_t)i * 3 + 2] = u[i].z;
+		}
+	}
+	if (hits || ticks) {
+		std::vector<uint32_t> log((size_t)n * 2);
+		TM_HIP(hipMemcpy(log.data(), r->d_log, (size_t)n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+		for (uint32_t i = 0; i < n; ++i) {
+			if (hits) hits[i] = log[(size_t)i * 2];
+			if (ticks) ticks[i] = log[(size_t)i * 2 + 1];
+		}
+	}
+	return TM_OK;
+}
+
+int tm_ring_set_cost_model(tm_ring * r, tm_mlp * m, uint32_t flags) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
+	if (m && !mlp_fused_shape(m))
+		return set_error(TM_ERR_INVALID, "cost model must be %d-64-64-%d", N_FEAT, N_TARGET);
+	if (flags & ~(TM_COST_PRESIZE | TM_COST_ORDER)) return set_error(TM_ERR_INVALID, "unknown cost model flags");
+	r->model = m;
+	r->model_flags = m ? flags : 0;
 	return TM_OK;
 }
 
