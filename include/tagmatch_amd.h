@@ -127,6 +127,28 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 				  const uint32_t * filter_ids);
 int tm_fib_stats_get(const tm_fib * fib, tm_fib_stats * st);
 
+/* Spare blocks reserved per partition at upload for tm_fib_insert, in
+ * percent of its used blocks (plus one block); default 13, 0 = none.
+ * Applies to the next tm_fib_upload. */
+int tm_fib_set_reserve(tm_fib * fib, uint32_t percent);
+
+/* Insert n filters into a resident table without rebuilding it.  The
+ * reference adds filters on the host and rebuilds the whole FIB
+ * (tagmatch::add + consolidate, tagmatch.cc:22-77, :98-165, then
+ * back_end::compile_fibs, back_end.cc:404-494).  Here each filter joins the
+ * existing partition with the most mask bits among those whose mask it
+ * contains and that have a spare lane; filters that fit none are
+ * partitioned on their own with maxp and appended as new partitions.  Keys
+ * new to the table extend its dictionary.  Filter ids default to the
+ * running count of filters (the first upload's n, then n + each insert).
+ * A filter identical to one already present is kept as its own entry: key
+ * results are the same as after merging, TM_MODE_HITS reports both ids.
+ * Returns TM_ERR_BUSY while a ring on this FIB has a submission that
+ * tm_wait has not collected; rings pick the change up at their next
+ * submission. */
+int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+				  const uint32_t * keys, const uint32_t * filter_ids, uint32_t maxp);
+
 /* ------------------------------------------------------------------------ */
 /* Matching.
  *
@@ -291,6 +313,86 @@ int tm_ring_unit_log(tm_ring * ring, float * features, uint32_t * hits, uint32_t
 #define TM_COST_PRESIZE 1u  /* grow result buffers to the predicted size first */
 #define TM_COST_ORDER   2u  /* visit query tiles longest-predicted first        */
 int tm_ring_set_cost_model(tm_ring * ring, tm_mlp * model, uint32_t flags);
+
+/* ------------------------------------------------------------------------ */
+/* Serving loop.  Replaces the reference front end's batching and latency
+ * flush (front_end.cc:249-267, :350-376, :563-598) and its per-GPU double
+ * buffering (back_end.cc:172-343) with continuous batching: an engine thread
+ * admits waiting requests (batches of queries) into free slots (rings, each
+ * on its own HIP stream) as soon as a slot is free and the requests' result
+ * bytes fit the HBM budget, so steps of several slots overlap on the
+ * device.  A request is never split: its queries run in one device step,
+ * possibly together with other requests.  Inserts queue in the same order
+ * as requests and are applied when every earlier request has completed
+ * (tm_fib_insert), so each request sees exactly the inserts submitted
+ * before it.  No counterpart in the reference for the budget or inserts:
+ * its add/consolidate run only while the engine is stopped. */
+typedef struct tm_server tm_server;
+
+typedef struct tm_server_config {
+	uint32_t n_slots;        /* rings in flight (0 = 4)                       */
+	uint32_t slot_queries;   /* queries per device step and largest request
+	                            (0 = 16384)                                   */
+	uint64_t hbm_budget;     /* result bytes admitted at once (0 = a quarter
+	                            of the free HBM at creation)                  */
+	uint32_t reserved[8];
+} tm_server_config;
+
+typedef struct tm_server_stats {
+	uint64_t n_requests;          /* submitted                               */
+	uint64_t n_requests_done;
+	uint64_t n_queries_done;
+	uint64_t n_batches;           /* device steps started                    */
+	uint64_t n_steps;             /* device steps completed without error    */
+	uint64_t n_retries;           /* steps that needed an overflow retry     */
+	uint64_t n_inserts;
+	uint64_t n_budget_waits;      /* admissions deferred by the HBM budget   */
+	uint64_t requests_per_batch_sum;
+	uint64_t reserved_peak;       /* largest budget use, bytes               */
+	uint64_t reserved_now;
+	uint64_t hbm_budget;
+	uint64_t n_waiting;           /* requests and inserts not yet admitted   */
+	double   latency_sum_ms;      /* submit -> results on the host           */
+	double   latency_max_ms;
+	double   device_ms_sum;       /* device time of the completed steps      */
+} tm_server_stats;
+
+typedef struct tm_request_info {
+	int32_t  status;              /* TM_* of the request                     */
+	uint32_t n_queries;
+	uint64_t n_values;            /* result values (offsets has n_queries+1) */
+	uint32_t step_queries;        /* queries of the device step it ran in    */
+	uint32_t pad;
+	double   latency_ms;          /* submit -> results on the host           */
+	double   queue_ms;            /* submit -> admitted                      */
+} tm_request_info;
+
+void tm_server_config_default(tm_server_config * cfg);
+/* mcfg: match configuration of every slot (NULL: tm_match_config_default). */
+int tm_server_create(tm_fib * fib, const tm_server_config * cfg, const tm_match_config * mcfg, tm_server ** out);
+/* Drains every queued request and insert, then stops. */
+int tm_server_destroy(tm_server * srv);
+/* Price the slots' steps with a cost model (tm_ring_set_cost_model); waits
+ * until the server is idle. */
+int tm_server_set_cost_model(tm_server * srv, tm_mlp * model, uint32_t flags);
+/* Queue a request of nq queries (copied); returns at once with its id. */
+int tm_server_submit(tm_server * srv, const uint32_t * queries, uint32_t nq, uint64_t * id);
+/* Queue an insert (arguments as tm_fib_insert, copied); completes like a
+ * request, with tm_server_result's status = tm_fib_insert's. */
+int tm_server_insert(tm_server * srv, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+					 const uint32_t * keys, const uint32_t * filter_ids, uint32_t maxp, uint64_t * id);
+/* Block until request id completed (timeout_ms < 0: no limit); returns its
+ * status, or TM_ERR_BUSY on timeout. */
+int tm_server_wait(tm_server * srv, uint64_t id, double timeout_ms);
+/* Ids of requests completed since the last call, oldest first. */
+int tm_server_collect(tm_server * srv, uint64_t * ids, uint32_t capacity, uint32_t * n);
+/* Results of a completed request: offsets (n_queries + 1) and values as in
+ * tm_read_results; any output may be NULL. */
+int tm_server_result(tm_server * srv, uint64_t id, tm_request_info * info, uint64_t * offsets, uint32_t * values,
+					 uint64_t capacity);
+/* Forget a completed request. */
+int tm_server_release(tm_server * srv, uint64_t id);
+int tm_server_stats_get(tm_server * srv, tm_server_stats * st);
 
 /* ------------------------------------------------------------------------ */
 /* Host: canonical filter table from (filter, keys) entries, as

@@ -77,6 +77,24 @@ class FibStats(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("n_slots", C.c_uint32), ("slot_queries", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("reserved", C.c_uint32 * 8)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in (
+        "n_requests", "n_requests_done", "n_queries_done", "n_batches", "n_steps", "n_retries", "n_inserts",
+        "n_budget_waits", "requests_per_batch_sum", "reserved_peak", "reserved_now", "hbm_budget", "n_waiting")] + [
+        ("latency_sum_ms", C.c_double), ("latency_max_ms", C.c_double), ("device_ms_sum", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_queries", C.c_uint32), ("n_values", C.c_uint64),
+                ("step_queries", C.c_uint32), ("pad", C.c_uint32), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double)]
+
+
 P = C.c_void_p
 U32P = C.POINTER(C.c_uint32)
 U64P = C.POINTER(C.c_uint64)
@@ -95,6 +113,8 @@ _SIGS = {
     "tm_fib_destroy": (C.c_int, [P]),
     "tm_fib_upload": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32, P, P, P]),
     "tm_fib_stats_get": (C.c_int, [P, C.POINTER(FibStats)]),
+    "tm_fib_set_reserve": (C.c_int, [P, C.c_uint32]),
+    "tm_fib_insert": (C.c_int, [P, P, C.c_uint32, P, P, P, C.c_uint32]),
     "tm_match_config_default": (None, [C.POINTER(MatchConfig)]),
     "tm_ring_create": (C.c_int, [P, C.c_uint32, C.POINTER(MatchConfig), C.POINTER(P)]),
     "tm_ring_destroy": (C.c_int, [P]),
@@ -119,6 +139,17 @@ _SIGS = {
                                C.POINTER(C.c_float)]),
     "tm_ring_unit_log": (C.c_int, [P, P, P, P, P, C.c_uint32, U32P]),
     "tm_ring_set_cost_model": (C.c_int, [P, P, C.c_uint32]),
+    "tm_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "tm_server_create": (C.c_int, [P, C.POINTER(ServerConfig), C.POINTER(MatchConfig), C.POINTER(P)]),
+    "tm_server_destroy": (C.c_int, [P]),
+    "tm_server_set_cost_model": (C.c_int, [P, P, C.c_uint32]),
+    "tm_server_submit": (C.c_int, [P, P, C.c_uint32, U64P]),
+    "tm_server_insert": (C.c_int, [P, P, C.c_uint32, P, P, P, C.c_uint32, U64P]),
+    "tm_server_wait": (C.c_int, [P, C.c_uint64, C.c_double]),
+    "tm_server_collect": (C.c_int, [P, P, C.c_uint32, U32P]),
+    "tm_server_result": (C.c_int, [P, C.c_uint64, C.POINTER(RequestInfo), P, P, C.c_uint64]),
+    "tm_server_release": (C.c_int, [P, C.c_uint64]),
+    "tm_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
     "tm_canonicalize": (C.c_int, [P, C.c_uint64, P, P, C.POINTER(P), U64P, C.POINTER(P), C.POINTER(P)]),
     "tm_free": (None, [P]),
     "tm_workload_default_params": (None, [C.POINTER(WorkloadParams)]),
@@ -288,9 +319,11 @@ def partition(words, maxp, max_parts=None):
 class Fib:
     """Device FIB (tm_fib)."""
 
-    def __init__(self, words, part_offsets, part_masks, key_off, keys, filter_ids=None):
+    def __init__(self, words, part_offsets, part_masks, key_off, keys, filter_ids=None, reserve=None):
         self.h = C.c_void_p()
         check(load().tm_fib_create(C.byref(self.h)), "tm_fib_create")
+        if reserve is not None:
+            self.set_reserve(reserve)
         self.upload(words, part_offsets, part_masks, key_off, keys, filter_ids)
 
     def upload(self, words, part_offsets, part_masks, key_off, keys, filter_ids=None):
@@ -303,6 +336,19 @@ class Fib:
         n = words.shape[0]
         check(load().tm_fib_upload(self.h, _ptr(words), n, _ptr(po), _ptr(pm), len(po) - 1, _ptr(ko), _ptr(kv),
                                    _ptr(fid)), "tm_fib_upload")
+
+    def set_reserve(self, percent):
+        """Spare blocks per partition (percent) for the next upload."""
+        check(load().tm_fib_set_reserve(self.h, percent), "tm_fib_set_reserve")
+
+    def insert(self, words, key_off, keys, filter_ids=None, maxp=256 * 1024):
+        """Insert filters into the resident table (tm_fib_insert)."""
+        words = np.ascontiguousarray(words, np.uint32).reshape(-1, 6)
+        ko = np.ascontiguousarray(key_off, np.uint32)
+        kv = np.ascontiguousarray(keys, np.uint32)
+        fid = None if filter_ids is None else np.ascontiguousarray(filter_ids, np.uint32)
+        check(load().tm_fib_insert(self.h, _ptr(words), words.shape[0], _ptr(ko), _ptr(kv), _ptr(fid), maxp),
+              "tm_fib_insert")
 
     def stats(self):
         s = FibStats()
@@ -321,7 +367,7 @@ class Fib:
             pass
 
 
-def build_fib(words, key_off, keys, maxp):
+def build_fib(words, key_off, keys, maxp, reserve=None):
     """Partition a canonical table on the GPU and upload it.
 
     Returns (fib, perm, part_offsets, masks); filter ids reported in HITS
@@ -334,7 +380,7 @@ def build_fib(words, key_off, keys, maxp):
     starts = key_off[:-1][perm].astype(np.int64)
     idx = np.repeat(starts - ko[:-1].astype(np.int64), cnt) + np.arange(int(ko[-1]), dtype=np.int64)
     pk = keys[idx] if len(idx) else np.zeros(0, np.uint32)
-    fib = Fib(pw, off, masks, ko.astype(np.uint32), pk, perm.astype(np.uint32))
+    fib = Fib(pw, off, masks, ko.astype(np.uint32), pk, perm.astype(np.uint32), reserve=reserve)
     return fib, perm, off, masks
 
 
@@ -448,6 +494,92 @@ def match(fib, queries, mode=MODE_KEYS_UNIQUE, schedule=SCHED_PERSISTENT, **kw):
     finally:
         r.close()
     return off, vals, st.as_dict()
+
+
+# ------------------------------------------------------------------ serving
+
+class Server:
+    """Continuous-batching serving loop (tm_server)."""
+
+    def __init__(self, fib, n_slots=4, slot_queries=16384, hbm_budget=0, cfg=None, **kw):
+        sc = ServerConfig()
+        load().tm_server_config_default(C.byref(sc))
+        sc.n_slots = n_slots
+        sc.slot_queries = slot_queries
+        sc.hbm_budget = hbm_budget
+        self.cfg = cfg if cfg is not None else config(**kw)
+        self.fib = fib
+        self.h = C.c_void_p()
+        check(load().tm_server_create(fib.h, C.byref(sc), C.byref(self.cfg), C.byref(self.h)), "tm_server_create")
+
+    def submit(self, queries):
+        q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 6)
+        rid = C.c_uint64()
+        check(load().tm_server_submit(self.h, _ptr(q), q.shape[0], C.byref(rid)), "tm_server_submit")
+        return rid.value
+
+    def insert(self, words, key_off, keys, filter_ids=None, maxp=256 * 1024):
+        w = np.ascontiguousarray(words, np.uint32).reshape(-1, 6)
+        ko = np.ascontiguousarray(key_off, np.uint32)
+        kv = np.ascontiguousarray(keys, np.uint32)
+        fid = None if filter_ids is None else np.ascontiguousarray(filter_ids, np.uint32)
+        rid = C.c_uint64()
+        check(load().tm_server_insert(self.h, _ptr(w), w.shape[0], _ptr(ko), _ptr(kv), _ptr(fid), maxp,
+                                      C.byref(rid)), "tm_server_insert")
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1.0):
+        rc = load().tm_server_wait(self.h, rid, timeout_ms)
+        if rc == TM_ERR_BUSY:
+            return False
+        check(rc, "tm_server_wait")
+        return True
+
+    def collect(self, cap=65536):
+        ids = np.zeros(cap, np.uint64)
+        n = C.c_uint32()
+        check(load().tm_server_collect(self.h, _ptr(ids), cap, C.byref(n)), "tm_server_collect")
+        return ids[:n.value]
+
+    def info(self, rid):
+        inf = RequestInfo()
+        check(load().tm_server_result(self.h, rid, C.byref(inf), None, None, 0), "tm_server_result")
+        return inf
+
+    def result(self, rid, release=True):
+        """(info, offsets, values) of a completed request."""
+        inf = self.info(rid)
+        off = np.zeros(inf.n_queries + 1, np.uint64)
+        vals = np.zeros(inf.n_values, np.uint32)
+        check(load().tm_server_result(self.h, rid, None, _ptr(off), _ptr(vals), vals.size), "tm_server_result")
+        if release:
+            self.release(rid)
+        return inf, off, vals
+
+    def release(self, rid):
+        check(load().tm_server_release(self.h, rid), "tm_server_release")
+
+    def set_cost_model(self, mlp, flags=None):
+        if flags is None:
+            flags = COST_PRESIZE | COST_ORDER
+        check(load().tm_server_set_cost_model(self.h, mlp.h if mlp is not None else None,
+                                              flags if mlp is not None else 0), "tm_server_set_cost_model")
+
+    def stats(self):
+        st = ServerStats()
+        check(load().tm_server_stats_get(self.h, C.byref(st)), "tm_server_stats_get")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            load().tm_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ cost model

@@ -4,6 +4,7 @@
 
 #include "hip_util.h"
 
+#include <atomic>
 #include <vector>
 
 namespace tmx {
@@ -58,9 +59,18 @@ struct tm_fib {
 	uint32_t * part_stat = nullptr;
 	uint32_t n_parts = 0, n_blocks = 0, n_keys = 0;
 	uint64_t n_filters = 0, n_key_instances = 0, device_bytes = 0, max_part = 0;
-	// host copies used by the schedulers
-	std::vector<uint32_t> h_part_info;   // P x 8
+	// host copies used by the schedulers and by tm_fib_insert
+	std::vector<uint32_t> h_part_info;   // P x 8 (as part_info)
 	std::vector<uint32_t> h_part_size;   // P, filters per partition
+	std::vector<uint32_t> h_part_stat;   // P x 8 (as part_stat)
+	std::vector<uint32_t> h_part_cap;    // P, blocks reserved (used + spare)
+	std::vector<uint32_t> h_ovf;         // overflow key lists
+	std::vector<uint32_t> h_dict;        // sorted distinct keys
+	std::vector<int> h_order;            // bit significance order of the members
+	size_t ovf_cap = 0;                  // device words of ovf
+	uint32_t reserve_pct = 13;           // spare blocks per partition, percent (+1)
+	uint64_t version = 0;                // bumped when P, K or device arrays change
+	std::atomic<int> in_flight{0};       // ring submissions not yet waited for
 
 	tmx::dev_fib_view view() const {
 		tmx::dev_fib_view v;

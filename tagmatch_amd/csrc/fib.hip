@@ -33,8 +33,14 @@ void tm_fib::release() {
 	dfree(part_stat);
 	n_parts = n_blocks = n_keys = 0;
 	n_filters = n_key_instances = device_bytes = max_part = 0;
+	ovf_cap = 0;
 	h_part_info.clear();
 	h_part_size.clear();
+	h_part_stat.clear();
+	h_part_cap.clear();
+	h_ovf.clear();
+	h_dict.clear();
+	h_order.clear();
 }
 
 namespace {
@@ -62,6 +68,199 @@ int upload(T ** dst, const std::vector<T> & src, uint64_t & bytes) {
 	if (!src.empty()) TM_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
 	bytes += std::max<size_t>(1, src.size()) * sizeof(T);
 	return TM_OK;
+}
+
+// Host side of a block range under construction: partitions are laid out
+// one after another from block `base`, each followed by its reserve.
+struct block_build {
+	std::vector<uint32_t> words, info, fid, part, stat, cap;
+	std::vector<uint64_t> keys;
+	uint32_t nb = 0;
+};
+
+// Dense key slot of filter i: up to four inline 16-bit ids, else an
+// overflow list (count, ids...) appended to ovf.
+int key_slot(const uint32_t * keys, uint32_t kb, uint32_t ke, const std::vector<uint32_t> & dict, bool inline_ok,
+			 std::vector<uint32_t> & ovf, uint64_t & out) {
+	auto key_id = [&](uint32_t k) -> uint32_t {
+		return (uint32_t)(std::lower_bound(dict.begin(), dict.end(), k) - dict.begin());
+	};
+	const uint32_t kn = ke - kb;
+	if (inline_ok && kn <= 4) {
+		uint64_t v = ~0ULL;
+		for (uint32_t x = 0; x < kn; ++x) {
+			const uint64_t id = key_id(keys[kb + x]);
+			v &= ~(0xFFFFULL << (16 * x));
+			v |= id << (16 * x);
+		}
+		out = v;
+		return TM_OK;
+	}
+	const uint64_t off = ovf.size();
+	if (off > 0xFFFFFFFFULL) return set_error(TM_ERR_INVALID, "overflow key table too large");
+	ovf.push_back(kn);
+	for (uint32_t x = 0; x < kn; ++x) ovf.push_back(key_id(keys[kb + x]));
+	out = 0xFFFFULL | (off << 16);
+	return TM_OK;
+}
+
+uint32_t reserve_blocks(uint32_t nb, uint32_t pct) { return pct ? (nb * pct + 99) / 100 + 1 : 0; }
+
+// Lay out partitions [0, n_parts) of `words` (members of p are
+// part_offsets[p] .. part_offsets[p+1] in partitioned order) as blocks,
+// partition ids starting at p_base, blocks at b_base.  Members are sorted
+// within each partition by the significance order `order` (rarest bit
+// first) so that block AND masks are selective.
+int build_blocks(const uint32_t * words, const uint32_t * part_offsets, const uint32_t * part_masks, uint32_t n_parts,
+				 const uint32_t * key_offsets, const uint32_t * keys, const uint32_t * filter_ids, uint64_t fid_base,
+				 const std::vector<int> & order, const std::vector<uint32_t> & dict, bool inline_ok,
+				 std::vector<uint32_t> & ovf, uint32_t p_base, uint32_t b_base, uint32_t reserve_pct,
+				 block_build & bb) {
+	const uint32_t n = part_offsets[n_parts];
+	std::vector<uint32_t> pblock(n_parts + 1, 0);
+	for (uint32_t p = 0; p < n_parts; ++p) {
+		const uint32_t sz = part_offsets[p + 1] - part_offsets[p];
+		const uint32_t used = (sz + 63) / 64;
+		pblock[p + 1] = pblock[p] + used + reserve_blocks(used, reserve_pct);
+	}
+	const uint32_t nb = pblock[n_parts];
+	std::vector<uint32_t> member(n);  // member[k] = input index at partitioned position k
+	std::iota(member.begin(), member.end(), 0u);
+	parallel_ranges(n_parts, [&](size_t pb, size_t pe) {
+		std::vector<std::array<uint64_t, 3>> key;
+		for (size_t p = pb; p < pe; ++p) {
+			uint32_t b = part_offsets[p], e = part_offsets[p + 1];
+			if (e - b < 2) continue;
+			key.resize(e - b);
+			for (uint32_t i = b; i < e; ++i) {
+				std::array<uint64_t, 3> k{0, 0, 0};
+				const uint32_t * w = words + (size_t)i * 6;
+				for (int r = 0; r < 192; ++r) {
+					int bit = order[r];
+					if ((w[bit >> 5] >> (bit & 31)) & 1)
+						k[r >> 6] |= 1ULL << (63 - (r & 63));
+				}
+				key[i - b] = k;
+			}
+			std::stable_sort(member.begin() + b, member.begin() + e, [&](uint32_t x, uint32_t y) {
+				return key[x - b] < key[y - b];
+			});
+		}
+	});
+
+	bb.nb = nb;
+	bb.words.assign((size_t)nb * BLOCK_WORDS, 0);
+	bb.info.assign((size_t)nb * 8, 0xFFFFFFFFu);
+	bb.keys.assign((size_t)nb * 64, ~0ULL);
+	bb.fid.assign((size_t)nb * 64, 0xFFFFFFFFu);
+	bb.part.assign((size_t)n_parts * 8, 0);
+	bb.stat.assign((size_t)n_parts * 8, 0);
+	bb.cap.assign(n_parts, 0);
+	for (uint32_t p = 0; p < n_parts; ++p) {
+		const uint32_t b0 = part_offsets[p], sz = part_offsets[p + 1] - b0;
+		for (int j = 0; j < 6; ++j) bb.part[(size_t)p * 8 + j] = part_masks[(size_t)p * 6 + j];
+		bb.part[(size_t)p * 8 + 6] = b_base + pblock[p];
+		bb.part[(size_t)p * 8 + 7] = (sz + 63) / 64;
+		bb.cap[p] = pblock[p + 1] - pblock[p];
+		for (uint32_t blk = pblock[p]; blk < pblock[p + 1]; ++blk) {   // empty: nv 0, mask all ones
+			bb.info[(size_t)blk * 8 + 6] = 0;
+			bb.info[(size_t)blk * 8 + 7] = p_base + p;
+		}
+		uint32_t * ps = &bb.stat[(size_t)p * 8];
+		for (uint32_t k = 0; k < sz; ++k) {
+			const uint32_t i = member[b0 + k];
+			const uint32_t blk = pblock[p] + k / 64, lane = k % 64;
+			const uint32_t * w = words + (size_t)i * 6;
+			uint32_t * info = &bb.info[(size_t)blk * 8];
+			for (int j = 0; j < 6; ++j) {
+				bb.words[(size_t)blk * BLOCK_WORDS + j * 64 + lane] = w[j];
+				info[j] &= w[j];
+				ps[j] |= w[j];
+				ps[7] += (uint32_t)__builtin_popcount(w[j]);
+			}
+			info[6] = lane + 1;
+			ps[6] += 1;
+			bb.fid[(size_t)blk * 64 + lane] = filter_ids ? filter_ids[i] : (uint32_t)(fid_base + i);
+			TM_CHECK(key_slot(keys, key_offsets[i], key_offsets[i + 1], dict, inline_ok, ovf,
+							  bb.keys[(size_t)blk * 64 + lane]));
+		}
+	}
+	return TM_OK;
+}
+
+int check_table(const uint32_t * words, uint32_t n, const uint32_t * key_offsets, const uint32_t * keys) {
+	if (n && (!words || !key_offsets || !keys)) return set_error(TM_ERR_INVALID, "NULL input array");
+	if (n == 0) return TM_OK;
+	if (key_offsets[0] != 0) return set_error(TM_ERR_INVALID, "key_offsets[0] must be 0");
+	for (uint32_t i = 0; i < n; ++i)
+		if (key_offsets[i + 1] <= key_offsets[i])
+			return set_error(TM_ERR_INVALID, "filter %u has no keys (key_offsets must increase)", i);
+	return TM_OK;
+}
+
+// Copy `old_n` elements of *p into a new allocation of new_n elements.
+template <typename T>
+int regrow(T ** p, size_t old_n, size_t new_n, hipStream_t s) {
+	T * q = nullptr;
+	TM_CHECK(dalloc(&q, new_n));
+	if (old_n && *p) {
+		hipError_t e = hipMemcpyAsync(q, *p, old_n * sizeof(T), hipMemcpyDeviceToDevice, s);
+		if (e == hipSuccess) e = hipStreamSynchronize(s);
+		if (e != hipSuccess) {
+			dfree(q);
+			return hip_error(e, "regrow copy", __FILE__, __LINE__);
+		}
+	}
+	dfree(*p);
+	*p = q;
+	return TM_OK;
+}
+
+template <typename T>
+int put(T * dst, const std::vector<T> & src, size_t n = SIZE_MAX) {
+	const size_t k = std::min(n, src.size());
+	if (k) TM_HIP(hipMemcpy(dst, src.data(), k * sizeof(T), hipMemcpyHostToDevice));
+	return TM_OK;
+}
+
+// One inserted filter: its lane and content.
+struct ins_rec {
+	uint32_t block, lane, fid, pad;
+	uint32_t w[6];
+	uint64_t kv;
+};
+
+// Writes each inserted filter into its lane and folds it into the block's
+// AND mask and valid-lane count.
+__global__ void __launch_bounds__(256)
+k_insert_lanes(const ins_rec * __restrict__ recs, uint32_t n, uint32_t * __restrict__ blk_words,
+			   uint32_t * __restrict__ blk_info, uint64_t * __restrict__ slot_keys, uint32_t * __restrict__ slot_fid) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n) return;
+	const ins_rec r = recs[i];
+	for (int j = 0; j < 6; ++j) {
+		blk_words[(size_t)r.block * BLOCK_WORDS + j * 64 + r.lane] = r.w[j];
+		atomicAnd(&blk_info[(size_t)r.block * 8 + j], r.w[j]);
+	}
+	atomicMax(&blk_info[(size_t)r.block * 8 + 6], r.lane + 1);
+	slot_keys[(size_t)r.block * 64 + r.lane] = r.kv;
+	slot_fid[(size_t)r.block * 64 + r.lane] = r.fid;
+}
+
+// Dense ids of every slot through `map` (old id -> new id) after the key
+// dictionary grew; overflow-list slots are rewritten on the host.
+__global__ void __launch_bounds__(256)
+k_remap_keys(uint64_t * __restrict__ slot_keys, uint64_t n, const uint32_t * __restrict__ map) {
+	const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+	if (i >= n) return;
+	uint64_t v = slot_keys[i];
+	if ((v & 0xFFFFu) == KEY_NONE) return;   // empty slot or overflow list
+	uint64_t o = 0;
+	for (int x = 0; x < 4; ++x) {
+		const uint64_t id = (v >> (16 * x)) & 0xFFFFu;
+		o |= (id == KEY_NONE ? (uint64_t)KEY_NONE : (uint64_t)map[id]) << (16 * x);
+	}
+	slot_keys[i] = o;
 }
 
 } // namespace
@@ -94,6 +293,13 @@ int tm_fib_stats_get(const tm_fib * fib, tm_fib_stats * st) {
 	return TM_OK;
 }
 
+int tm_fib_set_reserve(tm_fib * fib, uint32_t percent) {
+	if (!fib) return set_error(TM_ERR_INVALID, "fib is NULL");
+	if (percent > 400) return set_error(TM_ERR_INVALID, "reserve %u%% above 400%%", percent);
+	fib->reserve_pct = percent;
+	return TM_OK;
+}
+
 int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 				  const uint32_t * part_offsets, const uint32_t * part_masks, uint32_t n_parts,
 				  const uint32_t * key_offsets, const uint32_t * keys,
@@ -110,6 +316,7 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	for (uint32_t i = 0; i < n; ++i)
 		if (key_offsets[i + 1] <= key_offsets[i])
 			return set_error(TM_ERR_INVALID, "filter %u has no keys (key_offsets must increase)", i);
+	if (fib->in_flight.load() > 0) return set_error(TM_ERR_BUSY, "a ring on this FIB has a submission in flight");
 
 	// Every partition mask must be a subset of each member filter, or the
 	// lookup would drop matches (reference front_end.cc:465-477 relies on the
@@ -146,118 +353,246 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	std::iota(order.begin(), order.end(), 0);
 	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return freq[a] < freq[b]; });
 
-	// Per partition: member order and block count.
-	std::vector<uint32_t> pblock(n_parts + 1, 0);
+	std::vector<uint32_t> h_ovf(1, 0);   // overflow key lists (count, ids...)
+	block_build bb;
+	TM_CHECK(build_blocks(words, part_offsets, part_masks, n_parts, key_offsets, keys, filter_ids, 0, order, dict,
+						  inline_ok, h_ovf, 0, 0, fib->reserve_pct, bb));
 	uint64_t maxp = 0;
-	for (uint32_t p = 0; p < n_parts; ++p) {
-		uint32_t sz = part_offsets[p + 1] - part_offsets[p];
-		maxp = std::max<uint64_t>(maxp, sz);
-		pblock[p + 1] = pblock[p] + (sz + 63) / 64;
-	}
-	const uint32_t nb = pblock[n_parts];
-	std::vector<uint32_t> member(n);  // member[k] = input index at partitioned position k
-	std::iota(member.begin(), member.end(), 0u);
-	parallel_ranges(n_parts, [&](size_t pb, size_t pe) {
-		std::vector<std::array<uint64_t, 3>> key;
-		for (size_t p = pb; p < pe; ++p) {
-			uint32_t b = part_offsets[p], e = part_offsets[p + 1];
-			if (e - b < 2) continue;
-			key.resize(e - b);
-			for (uint32_t i = b; i < e; ++i) {
-				std::array<uint64_t, 3> k{0, 0, 0};
-				const uint32_t * w = words + (size_t)i * 6;
-				for (int r = 0; r < 192; ++r) {
-					int bit = order[r];
-					if ((w[bit >> 5] >> (bit & 31)) & 1)
-						k[r >> 6] |= 1ULL << (63 - (r & 63));
-				}
-				key[i - b] = k;
-			}
-			std::stable_sort(member.begin() + b, member.begin() + e, [&](uint32_t x, uint32_t y) {
-				return key[x - b] < key[y - b];
-			});
-		}
-	});
-
-	std::vector<uint32_t> h_words((size_t)nb * BLOCK_WORDS, 0);
-	std::vector<uint32_t> h_info((size_t)nb * 8, 0);
-	std::vector<uint64_t> h_keys((size_t)nb * 64, ~0ULL);
-	std::vector<uint32_t> h_fid((size_t)nb * 64, 0xFFFFFFFFu);
-	std::vector<uint32_t> h_part((size_t)n_parts * 8, 0);
 	std::vector<uint32_t> h_psize(n_parts, 0);
-	std::vector<uint32_t> h_pstat((size_t)n_parts * 8, 0);
-	// overflow key lists (count, ids...) for filters with more than 4 keys
-	std::vector<uint32_t> h_ovf;
-	h_ovf.push_back(0);
-
-	auto key_id = [&](uint32_t k) -> uint32_t {
-		return (uint32_t)(std::lower_bound(dict.begin(), dict.end(), k) - dict.begin());
-	};
-
 	for (uint32_t p = 0; p < n_parts; ++p) {
-		uint32_t b0 = part_offsets[p], sz = part_offsets[p + 1] - b0;
-		for (int j = 0; j < 6; ++j) h_part[(size_t)p * 8 + j] = part_masks[(size_t)p * 6 + j];
-		h_part[(size_t)p * 8 + 6] = pblock[p];
-		h_part[(size_t)p * 8 + 7] = pblock[p + 1] - pblock[p];
-		h_psize[p] = sz;
-		for (uint32_t k = 0; k < sz; ++k) {
-			uint32_t i = member[b0 + k];
-			uint32_t blk = pblock[p] + k / 64, lane = k % 64;
-			const uint32_t * w = words + (size_t)i * 6;
-			uint32_t * info = &h_info[(size_t)blk * 8];
-			if (lane == 0)
-				for (int j = 0; j < 6; ++j) info[j] = 0xFFFFFFFFu;
-			for (int j = 0; j < 6; ++j) {
-				h_words[(size_t)blk * BLOCK_WORDS + j * 64 + lane] = w[j];
-				info[j] &= w[j];
-			}
-			info[6] = lane + 1;
-			info[7] = p;
-			uint32_t * ps = &h_pstat[(size_t)p * 8];
-			for (int j = 0; j < 6; ++j) {
-				ps[j] |= w[j];
-				ps[7] += (uint32_t)__builtin_popcount(w[j]);
-			}
-			ps[6] += 1;
-			h_fid[(size_t)blk * 64 + lane] = filter_ids ? filter_ids[i] : i;
-			uint32_t kb = key_offsets[i], ke = key_offsets[i + 1], kn = ke - kb;
-			if (inline_ok && kn <= 4) {
-				uint64_t v = ~0ULL;
-				for (uint32_t x = 0; x < kn; ++x) {
-					uint64_t id = key_id(keys[kb + x]);
-					v &= ~(0xFFFFULL << (16 * x));
-					v |= id << (16 * x);
-				}
-				h_keys[(size_t)blk * 64 + lane] = v;
-			} else {
-				uint64_t off = h_ovf.size();
-				if (off > 0xFFFFFFFFULL) return set_error(TM_ERR_INVALID, "overflow key table too large");
-				h_ovf.push_back(kn);
-				for (uint32_t x = 0; x < kn; ++x) h_ovf.push_back(key_id(keys[kb + x]));
-				h_keys[(size_t)blk * 64 + lane] = 0xFFFFULL | (off << 16);
-			}
-		}
+		h_psize[p] = part_offsets[p + 1] - part_offsets[p];
+		maxp = std::max<uint64_t>(maxp, h_psize[p]);
 	}
 
 	fib->release();
 	uint64_t bytes = 0;
-	TM_CHECK(upload(&fib->blk_words, h_words, bytes));
-	TM_CHECK(upload(&fib->blk_info, h_info, bytes));
-	TM_CHECK(upload(&fib->slot_keys, h_keys, bytes));
-	TM_CHECK(upload(&fib->slot_fid, h_fid, bytes));
-	TM_CHECK(upload(&fib->ovf, h_ovf, bytes));
+	TM_CHECK(upload(&fib->blk_words, bb.words, bytes));
+	TM_CHECK(upload(&fib->blk_info, bb.info, bytes));
+	TM_CHECK(upload(&fib->slot_keys, bb.keys, bytes));
+	TM_CHECK(upload(&fib->slot_fid, bb.fid, bytes));
+	fib->ovf_cap = std::max<size_t>(h_ovf.size() * 2, 1024);
+	TM_CHECK(dalloc(&fib->ovf, fib->ovf_cap));
+	TM_CHECK(put(fib->ovf, h_ovf));
+	bytes += fib->ovf_cap * 4;
 	TM_CHECK(upload(&fib->key_dict, dict, bytes));
-	TM_CHECK(upload(&fib->part_info, h_part, bytes));
-	TM_CHECK(upload(&fib->part_stat, h_pstat, bytes));
+	TM_CHECK(upload(&fib->part_info, bb.part, bytes));
+	TM_CHECK(upload(&fib->part_stat, bb.stat, bytes));
 	fib->n_parts = n_parts;
-	fib->n_blocks = nb;
+	fib->n_blocks = bb.nb;
 	fib->n_keys = K;
 	fib->n_filters = n;
 	fib->n_key_instances = nkeys;
 	fib->device_bytes = bytes;
 	fib->max_part = maxp;
-	fib->h_part_info = std::move(h_part);
+	fib->h_part_info = std::move(bb.part);
+	fib->h_part_stat = std::move(bb.stat);
+	fib->h_part_cap = std::move(bb.cap);
 	fib->h_part_size = std::move(h_psize);
+	fib->h_ovf = std::move(h_ovf);
+	fib->h_dict = std::move(dict);
+	fib->h_order = std::move(order);
+	fib->version++;
+	return TM_OK;
+}
+
+int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+				  const uint32_t * keys, const uint32_t * filter_ids, uint32_t maxp) {
+	if (!fib) return set_error(TM_ERR_INVALID, "fib is NULL");
+	TM_CHECK(check_table(words, n, key_offsets, keys));
+	if (maxp == 0) return set_error(TM_ERR_INVALID, "maxp must be > 0");
+	if (fib->in_flight.load() > 0) return set_error(TM_ERR_BUSY, "a ring on this FIB has a submission in flight");
+	if (n == 0) return TM_OK;
+	if (fib->h_order.size() != 192) return set_error(TM_ERR_STATE, "no table uploaded");
+	TM_CHECK(require_device());
+	bool bump = false;   // device arrays, P or K change: rings must re-read the FIB
+
+	// 1. keys new to the dictionary: grow it and remap dense ids
+	const uint64_t nkeys = key_offsets[n];
+	std::vector<uint32_t> fresh;
+	for (uint64_t x = 0; x < nkeys; ++x)
+		if (!std::binary_search(fib->h_dict.begin(), fib->h_dict.end(), keys[x])) fresh.push_back(keys[x]);
+	if (!fresh.empty()) {
+		std::sort(fresh.begin(), fresh.end());
+		fresh.erase(std::unique(fresh.begin(), fresh.end()), fresh.end());
+		std::vector<uint32_t> dict;
+		std::merge(fib->h_dict.begin(), fib->h_dict.end(), fresh.begin(), fresh.end(), std::back_inserter(dict));
+		const bool was_inline = fib->n_keys < KEY_NONE;
+		if (was_inline && dict.size() >= KEY_NONE)
+			return set_error(TM_ERR_STATE, "key dictionary outgrows inline ids: upload the table again");
+		if (dict.size() > 0xFFFFFFF0u) return set_error(TM_ERR_INVALID, "too many distinct keys");
+		std::vector<uint32_t> map(fib->h_dict.size());
+		for (size_t i = 0; i < map.size(); ++i)
+			map[i] = (uint32_t)(std::lower_bound(dict.begin(), dict.end(), fib->h_dict[i]) - dict.begin());
+		if (was_inline && !map.empty()) {
+			uint32_t * d_map = nullptr;
+			TM_CHECK(dalloc(&d_map, map.size()));
+			int rc = put(d_map, map);
+			const uint64_t slots = (uint64_t)fib->n_blocks * 64;
+			if (rc == TM_OK) {
+				hipLaunchKernelGGL(k_remap_keys, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, 0,
+								   fib->slot_keys, slots, d_map);
+				hipError_t e = hipGetLastError();
+				if (e == hipSuccess) e = hipDeviceSynchronize();
+				if (e != hipSuccess) rc = hip_error(e, "key remap", __FILE__, __LINE__);
+			}
+			dfree(d_map);
+			TM_CHECK(rc);
+		}
+		// overflow lists: (count, ids...) from offset 1
+		for (size_t o = 1; o < fib->h_ovf.size();) {
+			const uint32_t c = fib->h_ovf[o];
+			for (uint32_t x = 0; x < c; ++x) fib->h_ovf[o + 1 + x] = map[fib->h_ovf[o + 1 + x]];
+			o += 1 + c;
+		}
+		TM_CHECK(put(fib->ovf, fib->h_ovf));
+		dfree(fib->key_dict);
+		uint64_t b = 0;
+		TM_CHECK(upload(&fib->key_dict, dict, b));
+		fib->h_dict = std::move(dict);
+		fib->n_keys = (uint32_t)fib->h_dict.size();
+		bump = true;
+	}
+	const bool inline_ok = fib->n_keys < KEY_NONE;
+	const size_t ovf_before = fib->h_ovf.size();
+
+	// 2. each filter joins the most selective partition whose mask it
+	//    contains and that has a free reserved lane
+	const uint32_t P = fib->n_parts;
+	std::vector<uint32_t> pmpop(P);
+	for (uint32_t p = 0; p < P; ++p) {
+		uint32_t c = 0;
+		for (int j = 0; j < 6; ++j) c += __builtin_popcount(fib->h_part_info[(size_t)p * 8 + j]);
+		pmpop[p] = c;
+	}
+	std::vector<ins_rec> recs;
+	std::vector<uint32_t> left;   // filters no partition takes
+	recs.reserve(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		const uint32_t * w = words + (size_t)i * 6;
+		int64_t best = -1;
+		for (uint32_t p = 0; p < P; ++p) {
+			const uint32_t * m = &fib->h_part_info[(size_t)p * 8];
+			if ((m[0] & ~w[0]) | (m[1] & ~w[1]) | (m[2] & ~w[2]) | (m[3] & ~w[3]) | (m[4] & ~w[4]) | (m[5] & ~w[5]))
+				continue;
+			if (fib->h_part_size[p] >= (uint64_t)fib->h_part_cap[p] * 64) continue;
+			if (best < 0 || pmpop[p] > pmpop[best] ||
+				(pmpop[p] == pmpop[best] && fib->h_part_size[p] < fib->h_part_size[best]))
+				best = p;
+		}
+		if (best < 0) {
+			left.push_back(i);
+			continue;
+		}
+		const uint32_t p = (uint32_t)best;
+		const uint32_t k = fib->h_part_size[p]++;
+		ins_rec r;
+		r.block = fib->h_part_info[(size_t)p * 8 + 6] + k / 64;
+		r.lane = k % 64;
+		r.fid = filter_ids ? filter_ids[i] : (uint32_t)(fib->n_filters + i);
+		r.pad = 0;
+		for (int j = 0; j < 6; ++j) r.w[j] = w[j];
+		TM_CHECK(key_slot(keys, key_offsets[i], key_offsets[i + 1], fib->h_dict, inline_ok, fib->h_ovf, r.kv));
+		recs.push_back(r);
+		fib->h_part_info[(size_t)p * 8 + 7] = (fib->h_part_size[p] + 63) / 64;
+		uint32_t * ps = &fib->h_part_stat[(size_t)p * 8];
+		for (int j = 0; j < 6; ++j) {
+			ps[j] |= w[j];
+			ps[7] += (uint32_t)__builtin_popcount(w[j]);
+		}
+		ps[6] += 1;
+		fib->max_part = std::max<uint64_t>(fib->max_part, fib->h_part_size[p]);
+	}
+
+	// 3. the rest: partitioned on its own, appended as new partitions
+	block_build bb;
+	uint32_t np = 0;
+	if (!left.empty()) {
+		const uint32_t m = (uint32_t)left.size();
+		std::vector<uint32_t> lw((size_t)m * 6), lko(m + 1, 0), lk, lfid(m);
+		for (uint32_t x = 0; x < m; ++x) {
+			const uint32_t i = left[x];
+			std::memcpy(&lw[(size_t)x * 6], words + (size_t)i * 6, 24);
+			lfid[x] = filter_ids ? filter_ids[i] : (uint32_t)(fib->n_filters + i);
+		}
+		std::vector<uint32_t> perm(m), poff(m + 2), pmask((size_t)(m + 1) * 6);
+		TM_CHECK(tm_partition(lw.data(), m, maxp, perm.data(), poff.data(), pmask.data(), m + 1, &np));
+		std::vector<uint32_t> pw((size_t)m * 6), pfid(m);
+		for (uint32_t x = 0; x < m; ++x) {
+			const uint32_t i = left[perm[x]];
+			std::memcpy(&pw[(size_t)x * 6], words + (size_t)i * 6, 24);
+			pfid[x] = lfid[perm[x]];
+			for (uint32_t y = key_offsets[i]; y < key_offsets[i + 1]; ++y) lk.push_back(keys[y]);
+			lko[x + 1] = (uint32_t)lk.size();
+		}
+		TM_CHECK(build_blocks(pw.data(), poff.data(), pmask.data(), np, lko.data(), lk.data(), pfid.data(), 0,
+							  fib->h_order, fib->h_dict, inline_ok, fib->h_ovf, P, fib->n_blocks, fib->reserve_pct,
+							  bb));
+	}
+
+	// 4. device: grow arrays for new partitions and overflow lists
+	hipStream_t s = nullptr;
+	if (np) {
+		const size_t ob = fib->n_blocks, nbt = ob + bb.nb;
+		TM_CHECK(regrow(&fib->blk_words, ob * BLOCK_WORDS, nbt * BLOCK_WORDS, s));
+		TM_CHECK(regrow(&fib->blk_info, ob * 8, nbt * 8, s));
+		TM_CHECK(regrow(&fib->slot_keys, ob * 64, nbt * 64, s));
+		TM_CHECK(regrow(&fib->slot_fid, ob * 64, nbt * 64, s));
+		TM_HIP(hipMemcpy(fib->blk_words + ob * BLOCK_WORDS, bb.words.data(), bb.words.size() * 4, hipMemcpyHostToDevice));
+		TM_HIP(hipMemcpy(fib->blk_info + ob * 8, bb.info.data(), bb.info.size() * 4, hipMemcpyHostToDevice));
+		TM_HIP(hipMemcpy(fib->slot_keys + ob * 64, bb.keys.data(), bb.keys.size() * 8, hipMemcpyHostToDevice));
+		TM_HIP(hipMemcpy(fib->slot_fid + ob * 64, bb.fid.data(), bb.fid.size() * 4, hipMemcpyHostToDevice));
+		fib->device_bytes += (uint64_t)bb.nb * (BLOCK_WORDS * 4 + 32 + 64 * 12);
+		fib->n_blocks = (uint32_t)nbt;
+		fib->h_part_info.insert(fib->h_part_info.end(), bb.part.begin(), bb.part.end());
+		fib->h_part_stat.insert(fib->h_part_stat.end(), bb.stat.begin(), bb.stat.end());
+		fib->h_part_cap.insert(fib->h_part_cap.end(), bb.cap.begin(), bb.cap.end());
+		for (uint32_t p = 0; p < np; ++p) {
+			const uint32_t sz = bb.stat[(size_t)p * 8 + 6];
+			fib->h_part_size.push_back(sz);
+			fib->max_part = std::max<uint64_t>(fib->max_part, sz);
+		}
+		dfree(fib->part_info);
+		dfree(fib->part_stat);
+		uint64_t b = 0;
+		TM_CHECK(upload(&fib->part_info, fib->h_part_info, b));
+		TM_CHECK(upload(&fib->part_stat, fib->h_part_stat, b));
+		fib->n_parts = P + np;
+		bump = true;
+	}
+	if (fib->h_ovf.size() > fib->ovf_cap) {
+		const size_t cap = fib->h_ovf.size() * 2;
+		dfree(fib->ovf);
+		TM_CHECK(dalloc(&fib->ovf, cap));
+		fib->device_bytes += (cap - fib->ovf_cap) * 4;
+		fib->ovf_cap = cap;
+		TM_CHECK(put(fib->ovf, fib->h_ovf));
+		bump = true;
+	} else if (fib->h_ovf.size() > ovf_before) {
+		TM_HIP(hipMemcpy(fib->ovf + ovf_before, fib->h_ovf.data() + ovf_before, (fib->h_ovf.size() - ovf_before) * 4,
+						 hipMemcpyHostToDevice));
+	}
+
+	// 5. device: lanes into reserved blocks, then the partitions' new block
+	//    counts and statistics
+	if (!recs.empty()) {
+		ins_rec * d_recs = nullptr;
+		TM_CHECK(dalloc(&d_recs, recs.size()));
+		int rc = put(d_recs, recs);
+		if (rc == TM_OK) {
+			hipLaunchKernelGGL(k_insert_lanes, dim3((uint32_t)((recs.size() + 255) / 256)), dim3(256), 0, 0, d_recs,
+							   (uint32_t)recs.size(), fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid);
+			hipError_t e = hipGetLastError();
+			if (e == hipSuccess) e = hipDeviceSynchronize();
+			if (e != hipSuccess) rc = hip_error(e, "insert", __FILE__, __LINE__);
+		}
+		dfree(d_recs);
+		TM_CHECK(rc);
+		TM_CHECK(put(fib->part_info, fib->h_part_info, (size_t)P * 8));
+		TM_CHECK(put(fib->part_stat, fib->h_part_stat, (size_t)P * 8));
+	}
+	fib->n_filters += n;
+	fib->n_key_instances += nkeys;
+	if (bump) fib->version++;
 	return TM_OK;
 }
 

@@ -413,7 +413,10 @@ struct mq_args {
 	const uint32_t * __restrict__ qinv;       // nq x 8 inverted words (VAR 1)
 	uint32_t nq;
 	const uint32_t * __restrict__ porder;     // partition schedule order (NULL: 0..P-1)
-	const uint32_t * __restrict__ iorder;     // query tile of each work item (NULL: identity)
+	const uint32_t * __restrict__ iorder;     // query tile of each work item (NULL: identity; nseg 1 only)
+	uint32_t ntiles;                          // query tiles
+	uint32_t nseg;                            // block segments per tile: item = seg * ntiles + tile
+	uint32_t seg_blocks;                      // blocks per segment (nseg > 1)
 	unsigned long long * qnext;               // NQUEUE queue cursors
 	const uint64_t * __restrict__ qbeg;       // NQUEUE + 1 queue bounds (items)
 	uint32_t * pcount;                        // per partition: queries selecting it
@@ -560,7 +563,20 @@ k_match_q(mq_args a) {
 		__syncthreads();
 		if (s_item[1]) break;
 		const uint32_t item0 = __builtin_amdgcn_readfirstlane(s_item[0]);
-		const uint32_t item = a.iorder ? sld1(a.iorder + item0) : item0;   // query tile
+		// A work item is a query tile, or with nseg > 1 a tile restricted to
+		// one contiguous range of blocks (small submissions spread over the
+		// whole GPU); segment-major, so concurrent items share filter blocks.
+		uint32_t item, seg_b0 = 0, seg_b1 = 0xFFFFFFFFu, r_begin = 0, r_end = P;
+		if (a.nseg > 1) {
+			const uint32_t seg = item0 / a.ntiles;
+			item = item0 - seg * a.ntiles;
+			seg_b0 = seg * a.seg_blocks;
+			seg_b1 = min(seg_b0 + a.seg_blocks, a.f.n_blocks);
+			r_begin = sld1(a.f.blk_info + (size_t)seg_b0 * 8 + 7);
+			r_end = sld1(a.f.blk_info + (size_t)(seg_b1 - 1) * 8 + 7) + 1;
+		} else {
+			item = a.iorder ? sld1(a.iorder + item0) : item0;
+		}
 		const uint32_t q0 = item * NQ;
 
 		for (uint32_t t = tid; t < (uint32_t)NQ; t += NT) {
@@ -609,12 +625,15 @@ k_match_q(mq_args a) {
 		}
 
 		uint32_t hit_acc = 0, n_visit = 0, n_test = 0, n_hev = 0;
-		for (uint32_t r = 0; r < P; ++r) {
-			const uint32_t p = a.porder ? sld1(a.porder + r) : r;
+		for (uint32_t r = r_begin; r < r_end; ++r) {
+			const uint32_t p = a.porder && a.nseg <= 1 ? sld1(a.porder + r) : r;
 			const uint32_t * pi = a.f.part_info + (size_t)p * 8;
 			const uint4 pa = sld4(pi);
 			const uint4 pb = sld4(pi + 4);
-			const uint32_t fb = pb.z, nb = pb.w;
+			// this item's share of the partition's blocks
+			const uint32_t fb = max(pb.z, seg_b0), fe = min(pb.z + pb.w, seg_b1);
+			const uint32_t nb = fe > fb ? fe - fb : 0;
+			const bool counts_here = pb.z >= seg_b0;   // count each partition once
 			uint64_t qsel[TILES];
 			uint64_t anyq = 0;
 			uint32_t nsel = 0;
@@ -626,8 +645,8 @@ k_match_q(mq_args a) {
 				anyq |= qsel[t];
 				nsel += (uint32_t)__popcll(qsel[t]);
 			}
+			if (wave == 0 && lane == 0 && a.pcount && counts_here && nsel) atomicAdd(&a.pcount[p], nsel);
 			if (anyq == 0 || nb == 0) continue;
-			if (wave == 0 && lane == 0 && a.pcount) atomicAdd(&a.pcount[p], nsel);
 			for (uint32_t b = fb + wave; b < fb + nb; b += NW) {
 				const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
 				const uint4 ma = sld4(bi);
@@ -727,13 +746,20 @@ k_match_q(mq_args a) {
 		if (MODE == 0 && LDS_BM) {
 			const uint32_t nw = (min(a.nq, q0 + NQ) - q0) * kw;
 			uint32_t * dst = a.gbm + (size_t)q0 * kw;
-			for (uint32_t w = tid; w < nw; w += NT) dst[w] = s_bm[w];
+			if (a.nseg > 1) {   // other segments of the tile write the same rows
+				for (uint32_t w = tid; w < nw; w += NT) {
+					const uint32_t v = s_bm[w];
+					if (v) atomicOr(&dst[w], v);
+				}
+			} else {
+				for (uint32_t w = tid; w < nw; w += NT) dst[w] = s_bm[w];
+			}
 		}
 		if (tid == 0) {
 			if (s_hits) atomicAdd(&a.counters[1], (unsigned long long)s_hits);
-			if (a.log && item < a.log_cap) {
-				a.log[(size_t)item * 2] = s_hits;
-				a.log[(size_t)item * 2 + 1] = (uint32_t)(wall_clock64() - t_start);
+			if (a.log && item0 < a.log_cap) {
+				a.log[(size_t)item0 * 2] = s_hits;
+				a.log[(size_t)item0 * 2 + 1] = (uint32_t)(wall_clock64() - t_start);
 			}
 		}
 		__syncthreads();
@@ -995,6 +1021,8 @@ struct tm_ring {
 	bool graph_ok = false;
 
 	uint32_t last_nq = 0;
+	uint64_t fib_version = 0;  // FIB version the device arrays are sized for
+	uint32_t last_nseg = 1;    // block segments per tile of the last persistent step
 	bool pending = false;
 	bool match_events = false;
 	int retries = 0;
@@ -1164,7 +1192,20 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	fill_u32(r->d_pcount, 0, P, s);
 	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 8, s);
 	const uint64_t nqg = (uint64_t)r->tiles * 64;
-	const uint64_t items = (nq + nqg - 1) / nqg;
+	const uint64_t tiles = (nq + nqg - 1) / nqg;
+	// Small submissions: cut each tile's work into block segments so that
+	// the items fill the GPU (about two per resident workgroup).  Needs the
+	// key bitmaps in LDS (flushed with atomicOr) or the hit-list modes.
+	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	uint32_t nseg = 1;
+	const uint32_t nblk = r->fib->n_blocks;
+	if ((r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE) && tiles > 0 && !r->use_iorder && !r->cfg.log_units)
+		nseg = (uint32_t)std::min<uint64_t>((2 * resident + tiles - 1) / tiles, std::max<uint32_t>(nblk / 16, 1));
+	const uint32_t seg_blocks = nseg > 1 ? (nblk + nseg - 1) / nseg : 0;
+	if (nseg > 1) nseg = (nblk + seg_blocks - 1) / seg_blocks;
+	const uint64_t items = tiles * nseg;
+	r->last_nseg = nseg;
+	if (nseg > 1 && r->cfg.mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 	hipLaunchKernelGGL(k_init_queues, dim3(1), dim3(64), 0, s, items, r->d_qbeg, r->d_qnext);
 	mq_args a;
 	std::memset(&a, 0, sizeof(a));
@@ -1173,6 +1214,9 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.nq = nq;
 	a.porder = r->have_order ? r->d_porder : nullptr;
 	a.iorder = r->use_iorder ? r->d_iorder : nullptr;
+	a.ntiles = (uint32_t)tiles;
+	a.nseg = nseg;
+	a.seg_blocks = seg_blocks;
 	a.qnext = r->d_qnext;
 	a.qbeg = r->d_qbeg;
 	a.pcount = r->d_pcount;
@@ -1399,7 +1443,10 @@ int price_submission(tm_ring * r, uint32_t nq) {
 	for (uint32_t t = 0; t < n_tiles; ++t) total += hits[t];
 	r->st.pred_hits = total;
 
-	if (r->model_flags & TM_COST_ORDER) {
+	// ordering pays when tiles outnumber the resident workgroups; smaller
+	// submissions are cut into block segments instead (enqueue_persistent)
+	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	if ((r->model_flags & TM_COST_ORDER) && n_tiles >= 2 * resident) {
 		std::vector<uint32_t> idx(n_tiles);
 		std::iota(idx.begin(), idx.end(), 0u);
 		std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
@@ -1459,15 +1506,47 @@ int price_submission(tm_ring * r, uint32_t nq) {
 	return TM_OK;
 }
 
+// After tm_fib_insert changed the partition count, the key count or the
+// FIB's device arrays: re-size what depends on them and drop any graph
+// captured with the old pointers.
+int sync_fib(tm_ring * r) {
+	const tm_fib * f = r->fib;
+	if (r->fib_version == f->version) return TM_OK;
+	drop_graph(r);
+	if (r->P != f->n_parts) {
+		dfree(r->d_pcount); dfree(r->d_pbase); dfree(r->d_pcur); dfree(r->d_sched); dfree(r->d_iprefix);
+		dfree(r->d_qbeg); dfree(r->d_qnext); dfree(r->d_porder); dfree(r->d_cnt); dfree(r->d_stat);
+		dfree(r->d_qcount); dfree(r->d_off); dfree(r->d_qcur); dfree(r->d_scan);
+		r->have_order = false;
+		r->P = f->n_parts;
+	}
+	const uint32_t kw = (f->n_keys + 31) / 32;
+	if (kw != r->kw) {
+		dfree(r->d_gbm);
+		r->kw = kw;
+	}
+	TM_CHECK(resolve_params(r));
+	r->fib_version = f->version;
+	return ensure_capacity(r);
+}
+
+void end_pending(tm_ring * r) {
+	if (!r->pending) return;
+	r->pending = false;
+	r->fib->in_flight.fetch_sub(1);
+}
+
 int submit_common(tm_ring * r, uint32_t nq) {
 	r->st = tm_status();
 	r->last_nq = nq;
 	r->retries = 0;
+	TM_CHECK(sync_fib(r));
 	TM_CHECK(ensure_capacity(r));
 	TM_CHECK(price_submission(r, nq));
 	int rc = enqueue_step(r, nq, true);
 	if (rc != TM_OK) return rc;
 	r->pending = true;
+	r->fib->in_flight.fetch_add(1);
 	return TM_OK;
 }
 
@@ -1492,6 +1571,7 @@ int tm_ring_create(tm_fib * fib, uint32_t max_queries, const tm_match_config * c
 	r->max_q = max_queries;
 	r->P = fib->n_parts;
 	r->kw = (fib->n_keys + 31) / 32;
+	r->fib_version = fib->version;
 	int rc = resolve_params(r);
 	if (rc != TM_OK) { delete r; return rc; }
 	hipError_t e = hipStreamCreateWithFlags(&r->s, hipStreamNonBlocking);
@@ -1519,6 +1599,7 @@ int tm_ring_destroy(tm_ring * r) {
 	if (!r) return TM_OK;
 	if (dev().device >= 0) (void)hipSetDevice(dev().device);
 	if (r->s) (void)hipStreamSynchronize(r->s);
+	end_pending(r);
 	r->release();
 	delete r;
 	return TM_OK;
@@ -1601,7 +1682,7 @@ int tm_wait(tm_ring * r, tm_status * st) {
 			}
 			if (!again) break;
 			if (++r->retries > 4) {
-				r->pending = false;
+				end_pending(r);
 				return set_error(TM_ERR_OVERFLOW, "capacity retry limit reached");
 			}
 			// overflow-retry: capacities changed, so drop any captured graph
@@ -1610,7 +1691,7 @@ int tm_wait(tm_ring * r, tm_status * st) {
 			r->graph_ok = false;
 			TM_CHECK(enqueue_step(r, r->last_nq, false));
 		}
-		r->pending = false;
+		end_pending(r);
 		float dms = 0.f, mms = 0.f;
 		(void)hipEventElapsedTime(&dms, r->e0, r->e1);
 		if (r->match_events) (void)hipEventElapsedTime(&mms, r->em0, r->em1);

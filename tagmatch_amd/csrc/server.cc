@@ -1,0 +1,457 @@
+// Serving loop: continuous batching of match requests over several rings
+// in flight, admission by an HBM budget for result buffers, and filter
+// inserts at version barriers.  It is a client of the C ABI only.
+//
+// The reference's serving path is the front end's per-partition queues
+// flushed when full or when the oldest query passes the latency limit
+// (front_end.cc:249-267, :350-376, :563-598), feeding two double-buffered
+// streams per GPU (back_end.cc:172-343).  Here a request is a batch of
+// queries; an engine thread admits as many waiting requests as fit one free
+// ring (slot) and the budget, submits them as one device step, and returns
+// each request's own results when the step completes.  Steps of different
+// slots run on different HIP streams and overlap on the device.
+#include "../../include/tagmatch_amd.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace tmx {
+int set_error(int code, const char * fmt, ...);   // hip_util.hip: text for tm_last_error()
+}
+
+namespace {
+
+using clock_t_ = std::chrono::steady_clock;
+
+double ms_since(clock_t_::time_point t0, clock_t_::time_point t1) {
+	return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+struct request {
+	uint64_t id = 0;
+	bool is_insert = false;
+	uint32_t nq = 0;
+	std::vector<uint32_t> q;          // nq x 6
+	// insert payload
+	std::vector<uint32_t> ins_words, ins_koff, ins_keys, ins_fid;
+	uint32_t ins_n = 0, ins_maxp = 0;
+	// results
+	std::vector<uint64_t> off;
+	std::vector<uint32_t> vals;
+	int status = TM_OK;
+	std::string error;
+	uint64_t reserved = 0;            // bytes of the HBM budget held
+	clock_t_::time_point t_submit, t_admit, t_done;
+	uint32_t step_queries = 0;        // queries of the device step it ran in
+	bool done = false;
+};
+
+struct slot {
+	tm_ring * ring = nullptr;
+	std::vector<request *> reqs;
+	uint64_t reserved = 0;
+	bool busy = false;
+	std::vector<uint32_t> words;
+	std::vector<uint64_t> off;
+	std::vector<uint32_t> vals;
+};
+
+} // namespace
+
+struct tm_server {
+	tm_fib * fib = nullptr;
+	tm_server_config cfg;
+	tm_match_config mcfg;
+	std::vector<slot> slots;
+	std::mutex mu;
+	std::condition_variable cv_engine, cv_done;
+	std::deque<request *> waiting;
+	std::unordered_map<uint64_t, request *> reqs;
+	std::deque<uint64_t> finished;    // ids not yet collected by tm_server_collect
+	uint64_t next_id = 1;
+	uint64_t reserved = 0;            // bytes held by admitted requests
+	double bytes_per_query = 0;       // running estimate of result bytes per query
+	uint64_t fixed_per_query = 0;     // result bytes per query that do not depend on hits
+	tm_server_stats st{};
+	std::thread th;
+	std::atomic<bool> stop{false};
+	int fatal = TM_OK;
+	std::string fatal_msg;
+};
+
+namespace {
+
+// Result bytes a request of nq queries is charged against the budget.
+uint64_t estimate(const tm_server * s, uint32_t nq) {
+	return (uint64_t)((double)nq * (s->bytes_per_query * 1.5 + (double)s->fixed_per_query)) + 4096;
+}
+
+void finish(tm_server * s, request * r) {
+	r->t_done = clock_t_::now();
+	r->done = true;
+	s->finished.push_back(r->id);
+	const double lat = ms_since(r->t_submit, r->t_done);
+	if (!r->is_insert) {
+		s->st.n_requests_done++;
+		s->st.n_queries_done += r->nq;
+		s->st.latency_sum_ms += lat;
+		s->st.latency_max_ms = std::max(s->st.latency_max_ms, lat);
+	}
+}
+
+// Collect a finished step: split results per request, release the budget.
+void complete(tm_server * s, slot & sl) {
+	tm_status st;
+	int rc = tm_wait(sl.ring, &st);
+	const uint32_t nq = (uint32_t)(sl.words.size() / 6);
+	if (rc == TM_OK) {
+		sl.off.resize((size_t)nq + 1);
+		rc = tm_read_results(sl.ring, sl.off.data(), nullptr, 0);
+	}
+	if (rc == TM_OK) {
+		sl.vals.resize(sl.off[nq]);
+		rc = tm_read_results(sl.ring, sl.off.data(), sl.vals.data(), sl.vals.size());
+	}
+	std::string err = rc == TM_OK ? std::string() : std::string(tm_last_error());
+	std::lock_guard<std::mutex> lk(s->mu);
+	uint32_t q0 = 0;
+	for (request * r : sl.reqs) {
+		r->status = rc;
+		r->step_queries = nq;
+		if (rc == TM_OK) {
+			const uint64_t base = sl.off[q0];
+			r->off.resize((size_t)r->nq + 1);
+			for (uint32_t i = 0; i <= r->nq; ++i) r->off[i] = sl.off[q0 + i] - base;
+			r->vals.assign(sl.vals.begin() + base, sl.vals.begin() + sl.off[q0 + r->nq]);
+		} else {
+			r->error = err;
+		}
+		q0 += r->nq;
+		finish(s, r);
+	}
+	if (rc == TM_OK && nq) {
+		// running estimate of result bytes per query for admission
+		const double b = 4.0 * (double)sl.off[nq] / (double)nq;
+		s->bytes_per_query = s->bytes_per_query == 0 ? b : 0.8 * s->bytes_per_query + 0.2 * b;
+		s->st.n_steps++;
+		s->st.device_ms_sum += st.device_ms;
+		if (st.overflow) s->st.n_retries++;
+	}
+	s->reserved -= sl.reserved;
+	sl.reserved = 0;
+	sl.reqs.clear();
+	sl.busy = false;
+	s->cv_done.notify_all();
+}
+
+void engine(tm_server * s) {
+	for (;;) {
+		bool progressed = false;
+		// 1. completions
+		for (auto & sl : s->slots) {
+			if (!sl.busy) continue;
+			if (tm_poll(sl.ring) == TM_ERR_BUSY) continue;
+			complete(s, sl);
+			progressed = true;
+		}
+		// 2. admission (and insert barriers), under the lock
+		std::unique_lock<std::mutex> lk(s->mu);
+		for (auto & sl : s->slots) {
+			if (sl.busy || s->waiting.empty()) continue;
+			if (s->waiting.front()->is_insert) break;
+			uint32_t nq = 0;
+			uint64_t bytes = 0;
+			while (!s->waiting.empty() && !s->waiting.front()->is_insert) {
+				request * r = s->waiting.front();
+				const uint64_t need = estimate(s, r->nq);
+				if (nq + r->nq > s->cfg.slot_queries && nq > 0) break;
+				// the budget admits a request when it fits, or when nothing at
+				// all is in flight (a request larger than the budget alone)
+				const bool idle = s->reserved + bytes == 0;
+				if (!idle && s->reserved + bytes + need > s->cfg.hbm_budget) {
+					if (nq == 0) s->st.n_budget_waits++;
+					break;
+				}
+				s->waiting.pop_front();
+				r->reserved = need;
+				r->t_admit = clock_t_::now();
+				sl.reqs.push_back(r);
+				nq += r->nq;
+				bytes += need;
+			}
+			if (sl.reqs.empty()) break;   // budget full: later slots cannot do better
+			s->reserved += bytes;
+			sl.reserved = bytes;
+			s->st.reserved_peak = std::max<uint64_t>(s->st.reserved_peak, s->reserved);
+			lk.unlock();
+			sl.words.resize((size_t)nq * 6);
+			uint32_t q0 = 0;
+			for (request * r : sl.reqs) {
+				if (r->nq) std::memcpy(&sl.words[(size_t)q0 * 6], r->q.data(), (size_t)r->nq * 24);
+				q0 += r->nq;
+			}
+			int rc = tm_submit(sl.ring, sl.words.data(), nq);
+			lk.lock();
+			if (rc != TM_OK) {
+				const std::string err = tm_last_error();
+				for (request * r : sl.reqs) {
+					r->status = rc;
+					r->error = err;
+					finish(s, r);
+				}
+				s->reserved -= sl.reserved;
+				sl.reserved = 0;
+				sl.reqs.clear();
+				s->cv_done.notify_all();
+				continue;
+			}
+			sl.busy = true;
+			s->st.n_batches++;
+			s->st.requests_per_batch_sum += sl.reqs.size();
+			progressed = true;
+		}
+		// insert barrier: every earlier request has completed
+		if (!s->waiting.empty() && s->waiting.front()->is_insert) {
+			bool idle = true;
+			for (auto & sl : s->slots) idle = idle && !sl.busy;
+			if (idle) {
+				request * r = s->waiting.front();
+				s->waiting.pop_front();
+				lk.unlock();
+				int rc = tm_fib_insert(s->fib, r->ins_words.data(), r->ins_n, r->ins_koff.data(), r->ins_keys.data(),
+									   r->ins_fid.empty() ? nullptr : r->ins_fid.data(), r->ins_maxp);
+				const std::string err = rc == TM_OK ? std::string() : std::string(tm_last_error());
+				lk.lock();
+				r->status = rc;
+				r->error = err;
+				s->st.n_inserts++;
+				finish(s, r);
+				s->cv_done.notify_all();
+				progressed = true;
+			}
+		}
+		if (s->stop.load() && s->waiting.empty()) {
+			bool idle = true;
+			for (auto & sl : s->slots) idle = idle && !sl.busy;
+			if (idle) return;
+		}
+		if (!progressed) {
+			bool any_busy = false;
+			for (auto & sl : s->slots) any_busy = any_busy || sl.busy;
+			// poll device completions quickly; sleep on the queue when idle
+			s->cv_engine.wait_for(lk, any_busy ? std::chrono::microseconds(20) : std::chrono::milliseconds(5));
+		}
+	}
+}
+
+} // namespace
+
+extern "C" {
+
+void tm_server_config_default(tm_server_config * c) {
+	std::memset(c, 0, sizeof(*c));
+	c->n_slots = 4;
+	c->slot_queries = 16384;
+	c->hbm_budget = 0;
+}
+
+int tm_server_create(tm_fib * fib, const tm_server_config * cfg, const tm_match_config * mcfg, tm_server ** out) {
+	if (!fib || !out) return tmx::set_error(TM_ERR_INVALID, "NULL argument");
+	tm_server * s = new tm_server();
+	s->fib = fib;
+	if (cfg) s->cfg = *cfg; else tm_server_config_default(&s->cfg);
+	if (mcfg) s->mcfg = *mcfg; else tm_match_config_default(&s->mcfg);
+	if (s->cfg.n_slots == 0) s->cfg.n_slots = 4;
+	if (s->cfg.slot_queries == 0) s->cfg.slot_queries = 16384;
+	if (s->cfg.n_slots > 64) {
+		delete s;
+		return tmx::set_error(TM_ERR_INVALID, "n_slots above 64");
+	}
+	if (s->cfg.hbm_budget == 0) {
+		size_t fr = 0, tot = 0;
+		int rc = tm_mem_info(&fr, &tot);
+		if (rc != TM_OK) {
+			delete s;
+			return rc;
+		}
+		s->cfg.hbm_budget = fr / 4;
+	}
+	tm_fib_stats fs;
+	tm_fib_stats_get(fib, &fs);
+	// key bitmap rows are per query and independent of hits
+	s->fixed_per_query = s->mcfg.mode == TM_MODE_KEYS_UNIQUE ? 4 * ((fs.n_distinct_keys + 31) / 32) : 8;
+	s->slots.resize(s->cfg.n_slots);
+	for (auto & sl : s->slots) {
+		int rc = tm_ring_create(fib, s->cfg.slot_queries, &s->mcfg, &sl.ring);
+		if (rc != TM_OK) {
+			for (auto & x : s->slots)
+				if (x.ring) tm_ring_destroy(x.ring);
+			delete s;
+			return rc;
+		}
+	}
+	s->th = std::thread(engine, s);
+	*out = s;
+	return TM_OK;
+}
+
+int tm_server_destroy(tm_server * s) {
+	if (!s) return TM_OK;
+	s->stop.store(true);
+	s->cv_engine.notify_all();
+	if (s->th.joinable()) s->th.join();
+	for (auto & sl : s->slots)
+		if (sl.ring) tm_ring_destroy(sl.ring);
+	for (auto & kv : s->reqs) delete kv.second;
+	delete s;
+	return TM_OK;
+}
+
+int tm_server_set_cost_model(tm_server * s, tm_mlp * model, uint32_t flags) {
+	if (!s) return tmx::set_error(TM_ERR_INVALID, "server is NULL");
+	// only between steps: wait until every slot is idle
+	std::unique_lock<std::mutex> lk(s->mu);
+	for (;;) {
+		bool idle = s->waiting.empty();
+		for (auto & sl : s->slots) idle = idle && !sl.busy;
+		if (idle) break;
+		s->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+	}
+	for (auto & sl : s->slots) {
+		int rc = tm_ring_set_cost_model(sl.ring, model, flags);
+		if (rc != TM_OK) return rc;
+	}
+	return TM_OK;
+}
+
+int tm_server_submit(tm_server * s, const uint32_t * queries, uint32_t nq, uint64_t * id) {
+	if (!s || !id || (nq && !queries)) return tmx::set_error(TM_ERR_INVALID, "NULL argument");
+	if (nq > s->cfg.slot_queries)
+		return tmx::set_error(TM_ERR_INVALID, "request of %u queries exceeds slot_queries %u", nq, s->cfg.slot_queries);
+	request * r = new request();
+	r->nq = nq;
+	r->q.assign(queries, queries + (size_t)nq * 6);
+	r->t_submit = clock_t_::now();
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		if (s->stop.load()) {
+			delete r;
+			return tmx::set_error(TM_ERR_STATE, "server is stopping");
+		}
+		r->id = s->next_id++;
+		s->reqs[r->id] = r;
+		s->waiting.push_back(r);
+		s->st.n_requests++;
+	}
+	s->cv_engine.notify_one();
+	*id = r->id;
+	return TM_OK;
+}
+
+int tm_server_insert(tm_server * s, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+					 const uint32_t * keys, const uint32_t * filter_ids, uint32_t maxp, uint64_t * id) {
+	if (!s || !id || (n && (!words || !key_offsets || !keys))) return tmx::set_error(TM_ERR_INVALID, "NULL argument");
+	request * r = new request();
+	r->is_insert = true;
+	r->ins_n = n;
+	r->ins_maxp = maxp;
+	r->ins_words.assign(words, words + (size_t)n * 6);
+	r->ins_koff.assign(key_offsets, key_offsets + n + 1);
+	if (n) r->ins_keys.assign(keys, keys + key_offsets[n]);
+	if (filter_ids) r->ins_fid.assign(filter_ids, filter_ids + n);
+	if (r->ins_keys.empty()) r->ins_keys.push_back(0);
+	r->t_submit = clock_t_::now();
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		r->id = s->next_id++;
+		s->reqs[r->id] = r;
+		s->waiting.push_back(r);
+	}
+	s->cv_engine.notify_one();
+	*id = r->id;
+	return TM_OK;
+}
+
+int tm_server_wait(tm_server * s, uint64_t id, double timeout_ms) {
+	if (!s) return tmx::set_error(TM_ERR_INVALID, "server is NULL");
+	std::unique_lock<std::mutex> lk(s->mu);
+	auto it = s->reqs.find(id);
+	if (it == s->reqs.end()) return tmx::set_error(TM_ERR_INVALID, "unknown request %llu", (unsigned long long)id);
+	request * r = it->second;
+	const auto deadline = clock_t_::now() + std::chrono::microseconds((int64_t)(timeout_ms * 1000));
+	while (!r->done) {
+		if (timeout_ms >= 0 && clock_t_::now() >= deadline) return TM_ERR_BUSY;
+		s->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+	}
+	return r->status;
+}
+
+int tm_server_collect(tm_server * s, uint64_t * ids, uint32_t cap, uint32_t * n) {
+	if (!s || !n || (cap && !ids)) return tmx::set_error(TM_ERR_INVALID, "NULL argument");
+	std::lock_guard<std::mutex> lk(s->mu);
+	uint32_t k = 0;
+	while (k < cap && !s->finished.empty()) {
+		ids[k++] = s->finished.front();
+		s->finished.pop_front();
+	}
+	*n = k;
+	return TM_OK;
+}
+
+int tm_server_result(tm_server * s, uint64_t id, tm_request_info * info, uint64_t * offsets, uint32_t * values,
+					 uint64_t capacity) {
+	if (!s) return tmx::set_error(TM_ERR_INVALID, "server is NULL");
+	std::lock_guard<std::mutex> lk(s->mu);
+	auto it = s->reqs.find(id);
+	if (it == s->reqs.end()) return tmx::set_error(TM_ERR_INVALID, "unknown request %llu", (unsigned long long)id);
+	request * r = it->second;
+	if (!r->done) return tmx::set_error(TM_ERR_BUSY, "request %llu still running", (unsigned long long)id);
+	if (info) {
+		info->status = r->status;
+		info->n_queries = r->nq;
+		info->n_values = r->vals.size();
+		info->step_queries = r->step_queries;
+		info->latency_ms = ms_since(r->t_submit, r->t_done);
+		info->queue_ms = r->is_insert ? 0.0 : ms_since(r->t_submit, r->t_admit);
+	}
+	if (offsets && !r->off.empty()) std::memcpy(offsets, r->off.data(), r->off.size() * sizeof(uint64_t));
+	if (values) {
+		if (r->vals.size() > capacity)
+			return tmx::set_error(TM_ERR_OVERFLOW, "%zu values, capacity %llu", r->vals.size(), (unsigned long long)capacity);
+		if (!r->vals.empty()) std::memcpy(values, r->vals.data(), r->vals.size() * 4);
+	}
+	return TM_OK;
+}
+
+int tm_server_release(tm_server * s, uint64_t id) {
+	if (!s) return tmx::set_error(TM_ERR_INVALID, "server is NULL");
+	std::lock_guard<std::mutex> lk(s->mu);
+	auto it = s->reqs.find(id);
+	if (it == s->reqs.end()) return tmx::set_error(TM_ERR_INVALID, "unknown request %llu", (unsigned long long)id);
+	if (!it->second->done) return tmx::set_error(TM_ERR_BUSY, "request %llu still running", (unsigned long long)id);
+	delete it->second;
+	s->reqs.erase(it);
+	return TM_OK;
+}
+
+int tm_server_stats_get(tm_server * s, tm_server_stats * st) {
+	if (!s || !st) return tmx::set_error(TM_ERR_INVALID, "NULL argument");
+	std::lock_guard<std::mutex> lk(s->mu);
+	*st = s->st;
+	st->reserved_now = s->reserved;
+	st->hbm_budget = s->cfg.hbm_budget;
+	st->n_waiting = s->waiting.size();
+	return TM_OK;
+}
+
+} // extern "C"

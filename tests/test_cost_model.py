@@ -209,19 +209,29 @@ def test_presize_avoids_retry_and_keeps_results(trained, mode):
 
 
 def test_ordered_schedule_matches_oracle(trained):
+    """Tile order from the model (enough tiles that ordering is used) gives
+    the same results as the identity order, which the oracle checks on a
+    sample."""
     d = trained
-    q = d["qb"][:3000]
-    eo, ev = orc.match(d["cw"], d["ck"], d["cv"], q, mode=0)
+    _, _, _, q, _ = tm.workload(200000, 262144, seed=302, query_first=2 * 10**6)
+    n_check = 3000
+    eo, ev = orc.match(d["cw"], d["ck"], d["cv"], q[:n_check], mode=0)
+    base = tm.Ring(d["fib"], len(q), batch_size=64)
+    base.submit(q)
+    base.wait()
+    ob, vb = base.results()
+    assert np.array_equal(ob[:n_check + 1], eo) and np.array_equal(vb[:int(eo[-1])], ev)
     for graph in (0, 1):
         ring = tm.Ring(d["fib"], len(q), batch_size=64, use_graph=graph)
         ring.set_cost_model(d["m"], tm.COST_ORDER)
         for rep in range(3):
             ring.submit(q)
-            ring.wait()
+            st = ring.wait()
             off, vals = ring.results()
-            assert np.array_equal(off, eo) and np.array_equal(vals, ev), (graph, rep)
+            log(f"graph {graph} rep {rep}: {st.match_ms:.2f} ms, model {st.model_ms:.2f} ms")
+            assert np.array_equal(off, ob) and np.array_equal(vals, vb), (graph, rep)
         ring.set_cost_model(None)
         ring.submit(q)
         ring.wait()
         off, vals = ring.results()
-        assert np.array_equal(off, eo) and np.array_equal(vals, ev)
+        assert np.array_equal(off, ob) and np.array_equal(vals, vb)

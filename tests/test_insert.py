@@ -1,0 +1,100 @@
+"""Incremental inserts into a resident FIB (tm_fib_insert) against the
+oracle run on the whole table: spare lanes, new partitions, new keys,
+graphs captured before the insert, and the busy check."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import tagmatch_amd as tm
+
+pytestmark = pytest.mark.gpu
+
+MODES = [(tm.MODE_KEYS_UNIQUE, 0), (tm.MODE_KEYS_ALL, 1), (tm.MODE_HITS, 2)]
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def check_all(fib, words, ko, kv, q, **kw):
+    for mode, om in MODES:
+        eo, ev = orc.match(words, ko, kv, q, mode=om)
+        off, vals, st = tm.match(fib, q, mode=mode, **kw)
+        assert np.array_equal(off, eo), (mode, kw)
+        if mode == tm.MODE_KEYS_UNIQUE:
+            assert np.array_equal(vals, ev), (mode, kw)
+        else:
+            for i in range(len(q)):
+                assert np.array_equal(np.sort(vals[off[i]:off[i + 1]]), ev[eo[i]:eo[i + 1]]), (mode, i, kw)
+
+
+def gen(n_filters, n_queries, seed):
+    g = tm.Generator(seed=seed)
+    w, ko, k = g.filters(n_filters)
+    q, _ = g.queries(n_queries, n_filters)
+    return w, ko, k, q
+
+
+def concat(a_w, a_ko, a_k, b_w, b_ko, b_k):
+    w = np.concatenate([a_w, b_w])
+    ko = np.concatenate([a_ko, a_ko[-1] + b_ko[1:]]).astype(np.uint64)
+    k = np.concatenate([a_k, b_k])
+    return w, ko, k
+
+
+@pytest.mark.parametrize("reserve", [13, 0])
+def test_insert_matches_oracle(gpu, reserve):
+    cw, ck, cv, q, _ = tm.workload(30000, 400, seed=401)
+    bw, bko, bk, _ = gen(4000, 1, seed=402)
+    fib, perm, off, masks = tm.build_fib(cw, ck, cv, 3000, reserve=reserve)
+    p0 = fib.stats()["n_parts"]
+    log(f"reserve {reserve}: {p0} partitions, inserting {len(bw)}")
+    fib.insert(bw, bko, bk, maxp=3000)
+    st = fib.stats()
+    log(f"after insert: {st['n_parts']} partitions, {st['n_filters']} filters")
+    assert st["n_filters"] == len(cw) + len(bw)
+    if reserve == 0:
+        assert st["n_parts"] > p0
+    w, ko, k = concat(cw, ck, cv, bw, bko, bk)
+    for sched in (tm.SCHED_PERSISTENT, tm.SCHED_PLAIN):
+        check_all(fib, w, ko, k, q, schedule=sched)
+
+
+def test_insert_new_keys_and_overflow_lists(gpu):
+    cw, ck, cv, q, _ = tm.workload(20000, 300, seed=403)
+    fib, perm, off, masks = tm.build_fib(cw, ck, cv, 2000)
+    bw, _, _, _ = gen(500, 1, seed=404)
+    rng = np.random.default_rng(5)
+    counts = rng.integers(1, 9, len(bw))            # up to 8 keys: some overflow lists
+    bko = np.zeros(len(bw) + 1, np.uint32)
+    np.cumsum(counts, out=bko[1:])
+    bk = np.concatenate([np.sort(rng.choice(np.arange(7_000_000, 7_000_040), c, replace=False))
+                         for c in counts]).astype(np.uint32)   # keys unknown to the table
+    k0 = fib.stats()["n_distinct_keys"]
+    fib.insert(bw, bko, bk)
+    assert fib.stats()["n_distinct_keys"] == k0 + len(np.unique(bk))
+    w, ko, k = concat(cw, ck, cv, bw, bko, bk)
+    check_all(fib, w, ko, k, q)
+
+
+def test_ring_across_inserts_with_graph(gpu):
+    cw, ck, cv, q, _ = tm.workload(30000, 500, seed=405)
+    fib, perm, off, masks = tm.build_fib(cw, ck, cv, 3000)
+    ring = tm.Ring(fib, len(q), use_graph=1, batch_size=64)
+    w, ko, k = cw, ck, cv
+    for step in range(4):
+        eo, ev = orc.match(w, ko, k, q, mode=0)
+        for rep in range(2):
+            ring.submit(q)
+            ring.wait()
+            o, v = ring.results()
+            assert np.array_equal(o, eo) and np.array_equal(v, ev), (step, rep)
+        bw, bko, bk, _ = gen(1500, 1, seed=500 + step)
+        # a submission in flight blocks the insert
+        ring.submit(q)
+        with pytest.raises(tm.TagMatchError):
+            fib.insert(bw, bko, bk, maxp=3000)
+        ring.wait()
+        fib.insert(bw, bko, bk, maxp=3000)
+        w, ko, k = concat(w, ko, k, bw, bko, bk)
+        log(f"step {step}: {fib.stats()['n_parts']} partitions")

@@ -1,0 +1,111 @@
+"""The serving loop (tm_server): each request's result is the same alone and
+in the mix, admission under a tight HBM budget, and inserts ordered with
+requests."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import tagmatch_amd as tm
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def table(gpu):
+    cw, ck, cv, q, _ = tm.workload(100000, 30000, seed=601)
+    fib, perm, off, masks = tm.build_fib(cw, ck, cv, 5000)
+    return cw, ck, cv, q, fib
+
+
+def request_sizes(rng, n, total):
+    s = np.exp(rng.uniform(0, np.log(2000), n)).astype(int) + 1
+    return s[np.cumsum(s) <= total]
+
+
+@pytest.mark.parametrize("mode", [tm.MODE_KEYS_UNIQUE, tm.MODE_HITS])
+def test_same_alone_and_in_mix(table, mode):
+    cw, ck, cv, q, fib = table
+    rng = np.random.default_rng(7)
+    sizes = request_sizes(rng, 80, len(q))
+    bounds = np.concatenate([[0], np.cumsum(sizes)])
+    srv = tm.Server(fib, n_slots=4, slot_queries=8192, mode=mode)
+    ids = [srv.submit(q[bounds[i]:bounds[i + 1]]) for i in range(len(sizes))]
+    got = []
+    for rid in ids:
+        assert srv.wait(rid, 60000.0)
+        got.append(srv.result(rid))
+    st = srv.stats()
+    srv.close()
+    log(f"mode {mode}: {len(sizes)} requests in {st['n_batches']} steps, "
+        f"{st['requests_per_batch_sum'] / max(st['n_batches'], 1):.2f} requests/step, "
+        f"max latency {st['latency_max_ms']:.1f} ms")
+    assert st["n_requests_done"] == len(sizes)
+    assert st["n_batches"] < len(sizes)          # requests were batched together
+    for i, (info, off, vals) in enumerate(got):
+        qa = q[bounds[i]:bounds[i + 1]]
+        eo, ev, _ = tm.match(fib, qa, mode=mode)
+        assert info.status == 0 and info.n_queries == len(qa)
+        assert info.latency_ms >= info.queue_ms >= 0
+        assert np.array_equal(off, eo), i
+        if mode == tm.MODE_KEYS_UNIQUE:
+            assert np.array_equal(vals, ev), i
+        else:
+            for j in range(len(qa)):
+                assert np.array_equal(np.sort(vals[off[j]:off[j + 1]]), np.sort(ev[eo[j]:eo[j + 1]])), (i, j)
+    # and the oracle on a sample of requests
+    for i in (0, len(sizes) // 2, len(sizes) - 1):
+        qa = q[bounds[i]:bounds[i + 1]]
+        eo, ev = orc.match(cw, ck, cv, qa, mode=0 if mode == tm.MODE_KEYS_UNIQUE else 2)
+        info, off, vals = got[i]
+        assert np.array_equal(off, eo)
+
+
+def test_tight_budget_admits_one_at_a_time(table):
+    cw, ck, cv, q, fib = table
+    srv = tm.Server(fib, n_slots=4, slot_queries=4096, hbm_budget=1)
+    ids = [srv.submit(q[i * 500:(i + 1) * 500]) for i in range(12)]
+    res = []
+    for rid in ids:
+        assert srv.wait(rid, 60000.0)
+        res.append(srv.result(rid))
+    st = srv.stats()
+    srv.close()
+    log(f"budget 1 byte: {st['n_batches']} steps, {st['n_budget_waits']} deferred admissions")
+    assert st["n_batches"] == 12                 # one request per step, one step in flight
+    assert st["n_budget_waits"] > 0
+    eo, ev, _ = tm.match(fib, q[:6000])
+    for i, (info, off, vals) in enumerate(res):
+        a, b = eo[i * 500], eo[(i + 1) * 500]
+        assert np.array_equal(off, eo[i * 500:(i + 1) * 500 + 1] - a)
+        assert np.array_equal(vals, ev[a:b])
+
+
+def test_inserts_ordered_with_requests(gpu):
+    cw, ck, cv, q, _ = tm.workload(40000, 600, seed=602)
+    fib, perm, off, masks = tm.build_fib(cw, ck, cv, 4000)
+    g = tm.Generator(seed=603)
+    bw, bko, bk = g.filters(3000)
+    srv = tm.Server(fib, n_slots=3, slot_queries=1024)
+    before = srv.submit(q)
+    ins = srv.insert(bw, bko, bk, maxp=4000)
+    after = srv.submit(q)
+    for rid in (before, ins, after):
+        assert srv.wait(rid, 60000.0)
+    _, o1, v1 = srv.result(before)
+    assert srv.info(ins).status == 0
+    _, o2, v2 = srv.result(after)
+    st = srv.stats()
+    srv.close()
+    assert st["n_inserts"] == 1
+    eo, ev = orc.match(cw, ck, cv, q, mode=0)
+    assert np.array_equal(o1, eo) and np.array_equal(v1, ev)
+    w = np.concatenate([cw, bw])
+    ko = np.concatenate([ck, ck[-1] + bko[1:]]).astype(np.uint64)
+    kv = np.concatenate([cv, bk])
+    eo2, ev2 = orc.match(w, ko, kv, q, mode=0)
+    assert np.array_equal(o2, eo2) and np.array_equal(v2, ev2)
+    assert not np.array_equal(o1, o2)            # the insert changed some results
