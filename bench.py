@@ -9,6 +9,15 @@ partition.  A step is one pass of the whole engine over the 1M queries
 already resident in HBM: on-device partition lookup, batched subset match,
 key expansion and per-query dedup into sorted key lists.
 
+Beside the headline step the run measures, on the same resident table:
+  * the plain single-launch form (one launch per (partition, 256-query
+    batch), the reference's dispatch shape) on a prefix of the queries;
+  * the learned cost model: trained on the plain form's own unit logs,
+    held-out MAPE, and the persistent step priced by it;
+  * the serving loop: Poisson arrivals of 64-query requests at 30/70/95% of
+    its measured peak, with 10k-filter inserts every 100 ms; p50/p99/p999;
+  * the CPU baseline: the oracle's CPU schedule on a bounded sample.
+
 Prints one JSON line (rank 0).  --gpus above 1 is refused: this engine runs
 on one GPU.
 """
@@ -43,17 +52,26 @@ def parse():
     ap.add_argument("--filters", type=int, default=10_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000)
     ap.add_argument("--maxp", type=int, default=256 * 1024)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=256, help="queries per (partition, batch) of the plain form")
+    ap.add_argument("--tile", type=int, default=0, help="queries per persistent work item (0: default)")
     ap.add_argument("--chunk", type=int, default=0, help="blocks per work item (0: default)")
     ap.add_argument("--groups-per-cu", type=int, default=0)
     ap.add_argument("--schedule", choices=["persistent", "plain"], default="persistent")
     ap.add_argument("--graph", action="store_true", help="replay steps from a hipGraph")
     ap.add_argument("--plain-steps", type=int, default=1, help="steps of the plain single-launch form")
-    ap.add_argument("--plain-queries", type=int, default=65536,
+    ap.add_argument("--plain-queries", type=int, default=16384,
                     help="queries of the plain form's step (a prefix of the resident queries)")
+    ap.add_argument("--model", type=int, default=1, help="cost model leg (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds per serving load level (0: skip)")
+    ap.add_argument("--serve-request", type=int, default=64, help="queries per serving request")
+    ap.add_argument("--serve-insert", type=int, default=10000, help="filters per insert (0: none)")
+    ap.add_argument("--serve-insert-ms", type=float, default=100.0, help="interval between inserts")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
     ap.add_argument("--seed", type=lambda x: int(x, 0), default=0x7461676D61746302)
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run an elementwise device copy of MIB MiB (read + write) three times, "
+                         "to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE, and exit")
     return ap.parse_args()
 
 
@@ -104,8 +122,154 @@ def traffic_from_profiles():
     return None
 
 
+def unit_logs(fib, q, batches):
+    """Plain-form unit logs (features, hits, ticks) over q at each batch size."""
+    ring = tm.Ring(fib, len(q), schedule=tm.SCHED_PLAIN, log_units=1)
+    cols = [[], [], [], []]
+    for bs in batches:
+        ring.configure(schedule=tm.SCHED_PLAIN, batch_size=bs, log_units=1)
+        ring.submit(q)
+        ring.wait()
+        for c, x in zip(cols, ring.unit_log()[:4]):
+            c.append(x)
+    ring.close()
+    return tuple(np.concatenate(c) for c in cols)
+
+
+def cost_model_leg(fib, q, ring, nq, steps, base_ms, n_log):
+    """Train the MLP on the plain form's unit logs (first n_log queries),
+    report held-out MAPE (next n_log queries) against a rate model, and time
+    the persistent step priced by the model."""
+    t0 = time.time()
+    fa, ha, ka, ta = unit_logs(fib, q[:n_log], (256, 128))
+    fb, hb, kb, tb = unit_logs(fib, q[n_log:2 * n_log], (256,))
+    t1 = time.time()
+    m = tm.MLP(seed=5)
+    loss = m.train(fa, tm.encode_targets(ha, ka, ta), epochs=150, batch=256, lr=2e-3)
+    m.calibrate(fa, ha, ka, ta)
+    t2 = time.time()
+    ph, pk, pt = tm.decode_targets(m.forward(fb))
+
+    def size(f):
+        return (np.exp2(f[:, 2] * 12) - 1) * (np.exp2(f[:, 0] * 20) - 1)
+
+    rate_h = ha.sum() / size(fa).sum()
+    rate_t = ta.sum() / size(fa).sum()
+    res = {
+        "train_units": int(len(fa)), "heldout_units": int(len(fb)), "train_loss": loss,
+        "mape_hits": tm.mape(ph, hb, floor=16), "mape_keys": tm.mape(pk, kb, floor=16),
+        "mape_ticks": tm.mape(pt, tb, floor=16),
+        "rate_model_mape_hits": tm.mape(rate_h * size(fb), hb, floor=16),
+        "rate_model_mape_ticks": tm.mape(rate_t * size(fb), tb, floor=16),
+        "total_hits_error": float(ph.sum() / max(hb.sum(), 1) - 1.0),
+        "log_s": t1 - t0, "train_s": t2 - t1,
+    }
+    log(f"cost model: {res}")
+    # the headline step priced by the model (tile order + pre-sizing)
+    ring.set_cost_model(m)
+    ring.submit_resident(nq)
+    st = ring.wait()
+    tt = []
+    for i in range(steps):
+        ts = time.perf_counter()
+        ring.submit_resident(nq)
+        st = ring.wait()
+        tt.append((time.perf_counter() - ts) * 1e3)
+    ring.set_cost_model(None)
+    res.update({"ms_per_step_with_model": float(np.mean(tt)), "ms_per_step_without": base_ms,
+                "speedup_from_model": base_ms / float(np.mean(tt)), "pricing_ms": st.model_ms,
+                "pred_hits_error": st.pred_hits / max(st.n_hits, 1) - 1.0})
+    log(f"model-priced step: {np.mean(tt):.1f} ms vs {base_ms:.1f} ms")
+    m.close()
+    return res
+
+
+def serving_leg(fib, args, n_filters):
+    """Poisson arrivals of fixed-size requests through tm_server at 30/70/95%
+    of the closed-loop peak, with inserts interleaved."""
+    req = args.serve_request
+    g = tm.Generator(seed=args.seed ^ 0x5E, superset_frac=0.2)
+    pool, _ = g.queries(1 << 17, n_filters)
+    gi = tm.Generator(seed=args.seed)
+    srv = tm.Server(fib, n_slots=4, slot_queries=16384)
+    rng = np.random.default_rng(11)
+
+    def take(i):
+        a = (i * req) % (len(pool) - req)
+        return pool[a:a + req]
+
+    # closed-loop peak: a backlog of requests drained as fast as possible
+    nback = max(64, int(300000 / req))
+    t = time.perf_counter()
+    ids = [srv.submit(take(i)) for i in range(nback)]
+    for rid in ids:
+        srv.wait(rid)
+    peak = nback * req / (time.perf_counter() - t)
+    for rid in ids:
+        srv.release(rid)
+    log(f"serving peak: {peak:.0f} queries/s with {req}-query requests")
+
+    levels = {}
+    ins_first = n_filters
+    for load in (0.3, 0.7, 0.95):
+        rate = load * peak / req                    # requests per second
+        dur = args.serve_seconds
+        n = int(rate * dur)
+        gaps = rng.exponential(1.0 / rate, n)
+        arrive = np.cumsum(gaps)
+        ids, inserts = [], []
+        next_ins = args.serve_insert_ms / 1e3
+        t0 = time.perf_counter()
+        for i in range(n):
+            now = time.perf_counter() - t0
+            if args.serve_insert and now >= next_ins:
+                w, ko, k = gi.filters(args.serve_insert, first=ins_first)
+                ins_first += args.serve_insert
+                inserts.append(srv.insert(w, ko.astype(np.uint32), k, maxp=args.maxp))
+                next_ins += args.serve_insert_ms / 1e3
+            while now < arrive[i]:
+                now = time.perf_counter() - t0
+            ids.append(srv.submit(take(i)))
+        for rid in ids + inserts:
+            srv.wait(rid)
+        wall = time.perf_counter() - t0
+        lat = np.array([srv.info(rid).latency_ms for rid in ids])
+        for rid in ids + inserts:
+            srv.release(rid)
+        levels[f"{int(load * 100)}%"] = {
+            "offered_qps": load * peak, "achieved_qps": n * req / wall, "requests": n,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "p999_ms": float(np.percentile(lat, 99.9)), "max_ms": float(lat.max()),
+            "inserts": len(inserts)}
+        log(f"serving {load:.0%}: {levels[f'{int(load * 100)}%']}")
+    st = srv.stats()
+    srv.close()
+    return {"peak_qps": peak, "request_queries": req, "slots": 4, "slot_queries": 16384,
+            "insert_filters": args.serve_insert, "insert_every_ms": args.serve_insert_ms,
+            "levels": levels, "steps": st["n_batches"], "budget_waits": st["n_budget_waits"],
+            "requests_per_step": st["requests_per_batch_sum"] / max(st["n_batches"], 1)}
+
+
+def copy_calibration(mib):
+    """A kernel of known traffic: y = x + 1 over mib MiB of int32 (reads and
+    writes mib MiB each), for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = (mib << 20) // 4
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.add(x, 1, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "launches": 3}), flush=True)
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} (WORLD_SIZE {world}) is not supported",
@@ -115,8 +279,10 @@ def main():
     cw, cko, ck, q, fib, perm, part_off, masks, setup_s = setup(args)
     nq = q.shape[0]
     sched = tm.SCHED_PERSISTENT if args.schedule == "persistent" else tm.SCHED_PLAIN
-    cfg = tm.config(mode=tm.MODE_KEYS_UNIQUE, schedule=sched, batch_size=args.batch, chunk_blocks=args.chunk,
-                    groups_per_cu=args.groups_per_cu, use_graph=int(args.graph))
+    kw = dict(chunk_blocks=args.chunk, groups_per_cu=args.groups_per_cu, use_graph=int(args.graph))
+    if sched == tm.SCHED_PLAIN or args.tile:
+        kw["batch_size"] = args.batch if sched == tm.SCHED_PLAIN else args.tile
+    cfg = tm.config(mode=tm.MODE_KEYS_UNIQUE, schedule=sched, **kw)
     ring = tm.Ring(fib, nq, cfg)
     # upload the queries once; every later step reads them from HBM
     t = time.time()
@@ -171,7 +337,15 @@ def main():
             log(f"plain step {i}: {npq} queries in {tt[-1] * 1e3:.1f} ms, {pst.n_items} launches")
         plain = {"queries_per_s": npq / np.mean(tt), "ms_per_step": 1e3 * np.mean(tt), "launches": pst.n_items,
                  "queries": npq, "match_ms": pst.match_ms}
+    model = None
+    if args.model and sched == tm.SCHED_PERSISTENT:
+        ring.configure(cfg)
+        model = cost_model_leg(fib, q, ring, nq, max(1, args.steps // 2), ms_per_step, 4096)
     ring.close()
+
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serving_leg(fib, args, args.filters)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -201,8 +375,12 @@ def main():
         "matches_per_s": n_hits / (ms_per_step / 1e3),
         "keys_per_s": st["n_results"] / (ms_per_step / 1e3),
         "hits_per_query": n_hits / nq,
-        "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
-                       "what": "completion time of a 1M-query step"},
+        "p99_latency_ms": serving["levels"]["70%"]["p99_ms"] if serving else None,
+        "latency_note": "p99 of 64-query requests through the serving loop at 70% of its peak, "
+                        "10k-filter inserts every 100 ms",
+        "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)), "max": float(np.max(step_ms))},
+        "serving": serving,
+        "cost_model": model,
         "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_match", "kernel_ms": kernel_s * 1e3,

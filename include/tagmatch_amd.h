@@ -179,8 +179,10 @@ typedef struct tm_ring tm_ring;
 typedef struct tm_match_config {
 	uint32_t mode;           /* TM_MODE_*                                   */
 	uint32_t schedule;       /* TM_SCHED_*                                  */
-	uint32_t batch_size;     /* PLAIN: queries per (partition, batch) launch;
-	                            0 = 256 (reference parameters.hh:22)        */
+	uint32_t batch_size;     /* PLAIN: queries per (partition, batch) launch,
+	                            0 = 256 (reference parameters.hh:22);
+	                            PERSISTENT: queries per work item (64, 128
+	                            or 256), 0 = 128                             */
 	uint32_t chunk_blocks;   /* blocks per work item; 0 = autotuned          */
 	uint32_t waves_per_group;/* waves per workgroup (1..16); 0 = autotuned   */
 	uint32_t groups_per_cu;  /* PERSISTENT: resident workgroups per CU; 0 = auto */
@@ -190,12 +192,17 @@ typedef struct tm_match_config {
 	uint32_t log_units;      /* record per-work-item hit counts and cycles   */
 	uint32_t order_by_cost;  /* PERSISTENT: visit partitions in the order set
 	                            by tm_ring_set_order                          */
-	uint32_t variant;        /* PERSISTENT inner loop, bit 0: candidate
-	                            query words from scalar loads, four tests per
-	                            iteration (else LDS, two); bit 1: hits go
-	                            through a per-wave key log drained 64 lanes
-	                            at a time; other values: default (2)         */
-	uint32_t reserved[6];
+	uint32_t variant;        /* PERSISTENT inner loop, 1..7 as bits; bit 0:
+	                            candidate query words from scalar loads
+	                            (else LDS broadcasts); bit 1: KEYS_UNIQUE
+	                            hits go through a per-wave key log drained
+	                            64 lanes at a time; bit 2: early exit after
+	                            words 0-1; 0 or other values: default (2)    */
+	uint32_t full_exit;      /* PERSISTENT KEYS_UNIQUE: stop testing a query
+	                            once its key set holds every key of the
+	                            table (results unchanged); 0 = default (on),
+	                            1 = on, 2 = off                              */
+	uint32_t reserved[5];
 } tm_match_config;
 
 void tm_match_config_default(tm_match_config * cfg);
@@ -227,6 +234,15 @@ int tm_ring_create(tm_fib * fib, uint32_t max_queries, const tm_match_config * c
 				   tm_ring ** out);
 int tm_ring_destroy(tm_ring * ring);
 int tm_ring_config(tm_ring * ring, const tm_match_config * cfg);
+
+/* Autotuner: run the nq queries already in the ring (a previous submission
+ * or tm_ring_query_buffer) under each candidate configuration (same mode),
+ * reps timed steps each after one warm-up, and keep the fastest by device
+ * time; *best = its index, times_ms (may be NULL) = each candidate's best
+ * step time.  Results do not depend on the configuration.  No counterpart
+ * in the reference, whose launch shape is fixed (gpu.cu:878-912). */
+int tm_ring_autotune(tm_ring * ring, uint32_t nq, const tm_match_config * cands, uint32_t n_cands, uint32_t reps,
+					 uint32_t * best, double * times_ms);
 
 /* Device pointer of the ring's query buffer (max_queries x 8 words: 6 filter
  * words + 2 pad), for callers that keep queries resident in HBM. */
@@ -267,10 +283,11 @@ int tm_match(tm_fib * fib, const uint32_t * query_words, uint32_t nq,
  *
  * Work units are (partition, query batch) pairs.  The plain schedule logs one
  * row per launch when cfg.log_units is set: TM_UNIT_FEATURES float features,
- * the unit's hits and its summed workgroup time in 10 ns ticks.  A
- * TM_UNIT_FEATURES-64-64-2 multilayer perceptron (FP32, ReLU) trained on
- * those rows with targets log2(1 + hits) / 16 and log2(1 + ticks) / 16
- * prices the persistent schedule's (query tile, partition) units.  Its
+ * the unit's hits, the key instances of its hit filters and its summed
+ * workgroup time in 10 ns ticks.  A TM_UNIT_FEATURES-64-64-3 multilayer
+ * perceptron (FP32, ReLU) trained on those rows with targets
+ * log2(1 + x) / 16 of (hits, keys, ticks) prices the persistent schedule's
+ * (query tile, partition) units.  Its
  * predictions pre-size result buffers and order work items; they never
  * skip work, and overflow-retry still catches an under-estimate.
  *
@@ -302,13 +319,14 @@ int tm_mlp_train(tm_mlp * m, const float * x, const float * t, uint32_t n, uint3
 				 float lr, uint64_t seed, float * loss);
 
 /* Unit log of the ring's last plain submission (cfg.log_units): up to
- * capacity rows of features (TM_UNIT_FEATURES floats), hits, ticks and the
- * unit itself (3 words: partition, batch index, queries in the batch);
- * *n_units = rows available.  Any output may be NULL. */
-int tm_ring_unit_log(tm_ring * ring, float * features, uint32_t * hits, uint32_t * ticks,
+ * capacity rows of features (TM_UNIT_FEATURES floats), hits, key instances
+ * of the hit filters, ticks and the unit itself (3 words: partition, batch
+ * index, queries in the batch); *n_units = rows available.  Any output may
+ * be NULL. */
+int tm_ring_unit_log(tm_ring * ring, float * features, uint32_t * hits, uint32_t * keys, uint32_t * ticks,
 					 uint32_t * units, uint32_t capacity, uint32_t * n_units);
 
-/* Use a trained TM_UNIT_FEATURES-64-64-2 model in the ring's PERSISTENT
+/* Use a trained TM_UNIT_FEATURES-64-64-3 model in the ring's PERSISTENT
  * submissions (NULL: none).  The model must outlive its use by the ring. */
 #define TM_COST_PRESIZE 1u  /* grow result buffers to the predicted size first */
 #define TM_COST_ORDER   2u  /* visit query tiles longest-predicted first        */

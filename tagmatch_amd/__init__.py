@@ -52,7 +52,7 @@ class MatchConfig(C.Structure):
         ("chunk_blocks", C.c_uint32), ("waves_per_group", C.c_uint32),
         ("groups_per_cu", C.c_uint32), ("hit_capacity", C.c_uint64), ("use_graph", C.c_uint32),
         ("log_units", C.c_uint32), ("order_by_cost", C.c_uint32), ("variant", C.c_uint32),
-        ("reserved", C.c_uint32 * 6),
+        ("full_exit", C.c_uint32), ("reserved", C.c_uint32 * 5),
     ]
 
 
@@ -137,7 +137,8 @@ _SIGS = {
     "tm_mlp_loss_grad": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(C.c_float), P]),
     "tm_mlp_train": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
                                C.POINTER(C.c_float)]),
-    "tm_ring_unit_log": (C.c_int, [P, P, P, P, P, C.c_uint32, U32P]),
+    "tm_ring_autotune": (C.c_int, [P, C.c_uint32, C.POINTER(MatchConfig), C.c_uint32, C.c_uint32, U32P, P]),
+    "tm_ring_unit_log": (C.c_int, [P, P, P, P, P, P, C.c_uint32, U32P]),
     "tm_ring_set_cost_model": (C.c_int, [P, P, C.c_uint32]),
     "tm_server_config_default": (None, [C.POINTER(ServerConfig)]),
     "tm_server_create": (C.c_int, [P, C.POINTER(ServerConfig), C.POINTER(MatchConfig), C.POINTER(P)]),
@@ -443,20 +444,33 @@ class Ring:
         check(load().tm_ring_lookup_counts(self.h, _ptr(c), n), "tm_ring_lookup_counts")
         return c[:n]
 
+    def autotune(self, nq, candidates, reps=1):
+        """Pick the fastest of `candidates` (MatchConfig or dicts of config
+        fields) on the nq resident queries; returns (index, times_ms)."""
+        cands = [c if isinstance(c, MatchConfig) else config(mode=self.cfg.mode, **c) for c in candidates]
+        arr = (MatchConfig * len(cands))(*cands)
+        best = C.c_uint32()
+        times = np.zeros(len(cands), np.float64)
+        check(load().tm_ring_autotune(self.h, nq, arr, len(cands), reps, C.byref(best), _ptr(times)),
+              "tm_ring_autotune")
+        self.cfg = cands[best.value]
+        return best.value, times
+
     def unit_log(self):
-        """(features [n,16] f32, hits [n] u32, ticks [n] u32, units [n,3]
-        u32 = partition, batch, queries) of the last plain submission made
-        with log_units=1."""
+        """(features [n,16] f32, hits [n], keys [n], ticks [n], units [n,3]
+        = partition, batch, queries) of the last plain submission made with
+        log_units=1."""
         n = C.c_uint32()
-        check(load().tm_ring_unit_log(self.h, None, None, None, None, 0, C.byref(n)), "tm_ring_unit_log")
+        check(load().tm_ring_unit_log(self.h, None, None, None, None, None, 0, C.byref(n)), "tm_ring_unit_log")
         k = n.value
         f = np.zeros((k, UNIT_FEATURES), np.float32)
         h = np.zeros(k, np.uint32)
+        kk = np.zeros(k, np.uint32)
         t = np.zeros(k, np.uint32)
         u = np.zeros((k, 3), np.uint32)
-        check(load().tm_ring_unit_log(self.h, _ptr(f), _ptr(h), _ptr(t), _ptr(u), k, C.byref(n)),
+        check(load().tm_ring_unit_log(self.h, _ptr(f), _ptr(h), _ptr(kk), _ptr(t), _ptr(u), k, C.byref(n)),
               "tm_ring_unit_log")
-        return f, h, t, u
+        return f, h, kk, t, u
 
     def set_cost_model(self, mlp, flags=None):
         """Price PERSISTENT submissions with `mlp` (None: stop)."""
@@ -589,22 +603,23 @@ COST_PRESIZE = 1
 COST_ORDER = 2
 
 
-def encode_targets(hits, ticks):
-    """Training targets of the cost model: log2(1 + x) / 16."""
-    return np.stack([np.log2(1.0 + np.asarray(hits, np.float64)) / 16.0,
-                     np.log2(1.0 + np.asarray(ticks, np.float64)) / 16.0], axis=1).astype(np.float32)
+def encode_targets(hits, keys, ticks):
+    """Training targets of the cost model: log2(1 + x) / 16 of (hits, key
+    instances, ticks)."""
+    return np.stack([np.log2(1.0 + np.asarray(x, np.float64)) / 16.0 for x in (hits, keys, ticks)],
+                    axis=1).astype(np.float32)
 
 
 def decode_targets(y):
-    """Inverse of encode_targets: (hits, ticks) as float64."""
+    """Inverse of encode_targets: (hits, keys, ticks) as float64."""
     y = np.maximum(np.asarray(y, np.float64), 0.0)
-    return np.exp2(16.0 * y[:, 0]) - 1.0, np.exp2(16.0 * y[:, 1]) - 1.0
+    return tuple(np.exp2(16.0 * y[:, i]) - 1.0 for i in range(3))
 
 
 class MLP:
     """n_in -> h -> h -> n_out FP32 perceptron on the GPU (tm_mlp)."""
 
-    def __init__(self, n_in=UNIT_FEATURES, n_hidden=64, n_out=2, seed=1):
+    def __init__(self, n_in=UNIT_FEATURES, n_hidden=64, n_out=3, seed=1):
         self.shape = (n_in, n_hidden, n_out)
         self.h = C.c_void_p()
         check(load().tm_mlp_create(n_in, n_hidden, n_out, seed, C.byref(self.h)), "tm_mlp_create")
@@ -641,6 +656,23 @@ class MLP:
         check(load().tm_mlp_train(self.h, _ptr(x), _ptr(t), x.shape[0], epochs, batch, lr, seed,
                                   C.byref(loss)), "tm_mlp_train")
         return float(loss.value)
+
+    def calibrate(self, x, *targets):
+        """Shift each output's bias so that the decoded predictions sum to
+        the measured totals on (x, targets): a model fitted in log space
+        predicts geometric means, which under-estimate sums of skewed
+        quantities (hits, keys, ticks) that buffer sizing needs."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.shape[0])
+        y = self.forward(x).astype(np.float64)
+        p = self.params()
+        n_in, h, n_out = self.shape
+        b3 = n_in * h + h + h * h + h + h * n_out
+        for i, t in enumerate(targets[:n_out]):
+            pred = np.exp2(16.0 * np.maximum(y[:, i], 0.0)) - 1.0
+            tot, ptot = float(np.sum(t)), float(pred.sum())
+            if tot > 0 and ptot > 0:
+                p[b3 + i] += np.float32(np.log2(tot / ptot) / 16.0)
+        self.set_params(p)
 
     def close(self):
         if self.h:

@@ -26,10 +26,10 @@ namespace tmx {
 //   13 mean popcount(query AND OR-of-members) / 192
 //   14 max popcount(query AND OR-of-members) / 192
 //   15 log2(1 + B * partition filters) / 32
-// Targets: log2(1 + hits) / 16 and log2(1 + ticks) / 16, ticks = summed
-// workgroup busy time in 10 ns units.
+// Targets: log2(1 + x) / 16 of hits, key instances of the hit filters, and
+// ticks = summed workgroup busy time in 10 ns units.
 constexpr int N_FEAT = TM_UNIT_FEATURES;
-constexpr int N_TARGET = 2;
+constexpr int N_TARGET = 3;
 
 // Plain units: units[u] = (partition, batch index, queries in the batch, 0);
 // the batch's query ids are list[pbase[p] + batch * bq ...].
@@ -42,14 +42,14 @@ void launch_unit_features_plain(const dev_fib_view & f, const uint32_t * q, cons
 void launch_unit_features_tiles(const dev_fib_view & f, const uint32_t * q, uint32_t nq, uint32_t nqt,
 								uint32_t n_tiles, float * out, hipStream_t s);
 
-// Batched inference of the fused 16-64-64-2 network on device rows; y gets
-// n x 2 floats.  Returns TM_ERR_INVALID for another shape.
+// Batched inference of the fused 16-64-64-3 network on device rows; y gets
+// n x 3 floats.  Returns TM_ERR_INVALID for another shape.
 bool mlp_fused_shape(const tm_mlp * m);
 int mlp_infer(const tm_mlp * m, const float * x, uint32_t n, float * y, hipStream_t s);
 
-// Per tile: cost[t] = sum over partitions of the decoded predicted ticks,
-// hits[t] = sum of the decoded predicted hits (empty units count zero).
+// Per tile: sums over partitions of the decoded predicted ticks (cost),
+// hits and key instances (empty units count zero).
 void launch_tile_costs(const float * x, const float * y, uint32_t n_tiles, uint32_t P, float * cost, float * hits,
-					   hipStream_t s);
+					   float * keys, hipStream_t s);
 
 } // namespace tmx

@@ -373,18 +373,20 @@ k_mlp_fused(const float * __restrict__ P, const float * __restrict__ X, uint32_t
 // One thread per tile: decoded sums over its partitions.
 __global__ void __launch_bounds__(256)
 k_tile_costs(const float * __restrict__ X, const float * __restrict__ Y, uint32_t n_tiles, uint32_t P,
-			 float * __restrict__ cost, float * __restrict__ hits) {
+			 float * __restrict__ cost, float * __restrict__ hits, float * __restrict__ keys) {
 	const uint32_t t = blockIdx.x * 256 + threadIdx.x;
 	if (t >= n_tiles) return;
-	float c = 0.f, h = 0.f;
+	float c = 0.f, h = 0.f, k = 0.f;
 	for (uint32_t p = 0; p < P; ++p) {
 		const size_t r = (size_t)t * P + p;
 		if (X[r * N_FEAT + 2] <= 0.f) continue;   // no query of the tile selects p
-		h += exp2f(16.f * fmaxf(Y[r * 2], 0.f)) - 1.f;
-		c += exp2f(16.f * fmaxf(Y[r * 2 + 1], 0.f)) - 1.f;
+		h += exp2f(16.f * fmaxf(Y[r * N_TARGET], 0.f)) - 1.f;
+		k += exp2f(16.f * fmaxf(Y[r * N_TARGET + 1], 0.f)) - 1.f;
+		c += exp2f(16.f * fmaxf(Y[r * N_TARGET + 2], 0.f)) - 1.f;
 	}
 	cost[t] = c;
 	hits[t] = h;
+	keys[t] = k;
 }
 
 } // namespace
@@ -416,9 +418,9 @@ int mlp_infer(const tm_mlp * m, const float * x, uint32_t n, float * y, hipStrea
 }
 
 void launch_tile_costs(const float * x, const float * y, uint32_t n_tiles, uint32_t P, float * cost, float * hits,
-					   hipStream_t s) {
+					   float * keys, hipStream_t s) {
 	if (n_tiles == 0) return;
-	hipLaunchKernelGGL(k_tile_costs, dim3((n_tiles + 255) / 256), dim3(256), 0, s, x, y, n_tiles, P, cost, hits);
+	hipLaunchKernelGGL(k_tile_costs, dim3((n_tiles + 255) / 256), dim3(256), 0, s, x, y, n_tiles, P, cost, hits, keys);
 }
 
 } // namespace tmx

@@ -171,9 +171,17 @@ struct match_args {
 	uint2 * hits;                             // HITS: (query, block slot)
 	uint64_t hit_cap;
 	unsigned long long * counters;            // [0] hit slots taken, [1] hits
-	uint32_t * log;                           // per item: hits, 10 ns ticks
+	uint32_t * log;                           // per unit: hits, key instances, 10 ns ticks
 	uint64_t log_cap;
 };
+
+// Number of keys in a slot (inline ids or an overflow list).
+__device__ inline uint32_t key_count(uint64_t kv, const uint32_t * __restrict__ ovf) {
+	if ((kv & 0xFFFFu) == KEY_NONE) return ovf[(uint32_t)(kv >> 16)];
+	uint32_t n = 0;
+	for (int x = 0; x < 4; ++x) n += ((kv >> (16 * x)) & 0xFFFFu) != KEY_NONE;
+	return n;
+}
 
 // OR the dense key ids of a filter into a bitmap (LDS or global).
 __device__ inline void add_keys(uint32_t * bm, uint64_t kv, const uint32_t * __restrict__ ovf) {
@@ -210,7 +218,7 @@ k_match(match_args a) {
 	__shared__ uint32_t s_q[NQ * 8];      // per query: 6 inverted words, query index, pad
 	__shared__ uint32_t s_item[5];        // partition, batch, chunk, queries in batch, done
 	__shared__ unsigned long long s_id;
-	__shared__ uint32_t s_hits;
+	__shared__ uint32_t s_hits, s_keys;
 	extern __shared__ uint32_t s_bm[];    // STAGE: NQ x kw
 	const uint32_t tid = threadIdx.x, lane = tid & 63;
 	const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -222,6 +230,7 @@ k_match(match_args a) {
 		if (tid == 0) {
 			s_item[4] = 1;
 			s_hits = 0;
+			s_keys = 0;
 			if (a.plain_rank >= 0) {
 				if (iter == 0) {
 					uint4 sc = a.sched[a.plain_rank];
@@ -299,7 +308,7 @@ k_match(match_args a) {
 		const uint32_t fb = pi[6], nb = pi[7];
 		const uint32_t b0 = fb + chunk * a.chunk;
 		const uint32_t b1 = min(fb + nb, b0 + a.chunk);
-		uint32_t hit_acc = 0;
+		uint32_t hit_acc = 0, key_acc = 0;   // key_acc: key instances of this lane's hits (log only)
 
 		for (uint32_t b = b0 + wave; b < b1; b += WAVES) {
 			const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
@@ -337,6 +346,7 @@ k_match(match_args a) {
 					if (h == 0) continue;
 					hit_acc += (uint32_t)__popcll(h);
 					const bool mine = (h >> lane) & 1;
+					if (a.log && mine) key_acc += key_count(a.f.slot_keys[(size_t)b * 64 + lane], a.f.ovf);
 					if (MODE == 0) {
 						if (!kv_loaded) {
 							kv = a.f.slot_keys[(size_t)b * 64 + lane];
@@ -360,6 +370,10 @@ k_match(match_args a) {
 		}
 
 		if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
+		if (a.log) {
+			for (int d = 32; d > 0; d >>= 1) key_acc += __shfl_xor(key_acc, d, 64);
+			if (lane == 0 && key_acc) atomicAdd(&s_keys, key_acc);
+		}
 		__syncthreads();
 		if (STAGE) {
 			for (uint32_t w = tid; w < (uint32_t)NQ * kw; w += WG) {
@@ -375,11 +389,13 @@ k_match(match_args a) {
 			if (a.log && s_id < a.log_cap) {
 				const uint32_t ticks = (uint32_t)(wall_clock64() - t_start);
 				if (a.plain_rank >= 0) {   // the unit's chunks add up
-					atomicAdd(&a.log[s_id * 2], s_hits);
-					atomicAdd(&a.log[s_id * 2 + 1], ticks);
+					atomicAdd(&a.log[s_id * 3], s_hits);
+					atomicAdd(&a.log[s_id * 3 + 1], s_keys);
+					atomicAdd(&a.log[s_id * 3 + 2], ticks);
 				} else {
-					a.log[s_id * 2] = s_hits;
-					a.log[s_id * 2 + 1] = ticks;
+					a.log[s_id * 3] = s_hits;
+					a.log[s_id * 3 + 1] = s_keys;
+					a.log[s_id * 3 + 2] = ticks;
 				}
 			}
 		}
@@ -425,8 +441,11 @@ struct mq_args {
 	uint64_t hit_cap;
 	unsigned long long * counters;            // [0] hit slots, [1] hits, [2] block visits,
 	                                          // [3] subset tests, [4] tests with hits
-	uint32_t * log;                           // per item: hits, 10 ns ticks
+	uint32_t * log;                           // per item: hits, 0, 10 ns ticks
 	uint64_t log_cap;
+	uint32_t full_exit;                       // KEYS_UNIQUE + LDS bitmaps: skip queries whose
+	                                          // bitmap holds every key
+	uint32_t last_mask;                       // valid bits of a bitmap row's last word
 };
 
 constexpr int HBUF = 256;                     // staged hits per wave (HITS mode)
@@ -647,6 +666,36 @@ k_match_q(mq_args a) {
 			}
 			if (wave == 0 && lane == 0 && a.pcount && counts_here && nsel) atomicAdd(&a.pcount[p], nsel);
 			if (anyq == 0 || nb == 0) continue;
+			if (MODE == 0 && LDS_BM && a.full_exit) {
+				// A query whose key bitmap already holds every key of the
+				// dictionary cannot gain anything: drop it from the rest of
+				// the item.  Bitmaps only grow, so a stale read only delays
+				// the drop.
+				anyq = 0;
+#pragma unroll
+				for (int t = 0; t < TILES; ++t) {
+					if (qsel[t] == 0) continue;
+					// full: words 0..kw-2 all ones, word kw-1 holds the last K % 32 keys
+					const uint32_t * row = &s_bm[(t * 64 + lane) * kw];
+					uint32_t all = 0xFFFFFFFFu;
+					if ((kw & 3) == 0) {
+						const uint4 * r4 = reinterpret_cast<const uint4 *>(row);
+						for (uint32_t w4 = 0; w4 + 1 < (kw >> 2); ++w4) {
+							const uint4 v = r4[w4];
+							all &= v.x & v.y & v.z & v.w;
+						}
+						const uint4 v = r4[(kw >> 2) - 1];
+						all &= v.x & v.y & v.z & (v.w | ~a.last_mask);
+					} else {
+						for (uint32_t w = 0; w + 1 < kw; ++w) all &= row[w];
+						all &= row[kw - 1] | ~a.last_mask;
+					}
+					const bool full = all == 0xFFFFFFFFu;
+					qsel[t] &= ~__ballot(full);
+					anyq |= qsel[t];
+				}
+				if (anyq == 0) continue;
+			}
 			for (uint32_t b = fb + wave; b < fb + nb; b += NW) {
 				const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
 				const uint4 ma = sld4(bi);
@@ -758,8 +807,9 @@ k_match_q(mq_args a) {
 		if (tid == 0) {
 			if (s_hits) atomicAdd(&a.counters[1], (unsigned long long)s_hits);
 			if (a.log && item0 < a.log_cap) {
-				a.log[(size_t)item0 * 2] = s_hits;
-				a.log[(size_t)item0 * 2 + 1] = (uint32_t)(wall_clock64() - t_start);
+				a.log[(size_t)item0 * 3] = s_hits;
+				a.log[(size_t)item0 * 3 + 1] = 0;
+				a.log[(size_t)item0 * 3 + 2] = (uint32_t)(wall_clock64() - t_start);
 			}
 		}
 		__syncthreads();
@@ -1004,6 +1054,7 @@ struct tm_ring {
 	float * d_tpred = nullptr;     // predictions, 2 per row
 	float * d_tcost = nullptr;     // per tile: predicted ticks
 	float * d_thits = nullptr;     // per tile: predicted hits
+	float * d_tkeys = nullptr;     // per tile: predicted key instances
 	uint32_t * d_iorder = nullptr; // per work item: query tile
 	uint64_t trows_cap = 0, tiles_cap = 0;
 	bool use_iorder = false;
@@ -1044,7 +1095,7 @@ void tm_ring::release() {
 	dfree(d_gbm); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
 	dfree(d_vals); dfree(d_cnt); dfree(d_scan); dfree(d_stat); dfree(d_log);
 	dfree(d_units); dfree(d_ufeat);
-	dfree(d_tfeat); dfree(d_tpred); dfree(d_tcost); dfree(d_thits); dfree(d_iorder);
+	dfree(d_tfeat); dfree(d_tpred); dfree(d_tcost); dfree(d_thits); dfree(d_tkeys); dfree(d_iorder);
 	units_cap = trows_cap = tiles_cap = 0;
 	if (e0) (void)hipEventDestroy(e0);
 	if (em0) (void)hipEventDestroy(em0);
@@ -1068,7 +1119,9 @@ int resolve_params(tm_ring * r) {
 		return set_error(TM_ERR_INVALID,
 						 "KEYS_UNIQUE supports up to 65536 distinct keys (FIB has %u); use KEYS_ALL",
 						 r->fib->n_keys);
-	uint32_t bs = c.batch_size ? c.batch_size : 256;
+	// default batch: the reference's 256 for the plain form (parameters.hh:22),
+	// 128-query tiles for the persistent kernel (measured best at 10M filters)
+	uint32_t bs = c.batch_size ? c.batch_size : (c.schedule == TM_SCHED_PLAIN ? 256 : 128);
 	if (bs != 64 && bs != 128 && bs != 256)
 		return set_error(TM_ERR_INVALID, "batch_size must be 64, 128 or 256 (got %u)", bs);
 	r->tiles = (int)(bs / 64);
@@ -1084,7 +1137,7 @@ int resolve_params(tm_ring * r) {
 		r->stage = (mode_of(c) == 0) && lds <= STAGE_LDS_MAX && lds + static_lds <= max_lds;
 		r->stage_lds = r->stage ? lds : 0;
 	} else {
-		r->variant = c.variant <= 7 ? (int)c.variant : 6;
+		r->variant = (c.variant >= 1 && c.variant <= 7) ? (int)c.variant : 2;
 		int nw = c.waves_per_group ? (int)c.waves_per_group : 8;
 		if (nw != 4 && nw != 8 && nw != 16)
 			return set_error(TM_ERR_INVALID, "waves_per_group must be 4, 8 or 16 (got %d)", nw);
@@ -1214,6 +1267,8 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.nq = nq;
 	a.porder = r->have_order ? r->d_porder : nullptr;
 	a.iorder = r->use_iorder ? r->d_iorder : nullptr;
+	a.full_exit = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
+	a.last_mask = (r->fib->n_keys & 31) ? (1u << (r->fib->n_keys & 31)) - 1u : 0xFFFFFFFFu;
 	a.ntiles = (uint32_t)tiles;
 	a.nseg = nseg;
 	a.seg_blocks = seg_blocks;
@@ -1278,7 +1333,7 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 		}
 		if (nu) {
 			TM_HIP(hipMemcpyAsync(r->d_units, units.data(), sizeof(uint4) * nu, hipMemcpyHostToDevice, r->s));
-			fill_u32(r->d_log, 0, (uint64_t)nu * 2, r->s);
+			fill_u32(r->d_log, 0, (uint64_t)nu * 3, r->s);
 			launch_unit_features_plain(r->fib->view(), r->d_q, r->d_list, r->d_pbase, r->d_units, nu, bq,
 									   r->d_ufeat, r->s);
 			TM_HIP(hipStreamSynchronize(r->s));   // the unit vector is pageable host memory
@@ -1344,7 +1399,7 @@ int ensure_capacity(tm_ring * r) {
 	}
 	if (r->cfg.log_units && !r->d_log) {
 		r->log_cap = 1 << 22;
-		TM_CHECK(dalloc(&r->d_log, r->log_cap * 2));
+		TM_CHECK(dalloc(&r->d_log, r->log_cap * 3));
 	}
 	return TM_OK;
 }
@@ -1423,24 +1478,30 @@ int price_submission(tm_ring * r, uint32_t nq) {
 		const bool graph_uses = r->d_iorder != nullptr;
 		dfree(r->d_tcost);
 		dfree(r->d_thits);
+		dfree(r->d_tkeys);
 		dfree(r->d_iorder);
 		r->tiles_cap = 0;
 		TM_CHECK(dalloc(&r->d_tcost, n_tiles));
 		TM_CHECK(dalloc(&r->d_thits, n_tiles));
+		TM_CHECK(dalloc(&r->d_tkeys, n_tiles));
 		TM_CHECK(dalloc(&r->d_iorder, n_tiles));
 		r->tiles_cap = n_tiles;
 		if (graph_uses) drop_graph(r);
 	}
 	launch_unit_features_tiles(r->fib->view(), r->d_q, nq, nqt, n_tiles, r->d_tfeat, s);
 	TM_CHECK(mlp_infer(r->model, r->d_tfeat, (uint32_t)rows, r->d_tpred, s));
-	launch_tile_costs(r->d_tfeat, r->d_tpred, n_tiles, r->P, r->d_tcost, r->d_thits, s);
+	launch_tile_costs(r->d_tfeat, r->d_tpred, n_tiles, r->P, r->d_tcost, r->d_thits, r->d_tkeys, s);
 	TM_HIP(hipGetLastError());
-	std::vector<float> cost(n_tiles), hits(n_tiles);
+	std::vector<float> cost(n_tiles), hits(n_tiles), keys(n_tiles);
 	TM_HIP(hipMemcpyAsync(cost.data(), r->d_tcost, 4ull * n_tiles, hipMemcpyDeviceToHost, s));
 	TM_HIP(hipMemcpyAsync(hits.data(), r->d_thits, 4ull * n_tiles, hipMemcpyDeviceToHost, s));
+	TM_HIP(hipMemcpyAsync(keys.data(), r->d_tkeys, 4ull * n_tiles, hipMemcpyDeviceToHost, s));
 	TM_HIP(hipStreamSynchronize(s));
-	double total = 0.0;
-	for (uint32_t t = 0; t < n_tiles; ++t) total += hits[t];
+	double total = 0.0, total_keys = 0.0;
+	for (uint32_t t = 0; t < n_tiles; ++t) {
+		total += hits[t];
+		total_keys += keys[t];
+	}
 	r->st.pred_hits = total;
 
 	// ordering pays when tiles outnumber the resident workgroups; smaller
@@ -1471,24 +1532,23 @@ int price_submission(tm_ring * r, uint32_t nq) {
 
 	if (r->model_flags & TM_COST_PRESIZE) {
 		const tm_fib * f = r->fib;
-		const double kpf = f->n_filters ? (double)f->n_key_instances / (double)f->n_filters : 1.0;
 		const double margin = 1.25;   // predictions are fitted in log space: allow for spread
 		uint64_t slots = 0, results = 0;
 		if (r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
-			// expected distinct keys of h hits with kpf uniformly drawn keys
-			// each, from the tile's mean hits per query (concave in h, so
-			// the tile mean over-estimates: a safe direction)
+			// expected distinct keys among k key instances drawn from K keys,
+			// from the tile's mean instances per query (concave in k, so the
+			// tile mean over-estimates: a safe direction)
 			const double K = std::max<double>(f->n_keys, 1.0);
 			double est = 0.0;
 			for (uint32_t t = 0; t < n_tiles; ++t) {
 				const uint32_t qn = std::min<uint32_t>(nqt, nq - t * nqt);
-				const double h = hits[t] / std::max<uint32_t>(qn, 1);
-				est += qn * K * (1.0 - std::exp(-h * kpf / K));
+				const double k = keys[t] / std::max<uint32_t>(qn, 1);
+				est += qn * K * (1.0 - std::exp(-k / K));
 			}
 			results = (uint64_t)(est * margin) + 4096;
 		} else {
 			slots = (uint64_t)(total * margin) + 4096;
-			results = r->cfg.mode == TM_MODE_KEYS_ALL ? (uint64_t)(total * kpf * margin) + 4096 : slots;
+			results = r->cfg.mode == TM_MODE_KEYS_ALL ? (uint64_t)(total_keys * margin) + 4096 : slots;
 			if (slots > r->hit_cap) {
 				dfree(r->d_hits);
 				r->hit_cap = 0;
@@ -1558,7 +1618,6 @@ void tm_match_config_default(tm_match_config * c) {
 	std::memset(c, 0, sizeof(*c));
 	c->mode = TM_MODE_KEYS_UNIQUE;
 	c->schedule = TM_SCHED_PERSISTENT;
-	c->batch_size = 256;
 }
 
 int tm_ring_create(tm_fib * fib, uint32_t max_queries, const tm_match_config * cfg, tm_ring ** out) {
@@ -1740,8 +1799,37 @@ int tm_ring_results_device(tm_ring * r, const uint64_t ** off, const uint32_t **
 	return TM_OK;
 }
 
-int tm_ring_unit_log(tm_ring * r, float * features, uint32_t * hits, uint32_t * ticks, uint32_t * units,
-					 uint32_t capacity, uint32_t * n_units) {
+int tm_ring_autotune(tm_ring * r, uint32_t nq, const tm_match_config * cands, uint32_t n_cands, uint32_t reps,
+					 uint32_t * best, double * times_ms) {
+	if (!r || !cands || n_cands == 0 || !best) return set_error(TM_ERR_INVALID, "NULL argument");
+	if (nq == 0 || nq > r->max_q) return set_error(TM_ERR_INVALID, "nq %u not in 1..%u", nq, r->max_q);
+	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
+	if (reps == 0) reps = 1;
+	TM_CHECK(require_device());
+	double best_ms = 1e300;
+	uint32_t bi = 0;
+	for (uint32_t c = 0; c < n_cands; ++c) {
+		if (cands[c].mode != r->cfg.mode) return set_error(TM_ERR_INVALID, "candidate %u changes the mode", c);
+		TM_CHECK(tm_ring_config(r, &cands[c]));
+		double t = 1e300;
+		for (uint32_t k = 0; k <= reps; ++k) {   // the first run warms up (capacities, graph)
+			TM_CHECK(tm_submit_resident(r, nq));
+			tm_status st;
+			TM_CHECK(tm_wait(r, &st));
+			if (k > 0) t = std::min(t, st.device_ms);
+		}
+		if (times_ms) times_ms[c] = t;
+		if (t < best_ms) {
+			best_ms = t;
+			bi = c;
+		}
+	}
+	*best = bi;
+	return tm_ring_config(r, &cands[bi]);
+}
+
+int tm_ring_unit_log(tm_ring * r, float * features, uint32_t * hits, uint32_t * keys, uint32_t * ticks,
+					 uint32_t * units, uint32_t capacity, uint32_t * n_units) {
 	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
 	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
 	if (n_units) *n_units = r->n_units;
@@ -1759,12 +1847,13 @@ int tm_ring_unit_log(tm_ring * r, float * features, uint32_t * hits, uint32_t * 
 			units[(size_t)i * 3 + 2] = u[i].z;
 		}
 	}
-	if (hits || ticks) {
-		std::vector<uint32_t> log((size_t)n * 2);
-		TM_HIP(hipMemcpy(log.data(), r->d_log, (size_t)n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+	if (hits || keys || ticks) {
+		std::vector<uint32_t> log((size_t)n * 3);
+		TM_HIP(hipMemcpy(log.data(), r->d_log, (size_t)n * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost));
 		for (uint32_t i = 0; i < n; ++i) {
-			if (hits) hits[i] = log[(size_t)i * 2];
-			if (ticks) ticks[i] = log[(size_t)i * 2 + 1];
+			if (hits) hits[i] = log[(size_t)i * 3];
+			if (keys) keys[i] = log[(size_t)i * 3 + 1];
+			if (ticks) ticks[i] = log[(size_t)i * 3 + 2];
 		}
 	}
 	return TM_OK;

@@ -17,7 +17,7 @@ def log(msg):
     print(msg, flush=True)
 
 
-@pytest.mark.parametrize("shape", [(16, 64, 2), (5, 7, 3)])
+@pytest.mark.parametrize("shape", [(16, 64, 3), (5, 7, 2)])
 def test_forward_matches_oracle(gpu, shape):
     log(f"mlp forward {shape}")
     m = tm.MLP(*shape, seed=11)
@@ -28,7 +28,7 @@ def test_forward_matches_oracle(gpu, shape):
     np.testing.assert_allclose(y, ye, rtol=1e-4, atol=1e-4 * np.abs(ye).max())
 
 
-@pytest.mark.parametrize("shape,n", [((16, 64, 2), 300), ((5, 7, 3), 41)])
+@pytest.mark.parametrize("shape,n", [((16, 64, 3), 300), ((5, 7, 2), 41)])
 def test_loss_and_gradient_match_oracle(gpu, shape, n):
     log(f"mlp loss/grad {shape} n={n}")
     m = tm.MLP(*shape, seed=12)
@@ -42,11 +42,11 @@ def test_loss_and_gradient_match_oracle(gpu, shape, n):
 
 
 def test_adam_step_matches_oracle(gpu):
-    shape = (16, 64, 2)
+    shape = (16, 64, 3)
     m = tm.MLP(*shape, seed=13)
     rng = np.random.default_rng(3)
     x = rng.normal(0, 1, (256, 16)).astype(np.float32)
-    t = rng.normal(0, 1, (256, 2)).astype(np.float32)
+    t = rng.normal(0, 1, (256, 3)).astype(np.float32)
     p0 = m.params().astype(np.float64)
     _, g = mo.loss_grad(p0, x, t, shape)
     lr = 1e-3
@@ -61,7 +61,7 @@ def test_adam_step_matches_oracle(gpu):
 def test_training_reduces_loss(gpu):
     rng = np.random.default_rng(4)
     x = rng.uniform(0, 1, (4096, 16)).astype(np.float32)
-    t = np.stack([x[:, :4].sum(1) / 4, np.sin(3 * x[:, 5]) * 0.5 + 0.5], 1).astype(np.float32)
+    t = np.stack([x[:, :4].sum(1) / 4, np.sin(3 * x[:, 5]) * 0.5 + 0.5, x[:, 7] * x[:, 8]], 1).astype(np.float32)
     m = tm.MLP(seed=14)
     l0, _ = m.loss_grad(x, t)
     l1 = m.train(x, t, epochs=30, batch=256, lr=3e-3)
@@ -71,21 +71,23 @@ def test_training_reduces_loss(gpu):
 
 # ----------------------------------------------------------------- unit log
 
-def small_table(n_filters, maxp, seed):
+def small_table(n_filters, maxp, seed, with_perm=False):
     cw, ck, cv, _, _ = tm.workload(n_filters, 1, seed=seed)
     fib, perm, off, masks = tm.build_fib(cw, ck, cv, maxp)
     pw = cw[perm]
+    if with_perm:
+        return cw, ck, cv, fib, pw, off, masks, perm
     return cw, ck, cv, fib, pw, off, masks
 
 
 def test_plain_unit_log_matches_brute_force(gpu):
-    cw, ck, cv, fib, pw, off, masks = small_table(20000, 1500, 301)
+    cw, ck, cv, fib, pw, off, masks, perm = small_table(20000, 1500, 301, with_perm=True)
     _, _, _, q, _ = tm.workload(20000, 200, seed=301)
     log(f"unit log: {len(off) - 1} partitions, {len(q)} queries")
     ring = tm.Ring(fib, len(q), schedule=tm.SCHED_PLAIN, batch_size=256, log_units=1)
     ring.submit(q)
     ring.wait()
-    f, h, t, u = ring.unit_log()
+    f, h, kk, t, u = ring.unit_log()
     orw, cnt, pop = mo.partition_stats(pw, off)
     # 200 queries, batch 256: every partition is one batch of all its queries
     assert len(u) == len(set(u[:, 0].tolist()))
@@ -96,8 +98,12 @@ def test_plain_unit_log_matches_brute_force(gpu):
         sel = np.all((masks[p][None, :] & ~q) == 0, axis=1)
         assert nb == int(sel.sum())
         fw = pw[a:e]
-        hits = int(sum(np.all((fw & ~qq) == 0, axis=1).sum() for qq in q[sel]))
+        hitmask = np.array([np.all((fw & ~qq) == 0, axis=1) for qq in q[sel]]).reshape(-1, e - a)
+        hits = int(hitmask.sum())
         assert int(h[k]) == hits, (p, int(h[k]), hits)
+        fid = perm[a:e]                              # canonical ids of the partition's filters
+        nkeys = (ck[fid + 1] - ck[fid]).astype(np.int64)
+        assert int(kk[k]) == int((hitmask * nkeys[None, :]).sum())
         assert int(t[k]) > 0
         fe = mo.unit_features(masks[p], orw[p], cnt[p], pop[p], q[sel])
         np.testing.assert_allclose(f[k], fe, rtol=1e-5, atol=1e-6)
@@ -105,18 +111,19 @@ def test_plain_unit_log_matches_brute_force(gpu):
 
 def collect_logs(fib, queries, batch_sizes=(64, 128, 256)):
     """Plain-schedule unit logs at several batch sizes."""
-    feats, hits, ticks = [], [], []
+    feats, hits, keys, ticks = [], [], [], []
     ring = tm.Ring(fib, len(queries), schedule=tm.SCHED_PLAIN, log_units=1)
     for bs in batch_sizes:
         ring.configure(schedule=tm.SCHED_PLAIN, batch_size=bs, log_units=1)
         ring.submit(queries)
         ring.wait()
-        f, h, t, _ = ring.unit_log()
+        f, h, k, t, _ = ring.unit_log()
         feats.append(f)
         hits.append(h)
+        keys.append(k)
         ticks.append(t)
     ring.close()
-    return np.concatenate(feats), np.concatenate(hits), np.concatenate(ticks)
+    return tuple(np.concatenate(x) for x in (feats, hits, keys, ticks))
 
 
 @pytest.fixture(scope="module")
@@ -125,20 +132,21 @@ def trained(gpu):
     _, _, _, qa, _ = tm.workload(200000, 4096, seed=302, query_first=0)
     _, _, _, qb, _ = tm.workload(200000, 4096, seed=302, query_first=10**6)
     log(f"cost model fixture: {len(off) - 1} partitions")
-    fa, ha, ta = collect_logs(fib, qa)
-    fb, hb, tb = collect_logs(fib, qb)
+    fa, ha, ka, ta = collect_logs(fib, qa)
+    fb, hb, kb, tb = collect_logs(fib, qb)
     m = tm.MLP(seed=21)
-    loss = m.train(fa, tm.encode_targets(ha, ta), epochs=60, batch=512, lr=2e-3)
+    loss = m.train(fa, tm.encode_targets(ha, ka, ta), epochs=60, batch=512, lr=2e-3)
+    m.calibrate(fa, ha, ka, ta)
     log(f"trained on {len(fa)} units, loss {loss:.4g}; held-out {len(fb)} units")
     return dict(cw=cw, ck=ck, cv=cv, fib=fib, pw=pw, off=off, masks=masks, qa=qa, qb=qb, m=m,
-                fa=fa, ha=ha, ta=ta, fb=fb, hb=hb, tb=tb)
+                fa=fa, ha=ha, ka=ka, ta=ta, fb=fb, hb=hb, kb=kb, tb=tb)
 
 
 def test_model_beats_rate_baseline(trained):
     """Held-out per-unit MAPE of the network against the method without it:
     a rate model hits = r * B * n_p, r fitted on the training units."""
     d = trained
-    ph, pt = tm.decode_targets(d["m"].forward(d["fb"]))
+    ph, pk, pt = tm.decode_targets(d["m"].forward(d["fb"]))
 
     def sizes(f):
         return (np.exp2(f[:, 2] * 12) - 1) * (np.exp2(f[:, 0] * 20) - 1)
@@ -175,8 +183,8 @@ def test_predicted_hits_match_oracle_pricing(trained):
     ring.submit(q)
     st = ring.wait()
     x = tile_rows(d["pw"], d["off"], d["masks"], q, 128)
-    y, _ = mo.forward(d["m"].params(), x, (16, 64, 2))
-    h, _ = tm.decode_targets(y)
+    y, _ = mo.forward(d["m"].params(), x, (16, 64, 3))
+    h, _, _ = tm.decode_targets(y)
     expect = h[x[:, 2] > 0].sum()
     log(f"pred_hits {st.pred_hits:.1f} oracle {expect:.1f} actual {st.n_hits}")
     assert st.pred_hits == pytest.approx(expect, rel=1e-3)

@@ -160,6 +160,42 @@ def test_batch_sizes_and_chunks(gpu, batch):
             assert_same(o, v, eo, ev, sort_got=False, what=f"b{batch} s{sched} c{chunk}")
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
+def test_inner_loop_variants_vs_oracle(gpu, variant):
+    """Every persistent inner-loop variant (scalar or LDS query words, key
+    log, early exit), large submissions (one tile per item) and small ones
+    (block segments), all output modes."""
+    cw, cko, ck, q, s = tm.workload(30000, 3000, seed=910 + variant)
+    fib, *_ = tm.build_fib(cw, cko, ck, 2500)
+    for nq in (3000, 37):
+        log(f"variant={variant} nq={nq}")
+        run_all_modes(fib, cw, cko, ck, q[:nq], schedules=(tm.SCHED_PERSISTENT,), variant=variant,
+                      batch_size=64 if variant % 2 else 128)
+
+
+@pytest.mark.parametrize("n_keys", [5, 37, 64, 100])
+def test_full_key_set_exit(gpu, n_keys):
+    """With few distinct keys most queries collect every key early; the exit
+    that stops testing such queries must not change any result (key counts
+    around word boundaries of the bitmap row: 5, 37, 64, 100)."""
+    cw, cko, ck, q, s = tm.workload(30000, 2000, seed=950 + n_keys)
+    ck = (ck % n_keys).astype(np.uint32) * 7 + 3          # n_keys distinct keys
+    # keys per filter must stay distinct for the canonical table
+    key_lists = [np.unique(ck[cko[i]:cko[i + 1]]) for i in range(len(cw))]
+    cko = np.zeros(len(cw) + 1, np.uint64)
+    np.cumsum([len(k) for k in key_lists], out=cko[1:])
+    ck = np.concatenate(key_lists).astype(np.uint32)
+    fib, *_ = tm.build_fib(cw, cko, ck, 3000)
+    eo, ev = orc.match(cw, cko, ck, q, mode=0)
+    res = {}
+    for fe in (1, 2):
+        o, v, st = tm.match(fib, q, full_exit=fe)
+        assert_same(o, v, eo, ev, sort_got=False, what=f"full_exit {fe}")
+        res[fe] = st["n_hits"]
+    log(f"n_keys {n_keys}: hits processed {res[1]} with the exit, {res[2]} without")
+    assert res[1] < res[2]
+
+
 def bitf(*bits):
     x = np.zeros(6, np.uint32)
     for b in bits:
