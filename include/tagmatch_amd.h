@@ -183,7 +183,10 @@ typedef struct tm_match_config {
 	                            0 = 256 (reference parameters.hh:22);
 	                            PERSISTENT: queries per work item (64, 128
 	                            or 256), 0 = 128                             */
-	uint32_t chunk_blocks;   /* blocks per work item; 0 = autotuned          */
+	uint32_t chunk_blocks;   /* PLAIN: least blocks per segment of a unit
+	                            (0 = 4); PERSISTENT: blocks per segment of
+	                            the table (0 = automatic: segments only for
+	                            submissions too small to fill the GPU)       */
 	uint32_t waves_per_group;/* waves per workgroup (1..16); 0 = autotuned   */
 	uint32_t groups_per_cu;  /* PERSISTENT: resident workgroups per CU; 0 = auto */
 	uint64_t hit_capacity;   /* HITS / KEYS_ALL: hit slots; 0 = default      */
@@ -192,12 +195,14 @@ typedef struct tm_match_config {
 	uint32_t log_units;      /* record per-work-item hit counts and cycles   */
 	uint32_t order_by_cost;  /* PERSISTENT: visit partitions in the order set
 	                            by tm_ring_set_order                          */
-	uint32_t variant;        /* PERSISTENT inner loop, 1..7 as bits; bit 0:
-	                            candidate query words from scalar loads
-	                            (else LDS broadcasts); bit 1: KEYS_UNIQUE
-	                            hits go through a per-wave key log drained
-	                            64 lanes at a time; bit 2: early exit after
-	                            words 0-1; 0 or other values: default (2)    */
+	uint32_t variant;        /* PERSISTENT inner loop, 1..7 or 10, as bits;
+	                            bit 0: candidate query words from scalar
+	                            loads (else LDS broadcasts); bit 1:
+	                            KEYS_UNIQUE hits go through a per-wave key
+	                            log drained 64 lanes at a time; bit 2: early
+	                            exit after words 0-1; bit 3: the next
+	                            block's data is loaded while one is tested;
+	                            0 or other values: default (2)               */
 	uint32_t full_exit;      /* PERSISTENT KEYS_UNIQUE: stop testing a query
 	                            once its key set holds every key of the
 	                            table (results unchanged); 0 = default (on),

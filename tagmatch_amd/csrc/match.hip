@@ -39,10 +39,8 @@ using namespace tmx;
 namespace {
 
 constexpr int WG = 256;          // threads per match workgroup (4 waves)
-constexpr int WAVES = WG / 64;
 constexpr int NQUEUE = 8;        // work queues, one per XCD of an MI355X
 constexpr uint32_t NO_QUERY = 0xFFFFFFFFu;
-constexpr size_t STAGE_LDS_MAX = 64 * 1024;
 
 __device__ inline uint32_t mbcnt64(uint64_t m) {
 	return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -153,254 +151,12 @@ k_build_items(const uint32_t * __restrict__ pcount, const uint64_t * __restrict_
 
 // ------------------------------------------------------------- match
 
-struct match_args {
-	dev_fib_view f;
-	const uint32_t * __restrict__ q;          // nq x 8 words
-	const uint32_t * __restrict__ list;       // query indices grouped per partition
-	const uint64_t * __restrict__ pbase;      // list offset per partition
-	const uint4 * __restrict__ sched;         // per rank: partition, batches, chunks, queries
-	const uint64_t * __restrict__ item_prefix;// per rank, n_ranks + 1
-	uint32_t n_ranks;
-	uint32_t chunk;                           // blocks per work item
-	unsigned long long * qnext;               // NQUEUE queue cursors
-	const uint64_t * __restrict__ qbeg;       // NQUEUE + 1 queue bounds
-	int32_t plain_rank;                       // >= 0: one (rank, batch) unit, chunk = blockIdx.x
-	uint32_t plain_batch;
-	uint32_t plain_unit;                      // log row of the (rank, batch) unit
-	uint32_t * gbm;                           // KEYS_UNIQUE: nq x kw bitmap words
-	uint2 * hits;                             // HITS: (query, block slot)
-	uint64_t hit_cap;
-	unsigned long long * counters;            // [0] hit slots taken, [1] hits
-	uint32_t * log;                           // per unit: hits, key instances, 10 ns ticks
-	uint64_t log_cap;
-};
-
 // Number of keys in a slot (inline ids or an overflow list).
 __device__ inline uint32_t key_count(uint64_t kv, const uint32_t * __restrict__ ovf) {
 	if ((kv & 0xFFFFu) == KEY_NONE) return ovf[(uint32_t)(kv >> 16)];
 	uint32_t n = 0;
 	for (int x = 0; x < 4; ++x) n += ((kv >> (16 * x)) & 0xFFFFu) != KEY_NONE;
 	return n;
-}
-
-// OR the dense key ids of a filter into a bitmap (LDS or global).
-__device__ inline void add_keys(uint32_t * bm, uint64_t kv, const uint32_t * __restrict__ ovf) {
-	uint32_t k0 = (uint32_t)(kv & 0xFFFFu);
-	if (k0 != KEY_NONE) {
-		atomicOr(&bm[k0 >> 5], 1u << (k0 & 31));
-		uint32_t k1 = (uint32_t)((kv >> 16) & 0xFFFFu);
-		if (k1 != KEY_NONE) {
-			atomicOr(&bm[k1 >> 5], 1u << (k1 & 31));
-			uint32_t k2 = (uint32_t)((kv >> 32) & 0xFFFFu);
-			if (k2 != KEY_NONE) {
-				atomicOr(&bm[k2 >> 5], 1u << (k2 & 31));
-				uint32_t k3 = (uint32_t)(kv >> 48);
-				if (k3 != KEY_NONE) atomicOr(&bm[k3 >> 5], 1u << (k3 & 31));
-			}
-		}
-	} else {
-		uint32_t off = (uint32_t)(kv >> 16);
-		uint32_t n = ovf[off];
-		for (uint32_t i = 0; i < n; ++i) {
-			uint32_t id = ovf[off + 1 + i];
-			atomicOr(&bm[id >> 5], 1u << (id & 31));
-		}
-	}
-}
-
-// MODE 0: key bitmaps (KEYS_UNIQUE); MODE 1: (query, slot) hit pairs.
-// STAGE: accumulate key bitmaps of the item's queries in LDS, flush once.
-// TILES: 64-query tiles per work item.
-template <int MODE, bool STAGE, int TILES>
-__global__ void __launch_bounds__(WG)
-k_match(match_args a) {
-	constexpr int NQ = TILES * 64;
-	__shared__ uint32_t s_q[NQ * 8];      // per query: 6 inverted words, query index, pad
-	__shared__ uint32_t s_item[5];        // partition, batch, chunk, queries in batch, done
-	__shared__ unsigned long long s_id;
-	__shared__ uint32_t s_hits, s_keys;
-	extern __shared__ uint32_t s_bm[];    // STAGE: NQ x kw
-	const uint32_t tid = threadIdx.x, lane = tid & 63;
-	const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-	const uint32_t kw = a.f.kw;
-	const uint32_t home = blockIdx.x % NQUEUE;
-
-	for (int iter = 0;; ++iter) {
-		uint64_t t_start = 0;
-		if (tid == 0) {
-			s_item[4] = 1;
-			s_hits = 0;
-			s_keys = 0;
-			if (a.plain_rank >= 0) {
-				if (iter == 0) {
-					uint4 sc = a.sched[a.plain_rank];
-					s_item[0] = sc.x;
-					s_item[1] = a.plain_batch;
-					s_item[2] = blockIdx.x;
-					uint32_t rem = sc.w - a.plain_batch * NQ;
-					s_item[3] = rem < (uint32_t)NQ ? rem : (uint32_t)NQ;
-					s_item[4] = 0;
-					s_id = a.plain_unit;
-				}
-			} else {
-				unsigned long long item = ~0ULL;
-				for (int k = 0; k < NQUEUE; ++k) {
-					uint32_t x = (home + k) % NQUEUE;
-					if (a.qnext[x] >= a.qbeg[x + 1]) continue;   // drained: skip the atomic
-					unsigned long long i = atomicAdd(&a.qnext[x], 1ULL);
-					if (i < a.qbeg[x + 1]) { item = i; break; }
-				}
-				if (item != ~0ULL) {
-					uint32_t lo = 0, hi = a.n_ranks;   // last rank with prefix <= item
-					while (hi - lo > 1) {
-						uint32_t mid = (lo + hi) / 2;
-						if (a.item_prefix[mid] <= item) lo = mid; else hi = mid;
-					}
-					uint4 sc = a.sched[lo];
-					uint64_t local = item - a.item_prefix[lo];
-					uint32_t batch = (uint32_t)(local % sc.y);
-					s_item[0] = sc.x;
-					s_item[1] = batch;
-					s_item[2] = (uint32_t)(local / sc.y);
-					uint32_t rem = sc.w - batch * NQ;
-					s_item[3] = rem < (uint32_t)NQ ? rem : (uint32_t)NQ;
-					s_item[4] = 0;
-					s_id = item;
-				}
-			}
-			t_start = wall_clock64();
-		}
-		__syncthreads();
-		if (s_item[4]) break;
-		const uint32_t p = s_item[0], batch = s_item[1], chunk = s_item[2], nqv = s_item[3];
-
-		// stage the batch's queries (inverted) in LDS
-		const uint64_t lbase = a.pbase[p] + (uint64_t)batch * NQ;
-		for (uint32_t t = tid; t < (uint32_t)NQ; t += WG) {
-			uint32_t * d = &s_q[t * 8];
-			if (t < nqv) {
-				uint32_t qi = a.list[lbase + t];
-				uint4 x = *reinterpret_cast<const uint4 *>(a.q + (size_t)qi * 8);
-				uint2 y = *reinterpret_cast<const uint2 *>(a.q + (size_t)qi * 8 + 4);
-				d[0] = ~x.x; d[1] = ~x.y; d[2] = ~x.z; d[3] = ~x.w; d[4] = ~y.x; d[5] = ~y.y;
-				d[6] = qi;
-			} else {
-				d[0] = d[1] = d[2] = d[3] = d[4] = d[5] = 0;
-				d[6] = NO_QUERY;
-			}
-		}
-		if (STAGE)
-			for (uint32_t w = tid; w < (uint32_t)NQ * kw; w += WG) s_bm[w] = 0;
-		__syncthreads();
-
-		// candidate-pass operands: query (t, lane) of each tile
-		uint32_t qv[TILES][6];
-		uint64_t qok[TILES];
-#pragma unroll
-		for (int t = 0; t < TILES; ++t) {
-			const uint32_t * d = &s_q[(t * 64 + lane) * 8];
-#pragma unroll
-			for (int j = 0; j < 6; ++j) qv[t][j] = d[j];
-			qok[t] = __ballot(d[6] != NO_QUERY);
-		}
-
-		const uint32_t * pi = a.f.part_info + (size_t)p * 8;
-		const uint32_t fb = pi[6], nb = pi[7];
-		const uint32_t b0 = fb + chunk * a.chunk;
-		const uint32_t b1 = min(fb + nb, b0 + a.chunk);
-		uint32_t hit_acc = 0, key_acc = 0;   // key_acc: key instances of this lane's hits (log only)
-
-		for (uint32_t b = b0 + wave; b < b1; b += WAVES) {
-			const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
-			const uint32_t m0 = bi[0], m1 = bi[1], m2 = bi[2], m3 = bi[3], m4 = bi[4], m5 = bi[5];
-			const uint32_t nv = bi[6];
-			uint64_t cand[TILES];
-			uint64_t any = 0;
-#pragma unroll
-			for (int t = 0; t < TILES; ++t) {
-				uint32_t x = (m0 & qv[t][0]) | (m1 & qv[t][1]) | (m2 & qv[t][2]) |
-							 (m3 & qv[t][3]) | (m4 & qv[t][4]) | (m5 & qv[t][5]);
-				cand[t] = __ballot(x == 0) & qok[t];
-				any |= cand[t];
-			}
-			if (any == 0) continue;
-
-			const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
-			const uint32_t f0 = bw[0], f1 = bw[64], f2 = bw[128], f3 = bw[192], f4 = bw[256], f5 = bw[320];
-			const uint64_t lmask = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
-			uint64_t kv = 0;
-			bool kv_loaded = false;
-
-#pragma unroll
-			for (int t = 0; t < TILES; ++t) {
-				uint64_t c = cand[t];
-				while (c) {
-					const uint32_t j = (uint32_t)__builtin_ctzll(c);
-					c &= c - 1;
-					const uint32_t * qq = &s_q[(t * 64 + j) * 8];
-					const uint4 qa = *reinterpret_cast<const uint4 *>(qq);
-					const uint2 qb = *reinterpret_cast<const uint2 *>(qq + 4);
-					const uint32_t acc = (f0 & qa.x) | (f1 & qa.y) | (f2 & qa.z) |
-										 (f3 & qa.w) | (f4 & qb.x) | (f5 & qb.y);
-					const uint64_t h = __ballot(acc == 0) & lmask;
-					if (h == 0) continue;
-					hit_acc += (uint32_t)__popcll(h);
-					const bool mine = (h >> lane) & 1;
-					if (a.log && mine) key_acc += key_count(a.f.slot_keys[(size_t)b * 64 + lane], a.f.ovf);
-					if (MODE == 0) {
-						if (!kv_loaded) {
-							kv = a.f.slot_keys[(size_t)b * 64 + lane];
-							kv_loaded = true;
-						}
-						if (mine) {
-							uint32_t * bm = STAGE ? &s_bm[(t * 64 + j) * kw] : &a.gbm[(size_t)qq[6] * kw];
-							add_keys(bm, kv, a.f.ovf);
-						}
-					} else {
-						unsigned long long base = 0;
-						if (lane == 0) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(h));
-						base = __shfl(base, 0);
-						if (mine) {
-							unsigned long long pos = base + mbcnt64(h);
-							if (pos < a.hit_cap) a.hits[pos] = make_uint2(qq[6], b * 64 + lane);
-						}
-					}
-				}
-			}
-		}
-
-		if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
-		if (a.log) {
-			for (int d = 32; d > 0; d >>= 1) key_acc += __shfl_xor(key_acc, d, 64);
-			if (lane == 0 && key_acc) atomicAdd(&s_keys, key_acc);
-		}
-		__syncthreads();
-		if (STAGE) {
-			for (uint32_t w = tid; w < (uint32_t)NQ * kw; w += WG) {
-				uint32_t v = s_bm[w];
-				if (v) {
-					uint32_t t = w / kw;
-					atomicOr(&a.gbm[(size_t)s_q[t * 8 + 6] * kw + (w - t * kw)], v);
-				}
-			}
-		}
-		if (tid == 0) {
-			if (s_hits) atomicAdd(&a.counters[1], (unsigned long long)s_hits);
-			if (a.log && s_id < a.log_cap) {
-				const uint32_t ticks = (uint32_t)(wall_clock64() - t_start);
-				if (a.plain_rank >= 0) {   // the unit's chunks add up
-					atomicAdd(&a.log[s_id * 3], s_hits);
-					atomicAdd(&a.log[s_id * 3 + 1], s_keys);
-					atomicAdd(&a.log[s_id * 3 + 2], ticks);
-				} else {
-					a.log[s_id * 3] = s_hits;
-					a.log[s_id * 3 + 1] = s_keys;
-					a.log[s_id * 3 + 2] = ticks;
-				}
-			}
-		}
-		__syncthreads();
-	}
 }
 
 // ------------------------------------------------------------- query-major match
@@ -430,6 +186,9 @@ struct mq_args {
 	uint32_t nq;
 	const uint32_t * __restrict__ porder;     // partition schedule order (NULL: 0..P-1)
 	const uint32_t * __restrict__ iorder;     // query tile of each work item (NULL: identity; nseg 1 only)
+	const uint32_t * __restrict__ qmap;       // query of each local index (NULL: identity)
+	uint32_t blk_lo, blk_hi;                  // blocks the launch covers
+	int32_t log_unit;                         // >= 0: add hits, keys, ticks into this log row
 	uint32_t ntiles;                          // query tiles
 	uint32_t nseg;                            // block segments per tile: item = seg * ntiles + tile
 	uint32_t seg_blocks;                      // blocks per segment (nseg > 1)
@@ -554,7 +313,7 @@ k_match_q(mq_args a) {
 	constexpr bool KLOG = MODE == 0 && (VAR & 2);
 	__shared__ uint32_t s_q[NQ * 8];
 	__shared__ uint32_t s_item[2];
-	__shared__ uint32_t s_hits;
+	__shared__ uint32_t s_hits, s_keys;
 	__shared__ uint2 s_hbuf[MODE == 1 ? NW * HBUF : 1];
 	__shared__ uint4 s_klog[KLOG ? NW * 64 : 1];
 	extern __shared__ uint32_t s_bm[];        // LDS_BM: NQ x kw
@@ -577,6 +336,7 @@ k_match_q(mq_args a) {
 			s_item[0] = (uint32_t)item;
 			s_item[1] = item == ~0ULL;
 			s_hits = 0;
+			s_keys = 0;
 			t_start = wall_clock64();
 		}
 		__syncthreads();
@@ -586,13 +346,17 @@ k_match_q(mq_args a) {
 		// one contiguous range of blocks (small submissions spread over the
 		// whole GPU); segment-major, so concurrent items share filter blocks.
 		uint32_t item, seg_b0 = 0, seg_b1 = 0xFFFFFFFFu, r_begin = 0, r_end = P;
-		if (a.nseg > 1) {
+		if (a.nseg > 1 || a.blk_lo != 0 || a.blk_hi < a.f.n_blocks) {
 			const uint32_t seg = item0 / a.ntiles;
 			item = item0 - seg * a.ntiles;
-			seg_b0 = seg * a.seg_blocks;
-			seg_b1 = min(seg_b0 + a.seg_blocks, a.f.n_blocks);
-			r_begin = sld1(a.f.blk_info + (size_t)seg_b0 * 8 + 7);
-			r_end = sld1(a.f.blk_info + (size_t)(seg_b1 - 1) * 8 + 7) + 1;
+			seg_b0 = a.blk_lo + seg * a.seg_blocks;
+			seg_b1 = min(seg_b0 + a.seg_blocks, a.blk_hi);
+			if (seg_b1 > seg_b0) {
+				r_begin = sld1(a.f.blk_info + (size_t)seg_b0 * 8 + 7);
+				r_end = sld1(a.f.blk_info + (size_t)(seg_b1 - 1) * 8 + 7) + 1;
+			} else {
+				r_begin = r_end = 0;
+			}
 		} else {
 			item = a.iorder ? sld1(a.iorder + item0) : item0;
 		}
@@ -602,8 +366,9 @@ k_match_q(mq_args a) {
 			uint32_t * d = &s_q[t * 8];
 			const uint32_t qi = q0 + t;
 			if (qi < a.nq) {
-				const uint4 x = *reinterpret_cast<const uint4 *>(a.q + (size_t)qi * 8);
-				const uint2 y = *reinterpret_cast<const uint2 *>(a.q + (size_t)qi * 8 + 4);
+				const uint32_t gq = a.qmap ? a.qmap[qi] : qi;
+				const uint4 x = *reinterpret_cast<const uint4 *>(a.q + (size_t)gq * 8);
+				const uint2 y = *reinterpret_cast<const uint2 *>(a.q + (size_t)gq * 8 + 4);
 				d[0] = ~x.x; d[1] = ~x.y; d[2] = ~x.z; d[3] = ~x.w; d[4] = ~y.x; d[5] = ~y.y;
 				d[6] = qi;
 			} else {
@@ -643,7 +408,16 @@ k_match_q(mq_args a) {
 			qok[t] = __ballot(d[6] != NO_QUERY);
 		}
 
-		uint32_t hit_acc = 0, n_visit = 0, n_test = 0, n_hev = 0;
+		uint32_t hit_acc = 0, n_visit = 0, n_test = 0, n_hev = 0, key_acc = 0;
+		// a hit event: lanes h of block b matched local query jl of the item
+		auto on_hit = [&](uint64_t h, uint32_t jl, uint32_t b, uint64_t kv) {
+			++n_hev;
+			hit_acc += (uint32_t)__popcll(h);
+			const uint32_t ql = q0 + jl;
+			const uint32_t gq = (MODE == 1 && a.qmap) ? sld1(a.qmap + ql) : ql;
+			if (a.log && ((h >> lane) & 1)) key_acc += key_count(kv, a.f.ovf);
+			sink.add(h, jl, gq, b, kv, lane);
+		};
 		for (uint32_t r = r_begin; r < r_end; ++r) {
 			const uint32_t p = a.porder && a.nseg <= 1 ? sld1(a.porder + r) : r;
 			const uint32_t * pi = a.f.part_info + (size_t)p * 8;
@@ -675,20 +449,31 @@ k_match_q(mq_args a) {
 #pragma unroll
 				for (int t = 0; t < TILES; ++t) {
 					if (qsel[t] == 0) continue;
-					// full: words 0..kw-2 all ones, word kw-1 holds the last K % 32 keys
+					// full: words 0..kw-2 all ones, word kw-1 holds the last K % 32 keys.
+					// With block segments the other segments of this tile merge
+					// their rows into gbm: the query's keys so far are the union.
 					const uint32_t * row = &s_bm[(t * 64 + lane) * kw];
+					const uint32_t qrow = q0 + t * 64 + lane;
+					const uint32_t * grow = (a.nseg > 1 && qrow < a.nq) ? a.gbm + (size_t)qrow * kw : nullptr;
 					uint32_t all = 0xFFFFFFFFu;
 					if ((kw & 3) == 0) {
 						const uint4 * r4 = reinterpret_cast<const uint4 *>(row);
-						for (uint32_t w4 = 0; w4 + 1 < (kw >> 2); ++w4) {
-							const uint4 v = r4[w4];
+						const uint4 * g4 = reinterpret_cast<const uint4 *>(grow);
+						for (uint32_t w4 = 0; w4 < (kw >> 2); ++w4) {
+							uint4 v = r4[w4];
+							if (grow) {
+								const uint4 g = g4[w4];
+								v.x |= g.x; v.y |= g.y; v.z |= g.z; v.w |= g.w;
+							}
+							if (w4 + 1 == (kw >> 2)) v.w |= ~a.last_mask;
 							all &= v.x & v.y & v.z & v.w;
 						}
-						const uint4 v = r4[(kw >> 2) - 1];
-						all &= v.x & v.y & v.z & (v.w | ~a.last_mask);
 					} else {
-						for (uint32_t w = 0; w + 1 < kw; ++w) all &= row[w];
-						all &= row[kw - 1] | ~a.last_mask;
+						for (uint32_t w = 0; w < kw; ++w) {
+							uint32_t v = row[w] | (grow ? grow[w] : 0u);
+							if (w + 1 == kw) v |= ~a.last_mask;
+							all &= v;
+						}
 					}
 					const bool full = all == 0xFFFFFFFFu;
 					qsel[t] &= ~__ballot(full);
@@ -696,10 +481,45 @@ k_match_q(mq_args a) {
 				}
 				if (anyq == 0) continue;
 			}
+			// VAR & 8: software pipeline, the next block's mask, words and
+			// key slots are loaded while this block is tested
+			constexpr bool PF = (VAR & 8) != 0;
+			uint4 p_ma = make_uint4(0, 0, 0, 0), p_mb = make_uint4(0, 0, 0, 0);
+			uint32_t p_f[6] = {0, 0, 0, 0, 0, 0};
+			uint64_t p_kv = 0;
+			if (PF && fb + wave < fb + nb) {
+				const uint32_t b = fb + wave;
+				p_ma = sld4(a.f.blk_info + (size_t)b * 8);
+				p_mb = sld4(a.f.blk_info + (size_t)b * 8 + 4);
+				const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
+#pragma unroll
+				for (int j = 0; j < 6; ++j) p_f[j] = bw[j * 64];
+				if (MODE == 0) p_kv = a.f.slot_keys[(size_t)b * 64 + lane];
+			}
 			for (uint32_t b = fb + wave; b < fb + nb; b += NW) {
-				const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
-				const uint4 ma = sld4(bi);
-				const uint4 mb = sld4(bi + 4);
+				uint4 ma, mb;
+				uint32_t c_f[6];
+				uint64_t c_kv = 0;
+				if (PF) {
+					ma = p_ma;
+					mb = p_mb;
+#pragma unroll
+					for (int j = 0; j < 6; ++j) c_f[j] = p_f[j];
+					c_kv = p_kv;
+					const uint32_t bn = b + NW;
+					if (bn < fb + nb) {
+						p_ma = sld4(a.f.blk_info + (size_t)bn * 8);
+						p_mb = sld4(a.f.blk_info + (size_t)bn * 8 + 4);
+						const uint32_t * bw = a.f.blk_words + (size_t)bn * BLOCK_WORDS + lane;
+#pragma unroll
+						for (int j = 0; j < 6; ++j) p_f[j] = bw[j * 64];
+						if (MODE == 0) p_kv = a.f.slot_keys[(size_t)bn * 64 + lane];
+					}
+				} else {
+					const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
+					ma = sld4(bi);
+					mb = sld4(bi + 4);
+				}
 				const uint32_t nv = mb.z;
 				uint64_t cand[TILES];
 				uint64_t any = 0;
@@ -714,9 +534,14 @@ k_match_q(mq_args a) {
 				if (any == 0) continue;
 				++n_visit;
 
-				const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
-				const uint32_t f0 = bw[0], f1 = bw[64], f2 = bw[128], f3 = bw[192], f4 = bw[256], f5 = bw[320];
-				const uint64_t kv = MODE == 0 ? a.f.slot_keys[(size_t)b * 64 + lane] : 0;
+				if (!PF) {
+					const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
+#pragma unroll
+					for (int j = 0; j < 6; ++j) c_f[j] = bw[j * 64];
+					c_kv = MODE == 0 ? a.f.slot_keys[(size_t)b * 64 + lane] : 0;
+				}
+				const uint32_t f0 = c_f[0], f1 = c_f[1], f2 = c_f[2], f3 = c_f[3], f4 = c_f[4], f5 = c_f[5];
+				const uint64_t kv = c_kv;
 				const uint64_t lmask = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
 
 #pragma unroll
@@ -751,8 +576,8 @@ k_match_q(mq_args a) {
 							acc1 |= (f2 & b1.x) | (f3 & b1.y) | (f4 & c1.x) | (f5 & c1.y);
 							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
 							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
-							if (h0) { ++n_hev; hit_acc += (uint32_t)__popcll(h0); sink.add(h0, t * 64 + j0, q0 + t * 64 + j0, b, kv, lane); }
-							if (h1) { ++n_hev; hit_acc += (uint32_t)__popcll(h1); sink.add(h1, t * 64 + j1, q0 + t * 64 + j1, b, kv, lane); }
+							if (h0) on_hit(h0, t * 64 + j0, b, kv);
+							if (h1) on_hit(h1, t * 64 + j1, b, kv);
 						}
 					} else {
 						// candidate query words through scalar loads, two per iteration
@@ -776,8 +601,8 @@ k_match_q(mq_args a) {
 							acc1 |= (f2 & b1.x) | (f3 & b1.y) | (f4 & b1.z) | (f5 & b1.w);
 							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
 							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
-							if (h0) { ++n_hev; hit_acc += (uint32_t)__popcll(h0); sink.add(h0, t * 64 + j0, q0 + t * 64 + j0, b, kv, lane); }
-							if (h1) { ++n_hev; hit_acc += (uint32_t)__popcll(h1); sink.add(h1, t * 64 + j1, q0 + t * 64 + j1, b, kv, lane); }
+							if (h0) on_hit(h0, t * 64 + j0, b, kv);
+							if (h1) on_hit(h1, t * 64 + j1, b, kv);
 						}
 					}
 				}
@@ -786,6 +611,10 @@ k_match_q(mq_args a) {
 		if (MODE == 1 && sink.hcnt) sink.flush(lane);
 		if (KLOG && sink.hcnt) sink.drain(lane);
 		if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
+		if (a.log) {
+			for (int d = 32; d > 0; d >>= 1) key_acc += __shfl_xor(key_acc, d, 64);
+			if (lane == 0 && key_acc) atomicAdd(&s_keys, key_acc);
+		}
 		if (lane == 0 && n_visit) {
 			atomicAdd(&a.counters[2], (unsigned long long)n_visit);
 			atomicAdd(&a.counters[3], (unsigned long long)n_test);
@@ -806,7 +635,11 @@ k_match_q(mq_args a) {
 		}
 		if (tid == 0) {
 			if (s_hits) atomicAdd(&a.counters[1], (unsigned long long)s_hits);
-			if (a.log && item0 < a.log_cap) {
+			if (a.log && a.log_unit >= 0) {   // plain: every item of the launch adds up
+				atomicAdd(&a.log[(size_t)a.log_unit * 3], s_hits);
+				atomicAdd(&a.log[(size_t)a.log_unit * 3 + 1], s_keys);
+				atomicAdd(&a.log[(size_t)a.log_unit * 3 + 2], (uint32_t)(wall_clock64() - t_start));
+			} else if (a.log && item0 < a.log_cap) {
 				a.log[(size_t)item0 * 3] = s_hits;
 				a.log[(size_t)item0 * 3 + 1] = 0;
 				a.log[(size_t)item0 * 3 + 2] = (uint32_t)(wall_clock64() - t_start);
@@ -943,23 +776,33 @@ __global__ void k_gather_stats(const uint64_t * __restrict__ pbase, uint32_t P,
 	}
 }
 
+// Work queues of every plain unit: unit u's NQUEUE queues split its items.
+__global__ void __launch_bounds__(256)
+k_init_unit_queues(const uint32_t * __restrict__ items, uint32_t n_units, uint64_t * __restrict__ qbeg,
+				   unsigned long long * __restrict__ qnext) {
+	const uint32_t u = blockIdx.x * 256 + threadIdx.x;
+	if (u >= n_units) return;
+	const uint64_t n = items[u];
+	for (int x = 0; x <= NQUEUE; ++x) qbeg[(size_t)u * (NQUEUE + 1) + x] = n * x / NQUEUE;
+	for (int x = 0; x < NQUEUE; ++x) qnext[(size_t)u * NQUEUE + x] = n * x / NQUEUE;
+}
+
+// OR one plain unit's staging rows into its queries' rows and clear them
+// for the next unit.  The queries of a batch are distinct and launches on
+// one stream run in order, so plain read-modify-writes suffice.
+__global__ void __launch_bounds__(256)
+k_or_rows(uint32_t * __restrict__ stage, uint32_t kw, const uint32_t * __restrict__ qmap, uint32_t nqv,
+		  uint32_t * __restrict__ gbm) {
+	const uint32_t w = blockIdx.x * 256 + threadIdx.x;
+	if (w >= nqv * kw) return;
+	const uint32_t v = stage[w];
+	if (v == 0) return;
+	const uint32_t t = w / kw;
+	gbm[(size_t)qmap[t] * kw + (w - t * kw)] |= v;
+	stage[w] = 0;
+}
+
 // ------------------------------------------------------------- kernel table
-
-using match_fn = void (*)(match_args);
-
-template <int MODE, bool STAGE>
-match_fn pick_tiles(int tiles) {
-	switch (tiles) {
-	case 1: return k_match<MODE, STAGE, 1>;
-	case 2: return k_match<MODE, STAGE, 2>;
-	default: return k_match<MODE, STAGE, 4>;
-	}
-}
-
-match_fn pick_match(int mode, bool stage, int tiles) {
-	if (mode == 0) return stage ? pick_tiles<0, true>(tiles) : pick_tiles<0, false>(tiles);
-	return pick_tiles<1, false>(tiles);
-}
 
 using mq_fn = void (*)(mq_args);
 
@@ -996,7 +839,8 @@ mq_fn pick_match_q(int mode, bool lds_bm, int tiles, int nw, int var) {
 	case 5: return pick_q_mode<5>(mode, lds_bm, tiles, nw);
 	case 6: return pick_q_mode<6>(mode, lds_bm, tiles, nw);
 	case 7: return pick_q_mode<7>(mode, lds_bm, tiles, nw);
-	default: return pick_q_mode<0>(mode, lds_bm, tiles, nw);
+	case 10: return pick_q_mode<10>(mode, lds_bm, tiles, nw);
+	default: return pick_q_mode<2>(mode, lds_bm, tiles, nw);
 	}
 }
 
@@ -1041,6 +885,13 @@ struct tm_ring {
 	uint64_t * h_stat = nullptr;   // pinned, 8
 	uint32_t * d_log = nullptr;
 	uint64_t log_cap = 0;
+	uint32_t * d_stage = nullptr;     // plain: key bitmap rows of one unit's batch
+	uint64_t stage_cap = 0;
+	uint32_t * d_uitems = nullptr;    // plain: work items per unit
+	uint64_t uitems_cap = 0;
+	uint64_t * d_ubeg = nullptr;      // plain: per unit NQUEUE + 1 queue bounds
+	unsigned long long * d_unext = nullptr;
+	uint32_t uq_cap = 0;
 	// plain unit log: per launch (partition, batch, queries) and features
 	uint4 * d_units = nullptr;
 	float * d_ufeat = nullptr;
@@ -1062,8 +913,8 @@ struct tm_ring {
 	// effective launch parameters
 	int tiles = 4, wgs_per_cu = 2, waves = 8, variant = 1;
 	uint32_t chunk = 256;
-	bool stage = false;        // PLAIN: LDS-staged key bitmaps
-	bool lds_bm = false;       // PERSISTENT: key bitmaps in LDS
+	uint32_t plain_bq = 256;   // PLAIN: queries per (partition, batch) unit
+	bool lds_bm = false;       // key bitmaps in LDS
 	size_t stage_lds = 0;      // dynamic LDS bytes of the match launch
 
 	// graph replay
@@ -1094,7 +945,9 @@ void tm_ring::release() {
 	dfree(d_sched); dfree(d_iprefix); dfree(d_qbeg); dfree(d_qnext); dfree(d_porder);
 	dfree(d_gbm); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
 	dfree(d_vals); dfree(d_cnt); dfree(d_scan); dfree(d_stat); dfree(d_log);
-	dfree(d_units); dfree(d_ufeat);
+	dfree(d_units); dfree(d_ufeat); dfree(d_stage); dfree(d_uitems); dfree(d_ubeg); dfree(d_unext);
+	stage_cap = uitems_cap = 0;
+	uq_cap = 0;
 	dfree(d_tfeat); dfree(d_tpred); dfree(d_tcost); dfree(d_thits); dfree(d_tkeys); dfree(d_iorder);
 	units_cap = trows_cap = tiles_cap = 0;
 	if (e0) (void)hipEventDestroy(e0);
@@ -1124,34 +977,26 @@ int resolve_params(tm_ring * r) {
 	uint32_t bs = c.batch_size ? c.batch_size : (c.schedule == TM_SCHED_PLAIN ? 256 : 128);
 	if (bs != 64 && bs != 128 && bs != 256)
 		return set_error(TM_ERR_INVALID, "batch_size must be 64, 128 or 256 (got %u)", bs);
-	r->tiles = (int)(bs / 64);
+	// plain: bs queries per (partition, batch) unit, run as 128-query tiles
+	r->plain_bq = bs;
+	r->tiles = c.schedule == TM_SCHED_PLAIN ? (bs == 64 ? 1 : 2) : (int)(bs / 64);
+	r->chunk = 1u << 30;   // work items of the lookup: one per (partition, batch)
 	const size_t max_lds = (size_t)dev().max_lds;
 	const size_t nq_grp = (size_t)r->tiles * 64;
-	if (c.schedule == TM_SCHED_PLAIN) {
-		// one launch per (partition, batch); each workgroup covers a chunk of
-		// the partition's blocks and flushes its LDS key bitmaps once
-		r->waves = WAVES;
-		r->chunk = c.chunk_blocks ? c.chunk_blocks : 16;
-		size_t lds = nq_grp * r->kw * 4;
-		size_t static_lds = nq_grp * 32 + 64;
-		r->stage = (mode_of(c) == 0) && lds <= STAGE_LDS_MAX && lds + static_lds <= max_lds;
-		r->stage_lds = r->stage ? lds : 0;
-	} else {
-		r->variant = (c.variant >= 1 && c.variant <= 7) ? (int)c.variant : 2;
-		int nw = c.waves_per_group ? (int)c.waves_per_group : 8;
-		if (nw != 4 && nw != 8 && nw != 16)
-			return set_error(TM_ERR_INVALID, "waves_per_group must be 4, 8 or 16 (got %d)", nw);
-		r->waves = nw;
-		size_t static_lds = nq_grp * 32 + 64 + (mode_of(c) == 1 ? (size_t)nw * HBUF * 8 : 8) +
-							((mode_of(c) == 0 && (r->variant & 2)) ? (size_t)nw * 64 * 16 : 16);
-		size_t bm = nq_grp * r->kw * 4;
-		r->lds_bm = (mode_of(c) == 0) && bm + static_lds <= max_lds;
-		r->stage_lds = r->lds_bm ? bm : 0;
-		int by_lds = (int)(std::min<size_t>(160 * 1024, max_lds) / (r->stage_lds + static_lds));
-		int by_waves = 32 / nw;
-		int want = c.groups_per_cu ? (int)c.groups_per_cu : 64;
-		r->wgs_per_cu = std::max(1, std::min(want, std::min(by_lds, by_waves)));
-	}
+	r->variant = ((c.variant >= 1 && c.variant <= 7) || c.variant == 10) ? (int)c.variant : 2;
+	int nw = c.waves_per_group ? (int)c.waves_per_group : 8;
+	if (nw != 4 && nw != 8 && nw != 16)
+		return set_error(TM_ERR_INVALID, "waves_per_group must be 4, 8 or 16 (got %d)", nw);
+	r->waves = nw;
+	size_t static_lds = nq_grp * 32 + 64 + (mode_of(c) == 1 ? (size_t)nw * HBUF * 8 : 8) +
+						((mode_of(c) == 0 && (r->variant & 2)) ? (size_t)nw * 64 * 16 : 16);
+	size_t bm = nq_grp * r->kw * 4;
+	r->lds_bm = (mode_of(c) == 0) && bm + static_lds <= max_lds;
+	r->stage_lds = r->lds_bm ? bm : 0;
+	int by_lds = (int)(std::min<size_t>(160 * 1024, max_lds) / (r->stage_lds + static_lds));
+	int by_waves = 32 / nw;
+	int want = c.groups_per_cu ? (int)c.groups_per_cu : 64;
+	r->wgs_per_cu = std::max(1, std::min(want, std::min(by_lds, by_waves)));
 	return TM_OK;
 }
 
@@ -1180,30 +1025,7 @@ void enqueue_front(tm_ring * r, uint32_t nq) {
 					   r->d_list, r->list_cap);
 	hipLaunchKernelGGL(k_build_items, dim3(1), dim3(1024), 0, s, r->d_pcount, r->d_pbase, r->list_cap,
 					   f->part_info, P, r->have_order ? r->d_porder : (const uint32_t *)nullptr,
-					   (uint32_t)(64 * r->tiles), r->chunk, r->d_sched, r->d_iprefix, r->d_qbeg, r->d_qnext);
-}
-
-match_args base_args(tm_ring * r) {
-	match_args a;
-	std::memset(&a, 0, sizeof(a));
-	a.f = r->fib->view();
-	a.q = r->d_q;
-	a.list = r->d_list;
-	a.pbase = r->d_pbase;
-	a.sched = r->d_sched;
-	a.item_prefix = r->d_iprefix;
-	a.n_ranks = r->P;
-	a.chunk = r->chunk;
-	a.qnext = r->d_qnext;
-	a.qbeg = r->d_qbeg;
-	a.plain_rank = -1;
-	a.gbm = r->d_gbm;
-	a.hits = r->d_hits;
-	a.hit_cap = r->hit_cap;
-	a.counters = r->d_cnt;
-	a.log = r->cfg.log_units ? r->d_log : nullptr;
-	a.log_cap = r->log_cap;
-	return a;
+					   r->plain_bq, r->chunk, r->d_sched, r->d_iprefix, r->d_qbeg, r->d_qnext);
 }
 
 void enqueue_finalize(tm_ring * r, uint32_t nq) {
@@ -1254,7 +1076,9 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	const uint32_t nblk = r->fib->n_blocks;
 	if ((r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE) && tiles > 0 && !r->use_iorder && !r->cfg.log_units)
 		nseg = (uint32_t)std::min<uint64_t>((2 * resident + tiles - 1) / tiles, std::max<uint32_t>(nblk / 16, 1));
-	const uint32_t seg_blocks = nseg > 1 ? (nblk + nseg - 1) / nseg : 0;
+	if (r->cfg.chunk_blocks && (r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE))   // fixed segment size
+		nseg = std::max<uint32_t>(1, (nblk + r->cfg.chunk_blocks - 1) / r->cfg.chunk_blocks);
+	const uint32_t seg_blocks = nseg > 1 ? (nblk + nseg - 1) / nseg : std::max<uint32_t>(nblk, 1);
 	if (nseg > 1) nseg = (nblk + seg_blocks - 1) / seg_blocks;
 	const uint64_t items = tiles * nseg;
 	r->last_nseg = nseg;
@@ -1272,6 +1096,10 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.ntiles = (uint32_t)tiles;
 	a.nseg = nseg;
 	a.seg_blocks = seg_blocks;
+	a.blk_lo = 0;
+	a.blk_hi = nblk;
+	a.log_unit = -1;
+	a.qmap = nullptr;
 	a.qnext = r->d_qnext;
 	a.qbeg = r->d_qbeg;
 	a.pcount = r->d_pcount;
@@ -1299,63 +1127,136 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	return TM_OK;
 }
 
-// The plain step: one launch per (partition, batch), as the reference
-// launches one kernel per (partition, batch) (back_end.cc:284-327).
+// The plain step: one launch per (partition, batch) unit, as the reference
+// launches one kernel per (partition, batch) (back_end.cc:284-327).  Each
+// launch runs the persistent kernel's inner loop over the unit only: the
+// batch's queries (through the lookup list) against the partition's blocks,
+// cut into block segments so that the launch spreads over the GPU; key
+// bitmaps go to staging rows that k_or_rows merges into the queries' rows.
 int enqueue_plain(tm_ring * r, uint32_t nq) {
-	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, r->s);
+	hipStream_t s = r->s;
+	const int mode = r->cfg.mode;
+	if (mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 	enqueue_front(r, nq);
 	std::vector<uint4> sched(r->P);
+	std::vector<uint64_t> pbase(r->P + 1, 0);
 	if (r->P) {
-		TM_HIP(hipMemcpyAsync(sched.data(), r->d_sched, sizeof(uint4) * r->P, hipMemcpyDeviceToHost, r->s));
-		TM_HIP(hipStreamSynchronize(r->s));
+		TM_HIP(hipMemcpyAsync(sched.data(), r->d_sched, sizeof(uint4) * r->P, hipMemcpyDeviceToHost, s));
+		TM_HIP(hipMemcpyAsync(pbase.data(), r->d_pbase, sizeof(uint64_t) * (r->P + 1), hipMemcpyDeviceToHost, s));
+		TM_HIP(hipStreamSynchronize(s));
 	}
-	match_args a = base_args(r);
-	match_fn fn = pick_match(mode_of(r->cfg), r->stage, r->tiles);
-	const uint32_t bq = 64 * r->tiles;
+	const uint32_t bq = r->plain_bq;
+	const uint32_t nqg = 64 * r->tiles;
+	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	const tm_fib * f = r->fib;
+	// the units, their block ranges and work items
+	struct unit { uint32_t p, b, nqv, blo, bhi, ntiles, nseg, seg_blocks; };
+	std::vector<unit> units;
+	for (uint32_t k = 0; k < r->P; ++k) {
+		const uint4 sc = sched[k];   // partition, batches, 1, queries
+		if (sc.z == 0) continue;
+		const uint32_t p = sc.x;
+		const uint32_t blo = f->h_part_info[(size_t)p * 8 + 6], nb = f->h_part_info[(size_t)p * 8 + 7];
+		for (uint32_t b = 0; b < sc.y; ++b) {
+			unit u;
+			u.p = p;
+			u.b = b;
+			u.nqv = std::min(bq, sc.w - b * bq);
+			u.blo = blo;
+			u.bhi = blo + nb;
+			u.ntiles = (u.nqv + nqg - 1) / nqg;
+			// segments: about two items per resident workgroup, at least
+			// r->cfg.chunk_blocks (or 4) blocks each
+			const uint32_t min_seg = r->cfg.chunk_blocks ? r->cfg.chunk_blocks : 4;
+			uint64_t want = (2 * resident + u.ntiles - 1) / u.ntiles;
+			uint32_t nseg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (nb + min_seg - 1) / min_seg));
+			if (mode == TM_MODE_KEYS_UNIQUE && !r->lds_bm) nseg = 1;   // rows in HBM: one workgroup per tile
+			u.seg_blocks = std::max<uint32_t>(1, (nb + nseg - 1) / nseg);
+			u.nseg = nb ? (nb + u.seg_blocks - 1) / u.seg_blocks : 1;
+			units.push_back(u);
+		}
+	}
+	const uint32_t nu = (uint32_t)units.size();
+	// per-unit work queues, initialised in one launch
+	TM_CHECK(grow_u32(&r->d_uitems, r->uitems_cap, std::max<uint32_t>(nu, 1)));
+	if (r->uq_cap < nu) {
+		dfree(r->d_ubeg);
+		dfree(r->d_unext);
+		r->uq_cap = 0;
+		TM_CHECK(dalloc(&r->d_ubeg, (size_t)std::max<uint32_t>(nu, 1) * (NQUEUE + 1)));
+		TM_CHECK(dalloc(&r->d_unext, (size_t)std::max<uint32_t>(nu, 1) * NQUEUE));
+		r->uq_cap = nu;
+	}
+	if (mode == TM_MODE_KEYS_UNIQUE && r->stage_cap < (uint64_t)bq * r->kw) {
+		TM_CHECK(grow_u32(&r->d_stage, r->stage_cap, (uint64_t)bq * r->kw));
+		fill_u32(r->d_stage, 0, r->stage_cap, s);
+	}
 	r->n_units = 0;
 	if (r->cfg.log_units) {
 		// one log row per launch: (partition, batch, queries in the batch)
-		std::vector<uint4> units;
-		for (uint32_t k = 0; k < r->P; ++k) {
-			const uint4 sc = sched[k];
-			if (sc.z == 0) continue;
-			for (uint32_t b = 0; b < sc.y; ++b)
-				units.push_back(make_uint4(sc.x, b, std::min(bq, sc.w - b * bq), 0));
-		}
-		const uint32_t nu = (uint32_t)std::min<uint64_t>(units.size(), r->log_cap);
-		if (nu > r->units_cap) {
+		std::vector<uint4> ud(nu);
+		for (uint32_t i = 0; i < nu; ++i) ud[i] = make_uint4(units[i].p, units[i].b, units[i].nqv, 0);
+		const uint32_t nl = (uint32_t)std::min<uint64_t>(nu, r->log_cap);
+		if (nl > r->units_cap) {
 			dfree(r->d_units);
 			dfree(r->d_ufeat);
 			r->units_cap = 0;
-			TM_CHECK(dalloc(&r->d_units, nu));
-			TM_CHECK(dalloc(&r->d_ufeat, (size_t)nu * N_FEAT));
-			r->units_cap = nu;
+			TM_CHECK(dalloc(&r->d_units, nl));
+			TM_CHECK(dalloc(&r->d_ufeat, (size_t)nl * N_FEAT));
+			r->units_cap = nl;
 		}
-		if (nu) {
-			TM_HIP(hipMemcpyAsync(r->d_units, units.data(), sizeof(uint4) * nu, hipMemcpyHostToDevice, r->s));
-			fill_u32(r->d_log, 0, (uint64_t)nu * 3, r->s);
-			launch_unit_features_plain(r->fib->view(), r->d_q, r->d_list, r->d_pbase, r->d_units, nu, bq,
-									   r->d_ufeat, r->s);
-			TM_HIP(hipStreamSynchronize(r->s));   // the unit vector is pageable host memory
+		if (nl) {
+			TM_HIP(hipMemcpyAsync(r->d_units, ud.data(), sizeof(uint4) * nl, hipMemcpyHostToDevice, s));
+			fill_u32(r->d_log, 0, (uint64_t)nl * 3, s);
+			launch_unit_features_plain(f->view(), r->d_q, r->d_list, r->d_pbase, r->d_units, nl, bq, r->d_ufeat, s);
+			TM_HIP(hipStreamSynchronize(s));   // ud is pageable host memory
 		}
-		r->n_units = nu;
+		r->n_units = nl;
 	}
-	uint32_t launches = 0;
-	TM_HIP(hipEventRecord(r->em0, r->s));
-	for (uint32_t k = 0; k < r->P; ++k) {
-		const uint4 sc = sched[k];   // partition, batches, chunks, queries
-		if (sc.z == 0) continue;
-		for (uint32_t b = 0; b < sc.y; ++b) {
-			a.plain_rank = (int32_t)k;
-			a.plain_batch = b;
-			a.plain_unit = launches;
-			hipLaunchKernelGGL(fn, dim3(sc.z), dim3(WG), r->stage_lds, r->s, a);
-			++launches;
-		}
+	std::vector<uint32_t> items(std::max<uint32_t>(nu, 1), 0);
+	for (uint32_t i = 0; i < nu; ++i) items[i] = units[i].ntiles * units[i].nseg;
+	TM_HIP(hipMemcpyAsync(r->d_uitems, items.data(), 4ull * nu, hipMemcpyHostToDevice, s));
+	if (nu)
+		hipLaunchKernelGGL(k_init_unit_queues, dim3((nu + 255) / 256), dim3(256), 0, s, r->d_uitems, nu, r->d_ubeg,
+						   r->d_unext);
+	TM_HIP(hipStreamSynchronize(s));   // items is pageable host memory
+
+	mq_args a;
+	std::memset(&a, 0, sizeof(a));
+	a.f = f->view();
+	a.q = r->d_q;
+	a.hits = r->d_hits;
+	a.hit_cap = r->hit_cap;
+	a.counters = r->d_cnt;
+	a.log_cap = r->log_cap;
+	a.full_exit = 0;   // the staging rows start empty for every unit: nothing to learn
+	a.last_mask = 0xFFFFFFFFu;
+	mq_fn fn = pick_match_q(mode_of(r->cfg), r->lds_bm, r->tiles, r->waves, r->variant);
+	TM_HIP(hipEventRecord(r->em0, s));
+	for (uint32_t i = 0; i < nu; ++i) {
+		const unit & u = units[i];
+		a.qmap = r->d_list + pbase[u.p] + (uint64_t)u.b * bq;
+		a.nq = u.nqv;
+		a.gbm = r->d_stage;
+		a.blk_lo = u.blo;
+		a.blk_hi = u.bhi;
+		a.ntiles = u.ntiles;
+		a.nseg = u.nseg;
+		a.seg_blocks = u.seg_blocks;
+		a.qbeg = r->d_ubeg + (size_t)i * (NQUEUE + 1);
+		a.qnext = r->d_unext + (size_t)i * NQUEUE;
+		a.log = (r->cfg.log_units && i < r->n_units) ? r->d_log : nullptr;
+		a.log_unit = a.log ? (int32_t)i : -1;
+		const uint64_t nit = (uint64_t)u.ntiles * u.nseg;
+		const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(resident, nit));
+		hipLaunchKernelGGL(fn, dim3(grid), dim3(r->waves * 64), r->stage_lds, s, a);
+		if (mode == TM_MODE_KEYS_UNIQUE)
+			hipLaunchKernelGGL(k_or_rows, dim3((u.nqv * r->kw + 255) / 256), dim3(256), 0, s, r->d_stage, r->kw,
+							   a.qmap, u.nqv, r->d_gbm);
 	}
 	TM_HIP(hipGetLastError());
-	TM_HIP(hipEventRecord(r->em1, r->s));
-	r->st.n_items = launches;
+	TM_HIP(hipEventRecord(r->em1, s));
+	r->st.n_items = nu;
 	enqueue_finalize(r, nq);
 	return TM_OK;
 }

@@ -160,7 +160,7 @@ def test_batch_sizes_and_chunks(gpu, batch):
             assert_same(o, v, eo, ev, sort_got=False, what=f"b{batch} s{sched} c{chunk}")
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 10])
 def test_inner_loop_variants_vs_oracle(gpu, variant):
     """Every persistent inner-loop variant (scalar or LDS query words, key
     log, early exit), large submissions (one tile per item) and small ones
@@ -178,22 +178,25 @@ def test_full_key_set_exit(gpu, n_keys):
     """With few distinct keys most queries collect every key early; the exit
     that stops testing such queries must not change any result (key counts
     around word boundaries of the bitmap row: 5, 37, 64, 100)."""
-    cw, cko, ck, q, s = tm.workload(30000, 2000, seed=950 + n_keys)
+    cw, cko, ck, q, s = tm.workload(150000, 2000, seed=950 + n_keys)
     ck = (ck % n_keys).astype(np.uint32) * 7 + 3          # n_keys distinct keys
     # keys per filter must stay distinct for the canonical table
     key_lists = [np.unique(ck[cko[i]:cko[i + 1]]) for i in range(len(cw))]
     cko = np.zeros(len(cw) + 1, np.uint64)
     np.cumsum([len(k) for k in key_lists], out=cko[1:])
     ck = np.concatenate(key_lists).astype(np.uint32)
-    fib, *_ = tm.build_fib(cw, cko, ck, 3000)
+    fib, *_ = tm.build_fib(cw, cko, ck, 10000)
     eo, ev = orc.match(cw, cko, ck, q, mode=0)
-    res = {}
-    for fe in (1, 2):
-        o, v, st = tm.match(fib, q, full_exit=fe)
-        assert_same(o, v, eo, ev, sort_got=False, what=f"full_exit {fe}")
-        res[fe] = st["n_hits"]
-    log(f"n_keys {n_keys}: hits processed {res[1]} with the exit, {res[2]} without")
-    assert res[1] < res[2]
+    # one segment (bitmaps accumulate in LDS), and 8-block segments (the
+    # rows other segments merged into HBM count too)
+    for seg in (1 << 30, 8):
+        res = {}
+        for fe in (1, 2):
+            o, v, st = tm.match(fib, q, full_exit=fe, chunk_blocks=seg)
+            assert_same(o, v, eo, ev, sort_got=False, what=f"full_exit {fe} segment {seg}")
+            res[fe] = st["n_hits"]
+        log(f"n_keys {n_keys} segment {seg}: hits processed {res[1]} with the exit, {res[2]} without")
+        assert res[1] < res[2]
 
 
 def bitf(*bits):
