@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--queries", type=int, default=1_000_000)
     ap.add_argument("--maxp", type=int, default=256 * 1024)
     ap.add_argument("--batch", type=int, default=256, help="queries per (partition, batch) of the plain form")
-    ap.add_argument("--tile", type=int, default=0, help="queries per persistent work item (0: default)")
+    ap.add_argument("--tile", type=int, default=0, help="queries per persistent work item (0: autotuned)")
+    ap.add_argument("--autotune", type=int, default=1, help="autotune tile size and waves before timing (0: off)")
     ap.add_argument("--chunk", type=int, default=0, help="blocks per work item (0: default)")
     ap.add_argument("--groups-per-cu", type=int, default=0)
     ap.add_argument("--schedule", choices=["persistent", "plain"], default="persistent")
@@ -62,6 +63,8 @@ def parse():
     ap.add_argument("--plain-queries", type=int, default=16384,
                     help="queries of the plain form's step (a prefix of the resident queries)")
     ap.add_argument("--model", type=int, default=1, help="cost model leg (0: skip)")
+    ap.add_argument("--uneven-rounds", type=int, default=8,
+                    help="batches per partition in the uneven-batching leg (0: skip the leg)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds per serving load level (0: skip)")
     ap.add_argument("--serve-request", type=int, default=64, help="queries per serving request")
     ap.add_argument("--serve-insert", type=int, default=10000, help="filters per insert (0: none)")
@@ -184,6 +187,54 @@ def cost_model_leg(fib, q, ring, nq, steps, base_ms, n_log):
     return res
 
 
+def uneven_leg(cw, cko, ck, args):
+    """Uneven batching at 10M filters: partitions are runs of the sorted
+    table with Zipf(1.2) sizes in [1k, 1M], each receives several batches of
+    log-uniform size in [1, 4096] (explicit units, no lookup).  The
+    persistent launch and the plain per-(partition, batch) form run the same
+    units."""
+    t0 = time.time()
+    off = tm.zipf_runs(len(cw), 1000, 1_000_000, s=1.2, seed=3)
+    fib, masks = tm.fib_from_runs(cw, cko, ck, off)
+    parts, offs = [], [0]
+    for r in range(args.uneven_rounds):
+        up, uo = tm.uneven_units(len(off) - 1, 1, 4096, seed=100 + r)
+        parts.append(up)
+        offs.extend((offs[-1] + uo[1:].astype(np.int64)).tolist())
+    up = np.concatenate(parts).astype(np.uint32)
+    uo = np.array(offs, np.uint64).astype(np.uint32)
+    nq = int(uo[-1])
+    g = tm.Generator(seed=args.seed)
+    q, _ = g.queries(nq, len(cw), first=5_000_000)
+    log(f"uneven: {len(off) - 1} partitions (sizes {int(np.diff(off).min())}..{int(np.diff(off).max())}), "
+        f"{len(up)} units, {nq} queries, setup {time.time() - t0:.1f}s")
+    res = {"partitions": int(len(off) - 1), "units": int(len(up)), "queries": nq,
+           "partition_sizes": [int(np.diff(off).min()), int(np.median(np.diff(off))), int(np.diff(off).max())]}
+    ring = tm.Ring(fib, nq)
+    ring.submit_units(q, up, uo)
+    ring.wait()
+    ref = ring.results()
+    for name, sched, steps in (("persistent", tm.SCHED_PERSISTENT, 3), ("plain", tm.SCHED_PLAIN, 1)):
+        ring.configure(schedule=sched)
+        ring.submit_units(None, up, uo)
+        ring.wait()
+        tt = []
+        for i in range(steps):
+            t = time.perf_counter()
+            ring.submit_units(None, up, uo)
+            st = ring.wait()
+            tt.append(time.perf_counter() - t)
+        o, v = ring.results()
+        same = bool(np.array_equal(o, ref[0]) and np.array_equal(v, ref[1]))
+        res[name] = {"queries_per_s": nq / float(np.mean(tt)), "ms_per_step": 1e3 * float(np.mean(tt)),
+                     "match_ms": st.match_ms, "launches": st.n_items, "same_results": same}
+        log(f"uneven {name}: {res[name]}")
+    res["speedup_persistent_vs_plain"] = res["persistent"]["queries_per_s"] / res["plain"]["queries_per_s"]
+    ring.close()
+    fib.close()
+    return res
+
+
 def serving_leg(fib, args, n_filters):
     """Poisson arrivals of fixed-size requests through tm_server at 30/70/95%
     of the closed-loop peak, with inserts interleaved."""
@@ -245,6 +296,7 @@ def serving_leg(fib, args, n_filters):
     st = srv.stats()
     srv.close()
     return {"peak_qps": peak, "request_queries": req, "slots": 4, "slot_queries": 16384,
+            "device_ms_per_step": st["device_ms_sum"] / max(st["n_steps"], 1),
             "insert_filters": args.serve_insert, "insert_every_ms": args.serve_insert_ms,
             "levels": levels, "steps": st["n_batches"], "budget_waits": st["n_budget_waits"],
             "requests_per_step": st["requests_per_batch_sum"] / max(st["n_batches"], 1)}
@@ -289,6 +341,18 @@ def main():
     ring.submit(q)
     st = ring.wait()
     log(f"first step (with H2D and capacity growth): {time.time() - t:.2f}s, overflow={st.overflow}")
+    tuned = None
+    if args.autotune and sched == tm.SCHED_PERSISTENT and not args.tile:
+        # the autotuner (tm_ring_autotune) picks the tile size and waves per
+        # workgroup on this workload, before the timed steps
+        cands = [dict(batch_size=b, waves_per_group=w, chunk_blocks=c, use_graph=int(args.graph))
+                 for b, w in ((64, 4), (64, 8), (128, 4), (128, 8)) for c in (0, 4096)]
+        best, times = ring.autotune(nq, cands, reps=1)
+        cfg = ring.cfg
+        kw.update(batch_size=cands[best]["batch_size"], waves_per_group=cands[best]["waves_per_group"],
+                  chunk_blocks=cands[best]["chunk_blocks"])
+        tuned = {"candidates": cands, "step_ms": [float(x) for x in times], "best": cands[best]}
+        log(f"autotuned: {cands[best]} ({times[best]:.1f} ms); all {np.round(times, 1).tolist()}")
     for i in range(max(0, args.warmup - 1)):
         ring.submit_resident(nq)
         ring.wait()
@@ -361,6 +425,10 @@ def main():
         model = cost_model_leg(fib, q, ring, nq, max(1, args.steps // 2), ms_per_step, 4096)
     ring.close()
 
+    uneven = None
+    if args.uneven_rounds > 0:
+        uneven = uneven_leg(cw, cko, ck, args)
+
     serving = None
     if args.serve_seconds > 0:
         serving = serving_leg(fib, args, args.filters)
@@ -401,6 +469,8 @@ def main():
         "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)), "max": float(np.max(step_ms))},
         "serving": serving,
         "cost_model": model,
+        "uneven_batching": uneven,
+        "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_match", "kernel_ms": kernel_s * 1e3,

@@ -182,12 +182,14 @@ typedef struct tm_match_config {
 	uint32_t batch_size;     /* PLAIN: queries per (partition, batch) launch,
 	                            0 = 256 (reference parameters.hh:22);
 	                            PERSISTENT: queries per work item (64, 128
-	                            or 256), 0 = 128                             */
+	                            or 256), 0 = 64                              */
 	uint32_t chunk_blocks;   /* PLAIN: least blocks per segment of a unit
 	                            (0 = 4); PERSISTENT: blocks per segment of
-	                            the table (0 = automatic: segments only for
-	                            submissions too small to fill the GPU)       */
-	uint32_t waves_per_group;/* waves per workgroup (1..16); 0 = autotuned   */
+	                            the table (0 = automatic: at most 1024, and
+	                            smaller for submissions too small to fill
+	                            the GPU)                                     */
+	uint32_t waves_per_group;/* waves per workgroup (4, 8 or 16); 0 = 4
+	                            (PERSISTENT) or 8 (PLAIN)                    */
 	uint32_t groups_per_cu;  /* PERSISTENT: resident workgroups per CU; 0 = auto */
 	uint64_t hit_capacity;   /* HITS / KEYS_ALL: hit slots; 0 = default      */
 	uint32_t use_graph;      /* PERSISTENT: replay steady-state steps from a
@@ -258,6 +260,16 @@ int tm_ring_query_buffer(tm_ring * ring, uint32_t ** dev_words);
 int tm_submit(tm_ring * ring, const uint32_t * query_words, uint32_t nq);
 /* Same, with the queries already in tm_ring_query_buffer(). */
 int tm_submit_resident(tm_ring * ring, uint32_t nq);
+/* Submit explicit (partition, query range) units instead of the lookup:
+ * unit u matches queries [unit_off[u], unit_off[u+1]) against partition
+ * unit_part[u] only (its mask still prunes, which is exact).  unit_off has
+ * n_units + 1 entries from 0 to nq.  query_words: nq x 6 host words, or
+ * NULL for the queries already in tm_ring_query_buffer().  This is the
+ * reference's back-end entry, a batch of queries for one partition
+ * (back_end::process_batch, back_end.cc:256-302), without the front end;
+ * results per query as for tm_submit. */
+int tm_submit_units(tm_ring * ring, const uint32_t * query_words, uint32_t nq, const uint32_t * unit_part,
+					const uint32_t * unit_off, uint32_t n_units);
 /* TM_OK when the last submission finished, TM_ERR_BUSY while running. */
 int tm_poll(tm_ring * ring);
 /* Block until the last submission finished; fill *st (may be NULL). */

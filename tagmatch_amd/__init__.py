@@ -122,6 +122,7 @@ _SIGS = {
     "tm_ring_query_buffer": (C.c_int, [P, C.POINTER(P)]),
     "tm_submit": (C.c_int, [P, P, C.c_uint32]),
     "tm_submit_resident": (C.c_int, [P, C.c_uint32]),
+    "tm_submit_units": (C.c_int, [P, P, C.c_uint32, P, P, C.c_uint32]),
     "tm_poll": (C.c_int, [P]),
     "tm_wait": (C.c_int, [P, C.POINTER(Status)]),
     "tm_read_results": (C.c_int, [P, P, P, C.c_uint64]),
@@ -385,6 +386,45 @@ def build_fib(words, key_off, keys, maxp, reserve=None):
     return fib, perm, off, masks
 
 
+def zipf_runs(n, smin, smax, s=1.2, seed=1):
+    """Offsets cutting n consecutive filters into runs of Zipf(s) sizes in
+    [smin, smax] (a power law p(size) ~ size^-s); the last run takes the
+    remainder (at least 1)."""
+    rng = np.random.default_rng(seed)
+    off = [0]
+    a = 1.0 - s
+    lo, hi = smin ** a, smax ** a
+    while off[-1] < n:
+        u = rng.random()
+        size = int(round((lo + u * (hi - lo)) ** (1.0 / a)))
+        off.append(min(n, off[-1] + max(size, 1)))
+    return np.array(off, np.uint32)
+
+
+def fib_from_runs(words, key_off, keys, offsets, reserve=None):
+    """Upload a table whose partitions are the given runs of consecutive
+    filters; each mask is the AND of its run (all-zero: selected by every
+    query).  Filter ids are positions in `words`."""
+    words = np.ascontiguousarray(words, np.uint32)
+    off = np.asarray(offsets, np.int64)
+    masks = np.stack([np.bitwise_and.reduce(words[off[i]:off[i + 1]], axis=0) if off[i + 1] > off[i]
+                      else np.full(6, 0xFFFFFFFF, np.uint32) for i in range(len(off) - 1)]).astype(np.uint32)
+    fid = np.arange(words.shape[0], dtype=np.uint32)
+    return Fib(words, off.astype(np.uint32), masks, np.asarray(key_off).astype(np.uint32), keys, fid,
+               reserve=reserve), masks
+
+
+def uneven_units(n_parts, bmin=1, bmax=4096, seed=2):
+    """Batch sizes per partition, log-uniform in [bmin, bmax]; returns
+    (unit_part, unit_off) for Ring.submit_units with one unit per partition."""
+    rng = np.random.default_rng(seed)
+    b = np.exp(rng.uniform(np.log(bmin), np.log(bmax + 1), n_parts)).astype(np.int64)
+    b = np.clip(b, bmin, bmax)
+    uo = np.zeros(n_parts + 1, np.uint64)
+    np.cumsum(b, out=uo[1:])
+    return np.arange(n_parts, dtype=np.uint32), uo.astype(np.uint32)
+
+
 def config(mode=MODE_KEYS_UNIQUE, schedule=SCHED_PERSISTENT, **kw):
     c = MatchConfig()
     load().tm_match_config_default(C.byref(c))
@@ -414,6 +454,16 @@ class Ring:
         q = np.ascontiguousarray(queries, np.uint32).reshape(-1, 6)
         self.nq = q.shape[0]
         check(load().tm_submit(self.h, _ptr(q), self.nq), "tm_submit")
+
+    def submit_units(self, queries, unit_part, unit_off):
+        """Explicit (partition, query range) units, no lookup (tm_submit_units);
+        queries None: the resident ones."""
+        up = np.ascontiguousarray(unit_part, np.uint32)
+        uo = np.ascontiguousarray(unit_off, np.uint32)
+        nq = int(uo[-1]) if len(uo) else 0
+        q = None if queries is None else np.ascontiguousarray(queries, np.uint32).reshape(-1, 6)
+        self.nq = nq
+        check(load().tm_submit_units(self.h, _ptr(q), nq, _ptr(up), _ptr(uo), len(up)), "tm_submit_units")
 
     def submit_resident(self, nq):
         self.nq = nq

@@ -31,11 +31,11 @@ namespace tmx {
 constexpr int N_FEAT = TM_UNIT_FEATURES;
 constexpr int N_TARGET = 3;
 
-// Plain units: units[u] = (partition, batch index, queries in the batch, 0);
-// the batch's query ids are list[pbase[p] + batch * bq ...].
+// Plain units: units[u] = (partition, batch index, queries in the batch,
+// first list entry); the batch's query ids are list[first .. first + queries),
+// at most 256 of them.
 void launch_unit_features_plain(const dev_fib_view & f, const uint32_t * q, const uint32_t * list,
-								const uint64_t * pbase, const uint4 * units, uint32_t n_units, uint32_t bq,
-								float * out, hipStream_t s);
+								const uint4 * units, uint32_t n_units, float * out, hipStream_t s);
 
 // Tile units: row t * P + p for query tile t (queries [t*nqt, (t+1)*nqt) of
 // nq) and partition p; the batch is the tile's queries that select p.

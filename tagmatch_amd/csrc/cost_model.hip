@@ -272,19 +272,18 @@ __device__ inline uint32_t pop6(const uint32_t * w) {
 // One 256-thread workgroup per plain unit.
 __global__ void __launch_bounds__(256)
 k_unit_features_plain(dev_fib_view f, const uint32_t * __restrict__ q, const uint32_t * __restrict__ list,
-					  const uint64_t * __restrict__ pbase, const uint4 * __restrict__ units, uint32_t bq,
-					  float * __restrict__ out) {
+					  const uint4 * __restrict__ units, float * __restrict__ out) {
 	__shared__ uint32_t red[2][4 * 12];
 	const uint4 u = units[blockIdx.x];
 	const uint32_t p = u.x;
 	const uint32_t * pinfo = f.part_info + (size_t)p * 8;
 	const uint32_t * pstat = f.part_stat + (size_t)p * 8;
-	// batches hold at most 256 queries (bq <= 256): thread t takes query t
+	// batches hold at most 256 queries: thread t takes query t
 	uint32_t c = 0, o = 0;
 	bool any = false;
 	const uint32_t t = threadIdx.x;
-	if (t < u.z && t < bq) {
-		const uint32_t qi = list[pbase[p] + (uint64_t)u.y * bq + t];
+	if (t < u.z) {
+		const uint32_t qi = list[u.w + t];
 		const uint32_t * w = q + (size_t)qi * 8;
 		c = pop6(w);
 		o = __popc(w[0] & pstat[0]) + __popc(w[1] & pstat[1]) + __popc(w[2] & pstat[2]) +
@@ -394,10 +393,9 @@ k_tile_costs(const float * __restrict__ X, const float * __restrict__ Y, uint32_
 namespace tmx {
 
 void launch_unit_features_plain(const dev_fib_view & f, const uint32_t * q, const uint32_t * list,
-								const uint64_t * pbase, const uint4 * units, uint32_t n_units, uint32_t bq,
-								float * out, hipStream_t s) {
+								const uint4 * units, uint32_t n_units, float * out, hipStream_t s) {
 	if (n_units == 0) return;
-	hipLaunchKernelGGL(k_unit_features_plain, dim3(n_units), dim3(256), 0, s, f, q, list, pbase, units, bq, out);
+	hipLaunchKernelGGL(k_unit_features_plain, dim3(n_units), dim3(256), 0, s, f, q, list, units, out);
 }
 
 void launch_unit_features_tiles(const dev_fib_view & f, const uint32_t * q, uint32_t nq, uint32_t nqt,

@@ -40,6 +40,7 @@ namespace {
 
 constexpr int WG = 256;          // threads per match workgroup (4 waves)
 constexpr int NQUEUE = 8;        // work queues, one per XCD of an MI355X
+constexpr uint32_t SEG_MAX_BLOCKS = 1024;   // persistent: largest block segment (measured at 10M)
 constexpr uint32_t NO_QUERY = 0xFFFFFFFFu;
 
 __device__ inline uint32_t mbcnt64(uint64_t m) {
@@ -187,6 +188,11 @@ struct mq_args {
 	const uint32_t * __restrict__ porder;     // partition schedule order (NULL: 0..P-1)
 	const uint32_t * __restrict__ iorder;     // query tile of each work item (NULL: identity; nseg 1 only)
 	const uint32_t * __restrict__ qmap;       // query of each local index (NULL: identity)
+	const uint32_t * __restrict__ units;      // explicit units, 8 words each: partition,
+	                                          // first query, queries, tiles, segments,
+	                                          // blocks per segment, first block, end block
+	const uint32_t * __restrict__ uprefix;    // n_units + 1: first work item of each unit
+	uint32_t n_units;
 	uint32_t blk_lo, blk_hi;                  // blocks the launch covers
 	int32_t log_unit;                         // >= 0: add hits, keys, ticks into this log row
 	uint32_t ntiles;                          // query tiles
@@ -346,7 +352,27 @@ k_match_q(mq_args a) {
 		// one contiguous range of blocks (small submissions spread over the
 		// whole GPU); segment-major, so concurrent items share filter blocks.
 		uint32_t item, seg_b0 = 0, seg_b1 = 0xFFFFFFFFu, r_begin = 0, r_end = P;
-		if (a.nseg > 1 || a.blk_lo != 0 || a.blk_hi < a.f.n_blocks) {
+		uint32_t q0, q_end = a.nq;
+		if (a.units) {
+			// explicit (partition, query range) units: the item's unit by
+			// binary search over the item prefix; segment-major in the unit
+			uint32_t lo = 0, hi = a.n_units;
+			while (hi - lo > 1) {
+				const uint32_t mid = (lo + hi) >> 1;
+				if (sld1(a.uprefix + mid) <= item0) lo = mid; else hi = mid;
+			}
+			const uint4 u0 = sld4(a.units + (size_t)lo * 8);       // part, first query, queries, tiles
+			const uint4 u1 = sld4(a.units + (size_t)lo * 8 + 4);   // segments, seg blocks, first, end block
+			const uint32_t local = item0 - sld1(a.uprefix + lo);
+			const uint32_t seg = local / u0.w;
+			item = local - seg * u0.w;
+			q0 = u0.y + item * NQ;
+			q_end = u0.y + u0.z;
+			seg_b0 = u1.z + seg * u1.y;
+			seg_b1 = min(seg_b0 + u1.y, u1.w);
+			r_begin = u0.x;
+			r_end = seg_b1 > seg_b0 ? u0.x + 1 : u0.x;
+		} else if (a.nseg > 1 || a.blk_lo != 0 || a.blk_hi < a.f.n_blocks) {
 			const uint32_t seg = item0 / a.ntiles;
 			item = item0 - seg * a.ntiles;
 			seg_b0 = a.blk_lo + seg * a.seg_blocks;
@@ -360,12 +386,12 @@ k_match_q(mq_args a) {
 		} else {
 			item = a.iorder ? sld1(a.iorder + item0) : item0;
 		}
-		const uint32_t q0 = item * NQ;
+		if (!a.units) q0 = item * NQ;
 
 		for (uint32_t t = tid; t < (uint32_t)NQ; t += NT) {
 			uint32_t * d = &s_q[t * 8];
 			const uint32_t qi = q0 + t;
-			if (qi < a.nq) {
+			if (qi < q_end) {
 				const uint32_t gq = a.qmap ? a.qmap[qi] : qi;
 				const uint4 x = *reinterpret_cast<const uint4 *>(a.q + (size_t)gq * 8);
 				const uint2 y = *reinterpret_cast<const uint2 *>(a.q + (size_t)gq * 8 + 4);
@@ -392,7 +418,7 @@ k_match_q(mq_args a) {
 				sink.bm_base = s_bm;
 			} else {
 				sink.bm_base = a.gbm + (size_t)q0 * kw;
-				const uint32_t nw = (min(a.nq, q0 + NQ) - q0) * kw;
+				const uint32_t nw = (min(q_end, q0 + NQ) - q0) * kw;
 				for (uint32_t w = tid; w < nw; w += NT) sink.bm_base[w] = 0;
 			}
 		}
@@ -454,7 +480,7 @@ k_match_q(mq_args a) {
 					// their rows into gbm: the query's keys so far are the union.
 					const uint32_t * row = &s_bm[(t * 64 + lane) * kw];
 					const uint32_t qrow = q0 + t * 64 + lane;
-					const uint32_t * grow = (a.nseg > 1 && qrow < a.nq) ? a.gbm + (size_t)qrow * kw : nullptr;
+					const uint32_t * grow = (a.nseg > 1 && qrow < q_end) ? a.gbm + (size_t)qrow * kw : nullptr;
 					uint32_t all = 0xFFFFFFFFu;
 					if ((kw & 3) == 0) {
 						const uint4 * r4 = reinterpret_cast<const uint4 *>(row);
@@ -622,12 +648,14 @@ k_match_q(mq_args a) {
 		}
 		__syncthreads();
 		if (MODE == 0 && LDS_BM) {
-			const uint32_t nw = (min(a.nq, q0 + NQ) - q0) * kw;
+			const uint32_t nw = (min(q_end, q0 + NQ) - q0) * kw;
 			uint32_t * dst = a.gbm + (size_t)q0 * kw;
 			if (a.nseg > 1) {   // other segments of the tile write the same rows
 				for (uint32_t w = tid; w < nw; w += NT) {
 					const uint32_t v = s_bm[w];
-					if (v) atomicOr(&dst[w], v);
+					// an atomic only for bits the row does not hold yet: rows
+					// fill up, and most words are then complete
+					if (v && (v & ~dst[w])) atomicOr(&dst[w], v);
 				}
 			} else {
 				for (uint32_t w = tid; w < nw; w += NT) dst[w] = s_bm[w];
@@ -924,6 +952,13 @@ struct tm_ring {
 
 	uint32_t last_nq = 0;
 	uint64_t fib_version = 0;  // FIB version the device arrays are sized for
+	// explicit units (tm_submit_units): the last submission's units
+	bool units_mode = false;
+	std::vector<uint32_t> u_part, u_off;
+	uint32_t * d_xunits = nullptr;     // 8 words per unit
+	uint32_t * d_xprefix = nullptr;    // items prefix, n + 1
+	uint32_t * d_iota = nullptr;       // 0 .. max_q - 1 (plain units' query lists)
+	uint64_t xunits_cap = 0, xprefix_cap = 0;
 	uint32_t last_nseg = 1;    // block segments per tile of the last persistent step
 	bool pending = false;
 	bool match_events = false;
@@ -946,6 +981,8 @@ void tm_ring::release() {
 	dfree(d_gbm); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
 	dfree(d_vals); dfree(d_cnt); dfree(d_scan); dfree(d_stat); dfree(d_log);
 	dfree(d_units); dfree(d_ufeat); dfree(d_stage); dfree(d_uitems); dfree(d_ubeg); dfree(d_unext);
+	dfree(d_xunits); dfree(d_xprefix); dfree(d_iota);
+	xunits_cap = xprefix_cap = 0;
 	stage_cap = uitems_cap = 0;
 	uq_cap = 0;
 	dfree(d_tfeat); dfree(d_tpred); dfree(d_tcost); dfree(d_thits); dfree(d_tkeys); dfree(d_iorder);
@@ -974,7 +1011,7 @@ int resolve_params(tm_ring * r) {
 						 r->fib->n_keys);
 	// default batch: the reference's 256 for the plain form (parameters.hh:22),
 	// 128-query tiles for the persistent kernel (measured best at 10M filters)
-	uint32_t bs = c.batch_size ? c.batch_size : (c.schedule == TM_SCHED_PLAIN ? 256 : 128);
+	uint32_t bs = c.batch_size ? c.batch_size : (c.schedule == TM_SCHED_PLAIN ? 256 : 64);
 	if (bs != 64 && bs != 128 && bs != 256)
 		return set_error(TM_ERR_INVALID, "batch_size must be 64, 128 or 256 (got %u)", bs);
 	// plain: bs queries per (partition, batch) unit, run as 128-query tiles
@@ -984,7 +1021,7 @@ int resolve_params(tm_ring * r) {
 	const size_t max_lds = (size_t)dev().max_lds;
 	const size_t nq_grp = (size_t)r->tiles * 64;
 	r->variant = ((c.variant >= 1 && c.variant <= 7) || c.variant == 10) ? (int)c.variant : 2;
-	int nw = c.waves_per_group ? (int)c.waves_per_group : 8;
+	int nw = c.waves_per_group ? (int)c.waves_per_group : (c.schedule == TM_SCHED_PLAIN ? 8 : 4);
 	if (nw != 4 && nw != 8 && nw != 16)
 		return set_error(TM_ERR_INVALID, "waves_per_group must be 4, 8 or 16 (got %d)", nw);
 	r->waves = nw;
@@ -1072,10 +1109,17 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	// the items fill the GPU (about two per resident workgroup).  Needs the
 	// key bitmaps in LDS (flushed with atomicOr) or the hit-list modes.
 	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	// Segments also bound the blocks a work item streams: with segment-major
+	// items, the workgroups on an XCD share one segment's blocks (at most
+	// SEG_MAX_BLOCKS x 2 KiB) in its L2 instead of each streaming the whole
+	// table from HBM.
 	uint32_t nseg = 1;
 	const uint32_t nblk = r->fib->n_blocks;
-	if ((r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE) && tiles > 0 && !r->use_iorder && !r->cfg.log_units)
+	const bool can_cut = (r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE) && !r->use_iorder && !r->cfg.log_units;
+	if (can_cut && tiles > 0) {
 		nseg = (uint32_t)std::min<uint64_t>((2 * resident + tiles - 1) / tiles, std::max<uint32_t>(nblk / 16, 1));
+		nseg = std::max<uint32_t>(nseg, (nblk + SEG_MAX_BLOCKS - 1) / SEG_MAX_BLOCKS);
+	}
 	if (r->cfg.chunk_blocks && (r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE))   // fixed segment size
 		nseg = std::max<uint32_t>(1, (nblk + r->cfg.chunk_blocks - 1) / r->cfg.chunk_blocks);
 	const uint32_t seg_blocks = nseg > 1 ? (nblk + nseg - 1) / nseg : std::max<uint32_t>(nblk, 1);
@@ -1127,6 +1171,31 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	return TM_OK;
 }
 
+// A (partition, query batch) unit of the plain schedule: its queries are
+// list[qbase .. qbase + nqv) (the lookup list, or the identity for explicit
+// units), its blocks [blo, bhi), cut into nseg segments of seg_blocks.
+struct punit {
+	uint32_t p, b, nqv, blo, bhi, ntiles, nseg, seg_blocks;
+	uint64_t qbase;
+};
+
+void plain_segments(const tm_ring * r, punit & u) {
+	const uint32_t nqg = 64 * r->tiles;
+	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	const uint32_t nb = u.bhi - u.blo;
+	u.ntiles = (u.nqv + nqg - 1) / nqg;
+	// segments: about two items per resident workgroup, at least
+	// r->cfg.chunk_blocks (or 4) blocks each
+	const uint32_t min_seg = r->cfg.chunk_blocks ? r->cfg.chunk_blocks : 4;
+	const uint64_t want = (2 * resident + std::max<uint32_t>(u.ntiles, 1) - 1) / std::max<uint32_t>(u.ntiles, 1);
+	uint32_t nseg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (nb + min_seg - 1) / min_seg));
+	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->lds_bm) nseg = 1;   // rows in HBM: one workgroup per tile
+	u.seg_blocks = std::max<uint32_t>(1, (nb + nseg - 1) / nseg);
+	u.nseg = nb ? (nb + u.seg_blocks - 1) / u.seg_blocks : 1;
+}
+
+int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit> & units);
+
 // The plain step: one launch per (partition, batch) unit, as the reference
 // launches one kernel per (partition, batch) (back_end.cc:284-327).  Each
 // launch runs the persistent kernel's inner loop over the unit only: the
@@ -1135,8 +1204,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 // bitmaps go to staging rows that k_or_rows merges into the queries' rows.
 int enqueue_plain(tm_ring * r, uint32_t nq) {
 	hipStream_t s = r->s;
-	const int mode = r->cfg.mode;
-	if (mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
+	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 	enqueue_front(r, nq);
 	std::vector<uint4> sched(r->P);
 	std::vector<uint64_t> pbase(r->P + 1, 0);
@@ -1146,36 +1214,34 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 		TM_HIP(hipStreamSynchronize(s));
 	}
 	const uint32_t bq = r->plain_bq;
-	const uint32_t nqg = 64 * r->tiles;
-	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
 	const tm_fib * f = r->fib;
-	// the units, their block ranges and work items
-	struct unit { uint32_t p, b, nqv, blo, bhi, ntiles, nseg, seg_blocks; };
-	std::vector<unit> units;
+	std::vector<punit> units;
 	for (uint32_t k = 0; k < r->P; ++k) {
 		const uint4 sc = sched[k];   // partition, batches, 1, queries
 		if (sc.z == 0) continue;
 		const uint32_t p = sc.x;
 		const uint32_t blo = f->h_part_info[(size_t)p * 8 + 6], nb = f->h_part_info[(size_t)p * 8 + 7];
 		for (uint32_t b = 0; b < sc.y; ++b) {
-			unit u;
+			punit u;
 			u.p = p;
 			u.b = b;
 			u.nqv = std::min(bq, sc.w - b * bq);
 			u.blo = blo;
 			u.bhi = blo + nb;
-			u.ntiles = (u.nqv + nqg - 1) / nqg;
-			// segments: about two items per resident workgroup, at least
-			// r->cfg.chunk_blocks (or 4) blocks each
-			const uint32_t min_seg = r->cfg.chunk_blocks ? r->cfg.chunk_blocks : 4;
-			uint64_t want = (2 * resident + u.ntiles - 1) / u.ntiles;
-			uint32_t nseg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (nb + min_seg - 1) / min_seg));
-			if (mode == TM_MODE_KEYS_UNIQUE && !r->lds_bm) nseg = 1;   // rows in HBM: one workgroup per tile
-			u.seg_blocks = std::max<uint32_t>(1, (nb + nseg - 1) / nseg);
-			u.nseg = nb ? (nb + u.seg_blocks - 1) / u.seg_blocks : 1;
+			u.qbase = pbase[p] + (uint64_t)b * bq;
+			plain_segments(r, u);
 			units.push_back(u);
 		}
 	}
+	return run_plain(r, nq, r->d_list, units);
+}
+
+int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit> & units) {
+	hipStream_t s = r->s;
+	const int mode = r->cfg.mode;
+	const uint32_t bq = r->plain_bq;
+	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	const tm_fib * f = r->fib;
 	const uint32_t nu = (uint32_t)units.size();
 	// per-unit work queues, initialised in one launch
 	TM_CHECK(grow_u32(&r->d_uitems, r->uitems_cap, std::max<uint32_t>(nu, 1)));
@@ -1195,7 +1261,8 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 	if (r->cfg.log_units) {
 		// one log row per launch: (partition, batch, queries in the batch)
 		std::vector<uint4> ud(nu);
-		for (uint32_t i = 0; i < nu; ++i) ud[i] = make_uint4(units[i].p, units[i].b, units[i].nqv, 0);
+		for (uint32_t i = 0; i < nu; ++i)
+			ud[i] = make_uint4(units[i].p, units[i].b, units[i].nqv, (uint32_t)units[i].qbase);
 		const uint32_t nl = (uint32_t)std::min<uint64_t>(nu, r->log_cap);
 		if (nl > r->units_cap) {
 			dfree(r->d_units);
@@ -1208,7 +1275,7 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 		if (nl) {
 			TM_HIP(hipMemcpyAsync(r->d_units, ud.data(), sizeof(uint4) * nl, hipMemcpyHostToDevice, s));
 			fill_u32(r->d_log, 0, (uint64_t)nl * 3, s);
-			launch_unit_features_plain(f->view(), r->d_q, r->d_list, r->d_pbase, r->d_units, nl, bq, r->d_ufeat, s);
+			launch_unit_features_plain(f->view(), r->d_q, list, r->d_units, nl, r->d_ufeat, s);
 			TM_HIP(hipStreamSynchronize(s));   // ud is pageable host memory
 		}
 		r->n_units = nl;
@@ -1234,8 +1301,8 @@ int enqueue_plain(tm_ring * r, uint32_t nq) {
 	mq_fn fn = pick_match_q(mode_of(r->cfg), r->lds_bm, r->tiles, r->waves, r->variant);
 	TM_HIP(hipEventRecord(r->em0, s));
 	for (uint32_t i = 0; i < nu; ++i) {
-		const unit & u = units[i];
-		a.qmap = r->d_list + pbase[u.p] + (uint64_t)u.b * bq;
+		const punit & u = units[i];
+		a.qmap = list + u.qbase;
 		a.nq = u.nqv;
 		a.gbm = r->d_stage;
 		a.blk_lo = u.blo;
@@ -1305,8 +1372,127 @@ int ensure_capacity(tm_ring * r) {
 	return TM_OK;
 }
 
+__global__ void __launch_bounds__(256) k_iota(uint32_t * __restrict__ p, uint32_t n) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i < n) p[i] = i;
+}
+
+// Explicit units (tm_submit_units): queries [u_off[u], u_off[u+1]) against
+// partition u_part[u] only, without the lookup.  PLAIN: one launch per
+// (unit, batch); PERSISTENT: one work-stealing launch over every unit's
+// (segment, tile) items.
+int enqueue_units(tm_ring * r, uint32_t nq) {
+	hipStream_t s = r->s;
+	const int mode = r->cfg.mode;
+	const tm_fib * f = r->fib;
+	const uint32_t n = (uint32_t)r->u_part.size();
+	if (mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
+	fill_u32(r->d_pcount, 0, r->P, s);
+	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 8, s);
+	scan_u32_to_u64(r->d_pcount, r->d_pbase, r->P, r->d_scan, s);
+	if (r->cfg.schedule == TM_SCHED_PLAIN) {
+		if (!r->d_iota) {
+			TM_CHECK(dalloc(&r->d_iota, std::max<uint32_t>(r->max_q, 1)));
+			hipLaunchKernelGGL(k_iota, dim3((r->max_q + 255) / 256), dim3(256), 0, s, r->d_iota, r->max_q);
+		}
+		const uint32_t bq = r->plain_bq;
+		std::vector<punit> units;
+		for (uint32_t i = 0; i < n; ++i) {
+			const uint32_t p = r->u_part[i], q0 = r->u_off[i], qn = r->u_off[i + 1] - q0;
+			const uint32_t blo = f->h_part_info[(size_t)p * 8 + 6], nb = f->h_part_info[(size_t)p * 8 + 7];
+			if (nb == 0) continue;
+			for (uint32_t b = 0; b * bq < qn; ++b) {
+				punit u;
+				u.p = p;
+				u.b = b;
+				u.nqv = std::min(bq, qn - b * bq);
+				u.blo = blo;
+				u.bhi = blo + nb;
+				u.qbase = (uint64_t)q0 + (uint64_t)b * bq;
+				plain_segments(r, u);
+				units.push_back(u);
+			}
+		}
+		return run_plain(r, nq, r->d_iota, units);
+	}
+	// persistent: segment size from the whole submission's work
+	const uint32_t nqg = 64 * r->tiles;
+	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	uint64_t work = 0;   // tile x blocks
+	for (uint32_t i = 0; i < n; ++i) {
+		const uint32_t p = r->u_part[i], qn = r->u_off[i + 1] - r->u_off[i];
+		work += (uint64_t)((qn + nqg - 1) / nqg) * f->h_part_info[(size_t)p * 8 + 7];
+	}
+	uint32_t seg = r->cfg.chunk_blocks ? r->cfg.chunk_blocks
+									   : (uint32_t)std::max<uint64_t>(4, (work + 4 * resident - 1) / (4 * resident));
+	if (mode == TM_MODE_KEYS_UNIQUE && !r->lds_bm) seg = 0xFFFFFFFFu;   // rows in HBM: one workgroup per tile
+	std::vector<uint32_t> tab((size_t)std::max<uint32_t>(n, 1) * 8, 0), pre(n + 1, 0);
+	uint32_t max_nseg = 1;
+	uint64_t items = 0;
+	for (uint32_t i = 0; i < n; ++i) {
+		const uint32_t p = r->u_part[i], q0 = r->u_off[i], qn = r->u_off[i + 1] - q0;
+		const uint32_t blo = f->h_part_info[(size_t)p * 8 + 6], nb = f->h_part_info[(size_t)p * 8 + 7];
+		const uint32_t ntiles = (qn + nqg - 1) / nqg;
+		const uint32_t sb = std::max<uint32_t>(1, std::min<uint32_t>(seg, std::max<uint32_t>(nb, 1)));
+		const uint32_t nseg = nb ? (nb + sb - 1) / sb : 1;
+		uint32_t * t = &tab[(size_t)i * 8];
+		t[0] = p; t[1] = q0; t[2] = qn; t[3] = std::max<uint32_t>(ntiles, 1);
+		t[4] = nseg; t[5] = sb; t[6] = blo; t[7] = blo + nb;
+		pre[i] = (uint32_t)items;
+		items += ntiles ? (uint64_t)ntiles * nseg : 0;
+		if (ntiles == 0) t[3] = 1;   // no items: the value only avoids a division
+		max_nseg = std::max(max_nseg, nseg);
+	}
+	if (items > 0xFFFFFFF0ull) return set_error(TM_ERR_INVALID, "too many work items");
+	pre[n] = (uint32_t)items;
+	TM_CHECK(grow_u32(&r->d_xunits, r->xunits_cap, tab.size()));
+	TM_CHECK(grow_u32(&r->d_xprefix, r->xprefix_cap, pre.size()));
+	TM_HIP(hipMemcpyAsync(r->d_xunits, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
+	TM_HIP(hipMemcpyAsync(r->d_xprefix, pre.data(), pre.size() * 4, hipMemcpyHostToDevice, s));
+	hipLaunchKernelGGL(k_init_queues, dim3(1), dim3(64), 0, s, items, r->d_qbeg, r->d_qnext);
+	mq_args a;
+	std::memset(&a, 0, sizeof(a));
+	a.f = f->view();
+	a.q = r->d_q;
+	a.nq = nq;
+	a.units = r->d_xunits;
+	a.uprefix = r->d_xprefix;
+	a.n_units = n;
+	a.ntiles = 1;
+	a.nseg = max_nseg;
+	a.seg_blocks = 1;
+	a.blk_lo = 0;
+	a.blk_hi = f->n_blocks;
+	a.log_unit = -1;
+	a.qnext = r->d_qnext;
+	a.qbeg = r->d_qbeg;
+	a.gbm = r->d_gbm;
+	a.hits = r->d_hits;
+	a.hit_cap = r->hit_cap;
+	a.counters = r->d_cnt;
+	a.full_exit = mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
+	a.last_mask = (f->n_keys & 31) ? (1u << (f->n_keys & 31)) - 1u : 0xFFFFFFFFu;
+	mq_fn fn = pick_match_q(mode_of(r->cfg), r->lds_bm, r->tiles, r->waves, r->variant);
+	const uint32_t grid = (uint32_t)std::min<uint64_t>(resident, std::max<uint64_t>(items, 1));
+	TM_HIP(hipEventRecord(r->em0, s));
+	hipLaunchKernelGGL(fn, dim3(grid), dim3(r->waves * 64), r->stage_lds, s, a);
+	TM_HIP(hipGetLastError());
+	TM_HIP(hipEventRecord(r->em1, s));
+	TM_HIP(hipStreamSynchronize(s));   // tab and pre are pageable host memory
+	fill_u64(r->d_iprefix + r->P, items, 1, s);
+	enqueue_finalize(r, nq);
+	return TM_OK;
+}
+
 int enqueue_step(tm_ring * r, uint32_t nq, bool allow_graph) {
 	TM_HIP(hipEventRecord(r->e0, r->s));
+	if (r->units_mode) {
+		TM_CHECK(enqueue_units(r, nq));
+		r->match_events = true;
+		TM_HIP(hipMemcpyAsync(r->h_stat, r->d_stat, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, r->s));
+		TM_HIP(hipEventRecord(r->e1, r->s));
+		return TM_OK;
+	}
 	const bool persistent = r->cfg.schedule == TM_SCHED_PERSISTENT;
 	if (persistent && allow_graph && r->cfg.use_graph) {
 		if (!r->graph_ok || r->graph_nq != nq) {
@@ -1405,10 +1591,13 @@ int price_submission(tm_ring * r, uint32_t nq) {
 	}
 	r->st.pred_hits = total;
 
-	// ordering pays when tiles outnumber the resident workgroups; smaller
-	// submissions are cut into block segments instead (enqueue_persistent)
+	// ordering pays when tiles outnumber the resident workgroups and the
+	// step is not cut into block segments (small submissions and tables
+	// above SEG_MAX_BLOCKS are, in enqueue_persistent)
 	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
-	if ((r->model_flags & TM_COST_ORDER) && n_tiles >= 2 * resident) {
+	const uint32_t nblk = r->fib->n_blocks;
+	const bool one_segment = nblk <= SEG_MAX_BLOCKS || r->cfg.chunk_blocks >= nblk;
+	if ((r->model_flags & TM_COST_ORDER) && n_tiles >= 2 * resident && one_segment) {
 		std::vector<uint32_t> idx(n_tiles);
 		std::iota(idx.begin(), idx.end(), 0u);
 		std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
@@ -1503,7 +1692,7 @@ int submit_common(tm_ring * r, uint32_t nq) {
 	r->retries = 0;
 	TM_CHECK(sync_fib(r));
 	TM_CHECK(ensure_capacity(r));
-	TM_CHECK(price_submission(r, nq));
+	if (!r->units_mode) TM_CHECK(price_submission(r, nq));
 	int rc = enqueue_step(r, nq, true);
 	if (rc != TM_OK) return rc;
 	r->pending = true;
@@ -1597,7 +1786,43 @@ int tm_submit(tm_ring * r, const uint32_t * words, uint32_t nq) {
 		r->h_q[(size_t)i * 8 + 7] = 0;
 	}
 	if (nq) TM_HIP(hipMemcpyAsync(r->d_q, r->h_q, (size_t)nq * 32, hipMemcpyHostToDevice, r->s));
+	r->units_mode = false;
 	return submit_common(r, nq);
+}
+
+int tm_submit_units(tm_ring * r, const uint32_t * words, uint32_t nq, const uint32_t * unit_part,
+					const uint32_t * unit_off, uint32_t n_units) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (nq > r->max_q) return set_error(TM_ERR_INVALID, "nq %u exceeds ring capacity %u", nq, r->max_q);
+	if (n_units && (!unit_part || !unit_off)) return set_error(TM_ERR_INVALID, "unit arrays are NULL");
+	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
+	if (n_units == 0 && nq) return set_error(TM_ERR_INVALID, "queries without units");
+	if (n_units) {
+		if (unit_off[0] != 0 || unit_off[n_units] != nq)
+			return set_error(TM_ERR_INVALID, "unit offsets must run from 0 to nq (%u)", nq);
+		for (uint32_t i = 0; i < n_units; ++i) {
+			if (unit_off[i + 1] < unit_off[i]) return set_error(TM_ERR_INVALID, "unit offsets not monotone at %u", i);
+			if (unit_part[i] >= r->fib->n_parts)
+				return set_error(TM_ERR_INVALID, "unit %u: partition %u of %u", i, unit_part[i], r->fib->n_parts);
+		}
+	}
+	TM_CHECK(require_device());
+	if (words && nq) {
+		for (uint32_t i = 0; i < nq; ++i) {
+			std::memcpy(r->h_q + (size_t)i * 8, words + (size_t)i * 6, 24);
+			r->h_q[(size_t)i * 8 + 6] = 0;
+			r->h_q[(size_t)i * 8 + 7] = 0;
+		}
+		TM_HIP(hipMemcpyAsync(r->d_q, r->h_q, (size_t)nq * 32, hipMemcpyHostToDevice, r->s));
+	}
+	r->u_part.assign(unit_part, unit_part + n_units);
+	r->u_off.assign(unit_off, unit_off + n_units + 1);
+	if (n_units == 0) r->u_off.assign(1, 0);
+	r->units_mode = true;
+	r->have_order = false;
+	int rc = submit_common(r, nq);
+	r->units_mode = rc == TM_OK;   // kept for the overflow retry in tm_wait
+	return rc;
 }
 
 int tm_submit_resident(tm_ring * r, uint32_t nq) {
@@ -1605,6 +1830,7 @@ int tm_submit_resident(tm_ring * r, uint32_t nq) {
 	if (nq > r->max_q) return set_error(TM_ERR_INVALID, "nq %u exceeds ring capacity %u", nq, r->max_q);
 	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
 	TM_CHECK(require_device());
+	r->units_mode = false;
 	return submit_common(r, nq);
 }
 

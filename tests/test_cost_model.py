@@ -230,7 +230,8 @@ def test_ordered_schedule_matches_oracle(trained):
     ob, vb = base.results()
     assert np.array_equal(ob[:n_check + 1], eo) and np.array_equal(vb[:int(eo[-1])], ev)
     for graph in (0, 1):
-        ring = tm.Ring(d["fib"], len(q), batch_size=64, use_graph=graph)
+        # one block segment per tile: the schedule the model orders
+        ring = tm.Ring(d["fib"], len(q), batch_size=64, use_graph=graph, chunk_blocks=1 << 30)
         ring.set_cost_model(d["m"], tm.COST_ORDER)
         for rep in range(3):
             ring.submit(q)
