@@ -149,6 +149,17 @@ int tm_fib_set_reserve(tm_fib * fib, uint32_t percent);
 int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
 				  const uint32_t * keys, const uint32_t * filter_ids, uint32_t maxp);
 
+/* Insert while submissions are in flight.  Only the fast case: every
+ * filter takes a spare lane of an existing partition and every key is
+ * already in the table (else TM_ERR_BUSY, nothing changed: collect the
+ * submissions and call tm_fib_insert).  Each insert starts a new epoch; a
+ * submission sees exactly the inserts that completed before it was made,
+ * in flight or not: lanes carry their epoch and the match skips later ones.
+ * Not to be called concurrently with tm_fib_insert / tm_fib_upload on the
+ * same FIB, nor with a submission being made on it. */
+int tm_fib_insert_live(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+					   const uint32_t * keys, const uint32_t * filter_ids);
+
 /* ------------------------------------------------------------------------ */
 /* Matching.
  *
@@ -184,7 +195,7 @@ typedef struct tm_match_config {
 	                            PERSISTENT: queries per work item (64, 128
 	                            or 256), 0 = 64                              */
 	uint32_t chunk_blocks;   /* PLAIN: least blocks per segment of a unit
-	                            (0 = 4); PERSISTENT: blocks per segment of
+	                            (0 = 64); PERSISTENT: blocks per segment of
 	                            the table (0 = automatic: at most 1024, and
 	                            smaller for submissions too small to fill
 	                            the GPU)                                     */
@@ -358,9 +369,11 @@ int tm_ring_set_cost_model(tm_ring * ring, tm_mlp * model, uint32_t flags);
  * bytes fit the HBM budget, so steps of several slots overlap on the
  * device.  A request is never split: its queries run in one device step,
  * possibly together with other requests.  Inserts queue in the same order
- * as requests and are applied when every earlier request has completed
- * (tm_fib_insert), so each request sees exactly the inserts submitted
- * before it.  No counterpart in the reference for the budget or inserts:
+ * as requests.  One that fits spare lanes of existing partitions applies at
+ * once (tm_fib_insert_live): the steps in flight do not see it, the
+ * requests behind it do.  Any other waits until every earlier request has
+ * completed (tm_fib_insert).  Either way each request sees exactly the
+ * inserts submitted before it.  No counterpart in the reference for the budget or inserts:
  * its add/consolidate run only while the engine is stopped. */
 typedef struct tm_server tm_server;
 
@@ -381,6 +394,7 @@ typedef struct tm_server_stats {
 	uint64_t n_steps;             /* device steps completed without error    */
 	uint64_t n_retries;           /* steps that needed an overflow retry     */
 	uint64_t n_inserts;
+	uint64_t n_live_inserts;      /* inserts applied beside the steps in flight */
 	uint64_t n_budget_waits;      /* admissions deferred by the HBM budget   */
 	uint64_t requests_per_batch_sum;
 	uint64_t reserved_peak;       /* largest budget use, bytes               */

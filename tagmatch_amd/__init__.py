@@ -85,6 +85,7 @@ class ServerConfig(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in (
         "n_requests", "n_requests_done", "n_queries_done", "n_batches", "n_steps", "n_retries", "n_inserts",
+        "n_live_inserts",
         "n_budget_waits", "requests_per_batch_sum", "reserved_peak", "reserved_now", "hbm_budget", "n_waiting")] + [
         ("latency_sum_ms", C.c_double), ("latency_max_ms", C.c_double), ("device_ms_sum", C.c_double)]
 
@@ -115,6 +116,7 @@ _SIGS = {
     "tm_fib_stats_get": (C.c_int, [P, C.POINTER(FibStats)]),
     "tm_fib_set_reserve": (C.c_int, [P, C.c_uint32]),
     "tm_fib_insert": (C.c_int, [P, P, C.c_uint32, P, P, P, C.c_uint32]),
+    "tm_fib_insert_live": (C.c_int, [P, P, C.c_uint32, P, P, P]),
     "tm_match_config_default": (None, [C.POINTER(MatchConfig)]),
     "tm_ring_create": (C.c_int, [P, C.c_uint32, C.POINTER(MatchConfig), C.POINTER(P)]),
     "tm_ring_destroy": (C.c_int, [P]),
@@ -351,6 +353,19 @@ class Fib:
         fid = None if filter_ids is None else np.ascontiguousarray(filter_ids, np.uint32)
         check(load().tm_fib_insert(self.h, _ptr(words), words.shape[0], _ptr(ko), _ptr(kv), _ptr(fid), maxp),
               "tm_fib_insert")
+
+    def insert_live(self, words, key_off, keys, filter_ids=None):
+        """Insert while submissions are in flight (tm_fib_insert_live);
+        returns False (nothing inserted) when the batch needs tm_fib_insert."""
+        words = np.ascontiguousarray(words, np.uint32).reshape(-1, 6)
+        ko = np.ascontiguousarray(key_off, np.uint32)
+        kv = np.ascontiguousarray(keys, np.uint32)
+        fid = None if filter_ids is None else np.ascontiguousarray(filter_ids, np.uint32)
+        rc = load().tm_fib_insert_live(self.h, _ptr(words), words.shape[0], _ptr(ko), _ptr(kv), _ptr(fid))
+        if rc == TM_ERR_BUSY:
+            return False
+        check(rc, "tm_fib_insert_live")
+        return True
 
     def stats(self):
         s = FibStats()

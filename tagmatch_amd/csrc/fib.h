@@ -70,6 +70,13 @@ struct tm_fib {
 	size_t ovf_cap = 0;                  // device words of ovf
 	uint32_t reserve_pct = 13;           // spare blocks per partition, percent (+1)
 	uint64_t version = 0;                // bumped when P, K or device arrays change
+	// live inserts: each lane's insert epoch; a submission sees the lanes
+	// whose epoch is at most the FIB's epoch when it was submitted
+	uint32_t * slot_epoch = nullptr;     // [nb][64], allocated by the first live insert
+	uint32_t epoch = 0;                  // inserts so far
+	hipStream_t ins_stream = nullptr;    // live inserts run here, beside the rings' streams
+	void * d_recs = nullptr;             // insert records staging
+	size_t recs_cap = 0;
 	std::atomic<int> in_flight{0};       // ring submissions not yet waited for
 
 	tmx::dev_fib_view view() const {

@@ -31,6 +31,8 @@ void tm_fib::release() {
 	dfree(key_dict);
 	dfree(part_info);
 	dfree(part_stat);
+	dfree(slot_epoch);
+	epoch = 0;
 	n_parts = n_blocks = n_keys = 0;
 	n_filters = n_key_instances = device_bytes = max_part = 0;
 	ovf_cap = 0;
@@ -234,10 +236,14 @@ struct ins_rec {
 // AND mask and valid-lane count.
 __global__ void __launch_bounds__(256)
 k_insert_lanes(const ins_rec * __restrict__ recs, uint32_t n, uint32_t * __restrict__ blk_words,
-			   uint32_t * __restrict__ blk_info, uint64_t * __restrict__ slot_keys, uint32_t * __restrict__ slot_fid) {
+			   uint32_t * __restrict__ blk_info, uint64_t * __restrict__ slot_keys, uint32_t * __restrict__ slot_fid,
+			   uint32_t * __restrict__ slot_epoch, uint32_t epoch) {
 	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
 	if (i >= n) return;
 	const ins_rec r = recs[i];
+	// a submission older than this insert sees the lane with its old epoch
+	// (none yet: invisible) or the new one (newer than it: invisible)
+	if (slot_epoch) slot_epoch[(size_t)r.block * 64 + r.lane] = epoch;
 	for (int j = 0; j < 6; ++j) {
 		blk_words[(size_t)r.block * BLOCK_WORDS + j * 64 + r.lane] = r.w[j];
 		atomicAnd(&blk_info[(size_t)r.block * 8 + j], r.w[j]);
@@ -245,6 +251,17 @@ k_insert_lanes(const ins_rec * __restrict__ recs, uint32_t n, uint32_t * __restr
 	atomicMax(&blk_info[(size_t)r.block * 8 + 6], r.lane + 1);
 	slot_keys[(size_t)r.block * 64 + r.lane] = r.kv;
 	slot_fid[(size_t)r.block * 64 + r.lane] = r.fid;
+}
+
+// Lane epochs of blocks [b0, nb): lanes already valid get `visible`, the
+// others (spare) the never-visible maximum.
+__global__ void __launch_bounds__(256)
+k_init_epochs(const uint32_t * __restrict__ blk_info, uint32_t b0, uint32_t nb, uint32_t * __restrict__ slot_epoch,
+			  uint32_t visible) {
+	const uint64_t i = (uint64_t)b0 * 64 + (uint64_t)blockIdx.x * 256 + threadIdx.x;
+	if (i >= (uint64_t)nb * 64) return;
+	const uint32_t b = (uint32_t)(i >> 6), lane = (uint32_t)(i & 63);
+	slot_epoch[i] = lane < blk_info[(size_t)b * 8 + 6] ? visible : 0xFFFFFFFFu;
 }
 
 // Dense ids of every slot through `map` (old id -> new id) after the key
@@ -276,7 +293,10 @@ int tm_fib_create(tm_fib ** out) {
 int tm_fib_destroy(tm_fib * fib) {
 	if (!fib) return TM_OK;
 	if (dev().device >= 0) (void)hipSetDevice(dev().device);
+	if (fib->ins_stream) (void)hipStreamSynchronize(fib->ins_stream);
 	fib->release();
+	if (fib->d_recs) (void)hipFree(fib->d_recs);
+	if (fib->ins_stream) (void)hipStreamDestroy(fib->ins_stream);
 	delete fib;
 	return TM_OK;
 }
@@ -405,6 +425,7 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 	if (fib->h_order.size() != 192) return set_error(TM_ERR_STATE, "no table uploaded");
 	TM_CHECK(require_device());
 	bool bump = false;   // device arrays, P or K change: rings must re-read the FIB
+	const uint32_t e = fib->epoch + 1;
 
 	// 1. keys new to the dictionary: grow it and remap dense ids
 	const uint64_t nkeys = key_offsets[n];
@@ -537,10 +558,17 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 		TM_CHECK(regrow(&fib->blk_info, ob * 8, nbt * 8, s));
 		TM_CHECK(regrow(&fib->slot_keys, ob * 64, nbt * 64, s));
 		TM_CHECK(regrow(&fib->slot_fid, ob * 64, nbt * 64, s));
+		if (fib->slot_epoch) TM_CHECK(regrow(&fib->slot_epoch, ob * 64, nbt * 64, s));
 		TM_HIP(hipMemcpy(fib->blk_words + ob * BLOCK_WORDS, bb.words.data(), bb.words.size() * 4, hipMemcpyHostToDevice));
 		TM_HIP(hipMemcpy(fib->blk_info + ob * 8, bb.info.data(), bb.info.size() * 4, hipMemcpyHostToDevice));
 		TM_HIP(hipMemcpy(fib->slot_keys + ob * 64, bb.keys.data(), bb.keys.size() * 8, hipMemcpyHostToDevice));
 		TM_HIP(hipMemcpy(fib->slot_fid + ob * 64, bb.fid.data(), bb.fid.size() * 4, hipMemcpyHostToDevice));
+		if (fib->slot_epoch) {
+			hipLaunchKernelGGL(k_init_epochs, dim3((uint32_t)((bb.nb * 64ull + 255) / 256)), dim3(256), 0, 0,
+							   fib->blk_info, (uint32_t)ob, (uint32_t)nbt, fib->slot_epoch, e);
+			TM_HIP(hipGetLastError());
+			TM_HIP(hipDeviceSynchronize());
+		}
 		fib->device_bytes += (uint64_t)bb.nb * (BLOCK_WORDS * 4 + 32 + 64 * 12);
 		fib->n_blocks = (uint32_t)nbt;
 		fib->h_part_info.insert(fib->h_part_info.end(), bb.part.begin(), bb.part.end());
@@ -580,7 +608,8 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 		int rc = put(d_recs, recs);
 		if (rc == TM_OK) {
 			hipLaunchKernelGGL(k_insert_lanes, dim3((uint32_t)((recs.size() + 255) / 256)), dim3(256), 0, 0, d_recs,
-							   (uint32_t)recs.size(), fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid);
+							   (uint32_t)recs.size(), fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid,
+							   fib->slot_epoch, e);
 			hipError_t e = hipGetLastError();
 			if (e == hipSuccess) e = hipDeviceSynchronize();
 			if (e != hipSuccess) rc = hip_error(e, "insert", __FILE__, __LINE__);
@@ -592,7 +621,112 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 	}
 	fib->n_filters += n;
 	fib->n_key_instances += nkeys;
+	fib->epoch = e;
 	if (bump) fib->version++;
+	return TM_OK;
+}
+
+int tm_fib_insert_live(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+					   const uint32_t * keys, const uint32_t * filter_ids) {
+	if (!fib) return set_error(TM_ERR_INVALID, "fib is NULL");
+	TM_CHECK(check_table(words, n, key_offsets, keys));
+	if (n == 0) return TM_OK;
+	if (fib->h_order.size() != 192) return set_error(TM_ERR_STATE, "no table uploaded");
+	TM_CHECK(require_device());
+	// 1. dry run: known keys, room in the overflow lists, a spare lane for
+	//    every filter; nothing changes unless all of them hold
+	const uint64_t nkeys = key_offsets[n];
+	for (uint64_t x = 0; x < nkeys; ++x)
+		if (!std::binary_search(fib->h_dict.begin(), fib->h_dict.end(), keys[x]))
+			return set_error(TM_ERR_BUSY, "key %u is new to the table: use tm_fib_insert", keys[x]);
+	const bool inline_ok = fib->n_keys < KEY_NONE;
+	size_t ovf_need = 0;
+	for (uint32_t i = 0; i < n; ++i) {
+		const uint32_t kn = key_offsets[i + 1] - key_offsets[i];
+		if (!inline_ok || kn > 4) ovf_need += 1 + kn;
+	}
+	if (fib->h_ovf.size() + ovf_need > fib->ovf_cap)
+		return set_error(TM_ERR_BUSY, "overflow key lists are full: use tm_fib_insert");
+	const uint32_t P = fib->n_parts;
+	std::vector<uint32_t> pmpop(P);
+	for (uint32_t p = 0; p < P; ++p) {
+		uint32_t c = 0;
+		for (int j = 0; j < 6; ++j) c += __builtin_popcount(fib->h_part_info[(size_t)p * 8 + j]);
+		pmpop[p] = c;
+	}
+	std::vector<uint32_t> size = fib->h_part_size, target(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		const uint32_t * w = words + (size_t)i * 6;
+		int64_t best = -1;
+		for (uint32_t p = 0; p < P; ++p) {
+			const uint32_t * m = &fib->h_part_info[(size_t)p * 8];
+			if ((m[0] & ~w[0]) | (m[1] & ~w[1]) | (m[2] & ~w[2]) | (m[3] & ~w[3]) | (m[4] & ~w[4]) | (m[5] & ~w[5]))
+				continue;
+			if (size[p] >= (uint64_t)fib->h_part_cap[p] * 64) continue;
+			if (best < 0 || pmpop[p] > pmpop[best] || (pmpop[p] == pmpop[best] && size[p] < size[best])) best = p;
+		}
+		if (best < 0) return set_error(TM_ERR_BUSY, "filter %u fits no spare lane: use tm_fib_insert", i);
+		target[i] = (uint32_t)best;
+		size[best]++;
+	}
+	// 2. device state for epochs and the insert stream
+	if (!fib->ins_stream) TM_HIP(hipStreamCreateWithFlags(&fib->ins_stream, hipStreamNonBlocking));
+	hipStream_t s = fib->ins_stream;
+	if (!fib->slot_epoch) {
+		TM_CHECK(dalloc(&fib->slot_epoch, (size_t)std::max<uint32_t>(fib->n_blocks, 1) * 64));
+		hipLaunchKernelGGL(k_init_epochs, dim3((uint32_t)((fib->n_blocks * 64ull + 255) / 256)), dim3(256), 0, s,
+						   fib->blk_info, 0u, fib->n_blocks, fib->slot_epoch, 0u);
+		TM_HIP(hipGetLastError());
+	}
+	if (fib->recs_cap < n) {
+		if (fib->d_recs) {
+			TM_HIP(hipStreamSynchronize(s));
+			(void)hipFree(fib->d_recs);
+			fib->d_recs = nullptr;
+			fib->recs_cap = 0;
+		}
+		TM_HIP(hipMalloc(&fib->d_recs, sizeof(ins_rec) * (size_t)n * 2));
+		fib->recs_cap = (size_t)n * 2;
+	}
+	// 3. commit: lanes, host mirrors, then the device copies
+	const uint32_t e = fib->epoch + 1;
+	const size_t ovf_before = fib->h_ovf.size();
+	std::vector<ins_rec> recs(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		const uint32_t p = target[i];
+		const uint32_t * w = words + (size_t)i * 6;
+		const uint32_t k = fib->h_part_size[p]++;
+		ins_rec & r = recs[i];
+		r.block = fib->h_part_info[(size_t)p * 8 + 6] + k / 64;
+		r.lane = k % 64;
+		r.fid = filter_ids ? filter_ids[i] : (uint32_t)(fib->n_filters + i);
+		r.pad = 0;
+		for (int j = 0; j < 6; ++j) r.w[j] = w[j];
+		TM_CHECK(key_slot(keys, key_offsets[i], key_offsets[i + 1], fib->h_dict, inline_ok, fib->h_ovf, r.kv));
+		fib->h_part_info[(size_t)p * 8 + 7] = (fib->h_part_size[p] + 63) / 64;
+		uint32_t * ps = &fib->h_part_stat[(size_t)p * 8];
+		for (int j = 0; j < 6; ++j) {
+			ps[j] |= w[j];
+			ps[7] += (uint32_t)__builtin_popcount(w[j]);
+		}
+		ps[6] += 1;
+		fib->max_part = std::max<uint64_t>(fib->max_part, fib->h_part_size[p]);
+	}
+	if (fib->h_ovf.size() > ovf_before)
+		TM_HIP(hipMemcpyAsync(fib->ovf + ovf_before, fib->h_ovf.data() + ovf_before,
+							  (fib->h_ovf.size() - ovf_before) * 4, hipMemcpyHostToDevice, s));
+	TM_HIP(hipMemcpyAsync(fib->d_recs, recs.data(), sizeof(ins_rec) * n, hipMemcpyHostToDevice, s));
+	hipLaunchKernelGGL(k_insert_lanes, dim3((n + 255) / 256), dim3(256), 0, s, (const ins_rec *)fib->d_recs, n,
+					   fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid, fib->slot_epoch, e);
+	TM_HIP(hipGetLastError());
+	// block counts and statistics after the lanes: a submission that sees a
+	// new block count sees lanes that its epoch hides
+	TM_HIP(hipMemcpyAsync(fib->part_info, fib->h_part_info.data(), (size_t)P * 32, hipMemcpyHostToDevice, s));
+	TM_HIP(hipMemcpyAsync(fib->part_stat, fib->h_part_stat.data(), (size_t)P * 32, hipMemcpyHostToDevice, s));
+	TM_HIP(hipStreamSynchronize(s));
+	fib->n_filters += n;
+	fib->n_key_instances += nkeys;
+	fib->epoch = e;
 	return TM_OK;
 }
 

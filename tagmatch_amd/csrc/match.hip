@@ -208,6 +208,8 @@ struct mq_args {
 	                                          // [3] subset tests, [4] tests with hits
 	uint32_t * log;                           // per item: hits, 0, 10 ns ticks
 	uint64_t log_cap;
+	const uint32_t * __restrict__ slot_epoch; // lane insert epochs (NULL: every valid lane counts)
+	uint32_t epoch;                           // lanes with a later epoch are not seen
 	uint32_t full_exit;                       // KEYS_UNIQUE + LDS bitmaps: skip queries whose
 	                                          // bitmap holds every key
 	uint32_t last_mask;                       // valid bits of a bitmap row's last word
@@ -568,7 +570,8 @@ k_match_q(mq_args a) {
 				}
 				const uint32_t f0 = c_f[0], f1 = c_f[1], f2 = c_f[2], f3 = c_f[3], f4 = c_f[4], f5 = c_f[5];
 				const uint64_t kv = c_kv;
-				const uint64_t lmask = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
+				uint64_t lmask = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
+				if (a.slot_epoch) lmask &= __ballot(a.slot_epoch[(size_t)b * 64 + lane] <= a.epoch);
 
 #pragma unroll
 				for (int t = 0; t < TILES; ++t) {
@@ -960,6 +963,9 @@ struct tm_ring {
 	uint32_t * d_iota = nullptr;       // 0 .. max_q - 1 (plain units' query lists)
 	uint64_t xunits_cap = 0, xprefix_cap = 0;
 	uint32_t last_nseg = 1;    // block segments per tile of the last persistent step
+	uint32_t epoch = 0;        // FIB insert epoch the submission sees
+	uint32_t graph_epoch = 0;
+	const uint32_t * graph_slot_epoch = nullptr;
 	bool pending = false;
 	bool match_events = false;
 	int retries = 0;
@@ -1137,6 +1143,8 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.iorder = r->use_iorder ? r->d_iorder : nullptr;
 	a.full_exit = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
 	a.last_mask = (r->fib->n_keys & 31) ? (1u << (r->fib->n_keys & 31)) - 1u : 0xFFFFFFFFu;
+	a.slot_epoch = r->fib->slot_epoch;
+	a.epoch = r->epoch;
 	a.ntiles = (uint32_t)tiles;
 	a.nseg = nseg;
 	a.seg_blocks = seg_blocks;
@@ -1185,8 +1193,9 @@ void plain_segments(const tm_ring * r, punit & u) {
 	const uint32_t nb = u.bhi - u.blo;
 	u.ntiles = (u.nqv + nqg - 1) / nqg;
 	// segments: about two items per resident workgroup, at least
-	// r->cfg.chunk_blocks (or 4) blocks each
-	const uint32_t min_seg = r->cfg.chunk_blocks ? r->cfg.chunk_blocks : 4;
+	// r->cfg.chunk_blocks (or 64) blocks each: an item's fixed cost (its
+	// bitmap rows cleared and merged) is about that of 30 block visits
+	const uint32_t min_seg = r->cfg.chunk_blocks ? r->cfg.chunk_blocks : 64;
 	const uint64_t want = (2 * resident + std::max<uint32_t>(u.ntiles, 1) - 1) / std::max<uint32_t>(u.ntiles, 1);
 	uint32_t nseg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (nb + min_seg - 1) / min_seg));
 	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->lds_bm) nseg = 1;   // rows in HBM: one workgroup per tile
@@ -1297,6 +1306,8 @@ int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit
 	a.counters = r->d_cnt;
 	a.log_cap = r->log_cap;
 	a.full_exit = 0;   // the staging rows start empty for every unit: nothing to learn
+	a.slot_epoch = f->slot_epoch;
+	a.epoch = r->epoch;
 	a.last_mask = 0xFFFFFFFFu;
 	mq_fn fn = pick_match_q(mode_of(r->cfg), r->lds_bm, r->tiles, r->waves, r->variant);
 	TM_HIP(hipEventRecord(r->em0, s));
@@ -1457,6 +1468,8 @@ int enqueue_units(tm_ring * r, uint32_t nq) {
 	a.nq = nq;
 	a.units = r->d_xunits;
 	a.uprefix = r->d_xprefix;
+	a.slot_epoch = f->slot_epoch;
+	a.epoch = r->epoch;
 	a.n_units = n;
 	a.ntiles = 1;
 	a.nseg = max_nseg;
@@ -1495,7 +1508,8 @@ int enqueue_step(tm_ring * r, uint32_t nq, bool allow_graph) {
 	}
 	const bool persistent = r->cfg.schedule == TM_SCHED_PERSISTENT;
 	if (persistent && allow_graph && r->cfg.use_graph) {
-		if (!r->graph_ok || r->graph_nq != nq) {
+		if (!r->graph_ok || r->graph_nq != nq || r->graph_epoch != r->epoch ||
+			r->graph_slot_epoch != r->fib->slot_epoch) {
 			if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
 			r->gexec = nullptr;
 			r->graph_ok = false;
@@ -1513,6 +1527,8 @@ int enqueue_step(tm_ring * r, uint32_t nq, bool allow_graph) {
 			TM_HIP(ie);
 			r->graph_ok = true;
 			r->graph_nq = nq;
+			r->graph_epoch = r->epoch;
+			r->graph_slot_epoch = r->fib->slot_epoch;
 		}
 		TM_HIP(hipGraphLaunch(r->gexec, r->s));
 		r->match_events = false;
@@ -1690,6 +1706,7 @@ int submit_common(tm_ring * r, uint32_t nq) {
 	r->st = tm_status();
 	r->last_nq = nq;
 	r->retries = 0;
+	r->epoch = r->fib->epoch;   // the inserts this submission sees
 	TM_CHECK(sync_fib(r));
 	TM_CHECK(ensure_capacity(r));
 	if (!r->units_mode) TM_CHECK(price_submission(r, nq));

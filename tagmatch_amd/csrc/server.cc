@@ -45,6 +45,7 @@ struct request {
 	// insert payload
 	std::vector<uint32_t> ins_words, ins_koff, ins_keys, ins_fid;
 	uint32_t ins_n = 0, ins_maxp = 0;
+	bool ins_tried_live = false;      // the live insert declined: apply at a barrier
 	// results
 	std::vector<uint64_t> off;
 	std::vector<uint32_t> vals;
@@ -220,8 +221,30 @@ void engine(tm_server * s) {
 			s->st.requests_per_batch_sum += sl.reqs.size();
 			progressed = true;
 		}
-		// insert barrier: every earlier request has completed
-		if (!s->waiting.empty() && s->waiting.front()->is_insert) {
+		// inserts: live (beside the steps in flight, which do not see it) when
+		// the batch fits spare lanes, else at a barrier once every earlier
+		// request has completed
+		if (!s->waiting.empty() && s->waiting.front()->is_insert && !s->waiting.front()->ins_tried_live) {
+			request * r = s->waiting.front();
+			lk.unlock();
+			int rc = tm_fib_insert_live(s->fib, r->ins_words.data(), r->ins_n, r->ins_koff.data(),
+										r->ins_keys.data(), r->ins_fid.empty() ? nullptr : r->ins_fid.data());
+			const std::string err = (rc == TM_OK || rc == TM_ERR_BUSY) ? std::string() : std::string(tm_last_error());
+			lk.lock();
+			if (rc == TM_ERR_BUSY) {
+				r->ins_tried_live = true;   // needs the barrier
+			} else {
+				s->waiting.pop_front();
+				r->status = rc;
+				r->error = err;
+				s->st.n_inserts++;
+				s->st.n_live_inserts += rc == TM_OK;
+				finish(s, r);
+				s->cv_done.notify_all();
+				progressed = true;
+			}
+		}
+		if (!s->waiting.empty() && s->waiting.front()->is_insert && s->waiting.front()->ins_tried_live) {
 			bool idle = true;
 			for (auto & sl : s->slots) idle = idle && !sl.busy;
 			if (idle) {

@@ -98,3 +98,59 @@ def test_ring_across_inserts_with_graph(gpu):
         fib.insert(bw, bko, bk, maxp=3000)
         w, ko, k = concat(w, ko, k, bw, bko, bk)
         log(f"step {step}: {fib.stats()['n_parts']} partitions")
+
+
+def test_live_insert_is_seen_only_by_later_submissions(gpu):
+    cw, ck, cv, q, _ = tm.workload(100000, 2000, seed=420)
+    fib, perm, off, masks = tm.build_fib(cw, ck, cv, 8000)
+    _, _, _, big, _ = tm.workload(100000, 200000, seed=421)
+    bw, bko, bk, _ = gen(3000, 1, seed=422)
+    w2, ko2, k2 = concat(cw, ck, cv, bw, bko, bk)
+    for mode, om in MODES:
+        eo1, ev1 = orc.match(cw, ck, cv, q, mode=om)
+        eo2, ev2 = orc.match(w2, ko2, k2, q, mode=om)
+        ring = tm.Ring(fib, len(big), mode=mode)
+        ring.submit(np.concatenate([q, big])[:len(big)])   # q first, then a long tail
+        # the insert lands while that submission runs; it must not see it
+        if mode == tm.MODE_KEYS_UNIQUE:
+            assert fib.insert_live(bw, bko, bk)
+            w_seen, ko_seen, k_seen = cw, ck, cv
+        else:
+            w_seen, ko_seen, k_seen = w2, ko2, k2          # inserted by the KEYS_UNIQUE round
+        ring.wait()
+        o, v = ring.results()
+        nq = len(q)
+        eo, ev = orc.match(w_seen, ko_seen, k_seen, q, mode=om)
+        assert np.array_equal(o[:nq + 1], eo), mode
+        for i in range(nq):
+            assert np.array_equal(np.sort(v[o[i]:o[i + 1]]), ev[eo[i]:eo[i + 1]]), (mode, i)
+        ring.close()
+        # every later submission sees it
+        off2, vals2, st = tm.match(fib, q, mode=mode)
+        assert np.array_equal(off2, eo2), mode
+        for i in range(len(q)):
+            assert np.array_equal(np.sort(vals2[off2[i]:off2[i + 1]]), ev2[eo2[i]:eo2[i + 1]]), (mode, i)
+
+
+def test_live_insert_declines_and_graph_follows_epochs(gpu):
+    cw, ck, cv, q, _ = tm.workload(30000, 500, seed=430)
+    fib, perm, off, masks = tm.build_fib(cw, ck, cv, 3000)
+    bw, bko, bk, _ = gen(200, 1, seed=431)
+    # a key the table does not hold: declined, nothing changes
+    bad = bk.copy()
+    bad[0] = 0xFFFFFFF0
+    n0 = fib.stats()["n_filters"]
+    assert not fib.insert_live(bw, bko, bad)
+    assert fib.stats()["n_filters"] == n0
+    ring = tm.Ring(fib, len(q), use_graph=1)
+    w, ko, k = cw, ck, cv
+    for step in range(3):
+        for rep in range(2):
+            ring.submit(q)
+            ring.wait()
+            o, v = ring.results()
+            eo, ev = orc.match(w, ko, k, q, mode=0)
+            assert np.array_equal(o, eo) and np.array_equal(v, ev), (step, rep)
+        bw, bko, bk, _ = gen(300, 1, seed=440 + step)
+        assert fib.insert_live(bw, bko, bk)
+        w, ko, k = concat(w, ko, k, bw, bko, bk)
