@@ -128,7 +128,10 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 int tm_fib_stats_get(const tm_fib * fib, tm_fib_stats * st);
 
 /* Spare blocks reserved per partition at upload for tm_fib_insert, in
- * percent of its used blocks (plus one block); default 13, 0 = none.
+ * percent of its used blocks (plus one block); default 13, 0 = none.  With a
+ * reserve, the upload also adds one empty partition with mask 0 (selected
+ * by every query, room for about 1% of the table) for inserted filters that
+ * contain no partition's mask; tm_fib_stats_get counts it in n_parts.
  * Applies to the next tm_fib_upload. */
 int tm_fib_set_reserve(tm_fib * fib, uint32_t percent);
 

@@ -94,7 +94,7 @@ def unit_features(mask, or_words, n_members, member_pop_sum, queries):
     f[11] = member_pop_sum / npart / 64.0 if npart > 0 else 0.0
     f[12] = popcount6(orw)[0] / 192.0
     B = q.shape[0]
-    if B:
+    if B and npart > 0:
         c = popcount6(q).astype(np.float64)
         o = popcount6(q & orw).astype(np.float64)
         f[2] = np.log2(1.0 + B) / 12.0

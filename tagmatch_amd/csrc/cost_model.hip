@@ -247,7 +247,7 @@ __device__ void unit_row(bool sel, uint32_t c, uint32_t o, const uint32_t * __re
 		f[1] = (float)mpop / 64.f;
 		f[11] = np > 0.f ? (float)pstat[7] / np / 64.f : 0.f;
 		f[12] = (float)opop / 192.f;
-		if (n) {
+		if (n && np > 0.f) {   // a partition without filters is no work
 			const float inv = 1.f / (float)n;
 			const float mean = (float)tsc * inv;
 			const float var = fmaxf((float)tsc2 * inv - mean * mean, 0.f);

@@ -34,6 +34,7 @@ struct dev_fib_view {
 	const uint64_t * __restrict__ slot_keys;
 	const uint32_t * __restrict__ slot_fid;
 	const uint32_t * __restrict__ ovf;
+	uint32_t ovf_n;                    // valid words of ovf when the view was made
 	const uint32_t * __restrict__ key_dict;
 	const uint32_t * __restrict__ part_info;
 	const uint32_t * __restrict__ part_stat;
@@ -86,6 +87,7 @@ struct tm_fib {
 		v.slot_keys = slot_keys;
 		v.slot_fid = slot_fid;
 		v.ovf = ovf;
+		v.ovf_n = (uint32_t)h_ovf.size();
 		v.key_dict = key_dict;
 		v.part_info = part_info;
 		v.part_stat = part_stat;

@@ -117,13 +117,15 @@ int build_blocks(const uint32_t * words, const uint32_t * part_offsets, const ui
 				 const uint32_t * key_offsets, const uint32_t * keys, const uint32_t * filter_ids, uint64_t fid_base,
 				 const std::vector<int> & order, const std::vector<uint32_t> & dict, bool inline_ok,
 				 std::vector<uint32_t> & ovf, uint32_t p_base, uint32_t b_base, uint32_t reserve_pct,
-				 block_build & bb) {
+				 block_build & bb, uint32_t last_cap = 0) {
 	const uint32_t n = part_offsets[n_parts];
 	std::vector<uint32_t> pblock(n_parts + 1, 0);
 	for (uint32_t p = 0; p < n_parts; ++p) {
 		const uint32_t sz = part_offsets[p + 1] - part_offsets[p];
 		const uint32_t used = (sz + 63) / 64;
-		pblock[p + 1] = pblock[p] + used + reserve_blocks(used, reserve_pct);
+		const uint32_t cap = (last_cap && p + 1 == n_parts) ? std::max(last_cap, used)
+															: used + reserve_blocks(used, reserve_pct);
+		pblock[p + 1] = pblock[p] + cap;
 	}
 	const uint32_t nb = pblock[n_parts];
 	std::vector<uint32_t> member(n);  // member[k] = input index at partitioned position k
@@ -375,12 +377,31 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 
 	std::vector<uint32_t> h_ovf(1, 0);   // overflow key lists (count, ids...)
 	block_build bb;
-	TM_CHECK(build_blocks(words, part_offsets, part_masks, n_parts, key_offsets, keys, filter_ids, 0, order, dict,
-						  inline_ok, h_ovf, 0, 0, fib->reserve_pct, bb));
+	// With a reserve, one more partition: empty, mask 0 (every query selects
+	// it), room for about 1% of the table.  Inserted filters that contain no
+	// partition's mask go there instead of needing a new partition, so that
+	// they can be inserted live.
+	const bool catch_all = fib->reserve_pct > 0;
+	std::vector<uint32_t> off2, masks2;
+	const uint32_t * po = part_offsets;
+	const uint32_t * pm = part_masks;
+	uint32_t np = n_parts, last_cap = 0;
+	if (catch_all) {
+		off2.assign(part_offsets, part_offsets + n_parts + 1);
+		off2.push_back(n);
+		masks2.assign(part_masks, part_masks + (size_t)n_parts * 6);
+		masks2.resize(masks2.size() + 6, 0u);
+		po = off2.data();
+		pm = masks2.data();
+		np = n_parts + 1;
+		last_cap = std::max<uint32_t>(16, (n / 64) / 100);
+	}
+	TM_CHECK(build_blocks(words, po, pm, np, key_offsets, keys, filter_ids, 0, order, dict, inline_ok, h_ovf, 0, 0,
+						  fib->reserve_pct, bb, last_cap));
 	uint64_t maxp = 0;
-	std::vector<uint32_t> h_psize(n_parts, 0);
-	for (uint32_t p = 0; p < n_parts; ++p) {
-		h_psize[p] = part_offsets[p + 1] - part_offsets[p];
+	std::vector<uint32_t> h_psize(np, 0);
+	for (uint32_t p = 0; p < np; ++p) {
+		h_psize[p] = po[p + 1] - po[p];
 		maxp = std::max<uint64_t>(maxp, h_psize[p]);
 	}
 
@@ -397,7 +418,7 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	TM_CHECK(upload(&fib->key_dict, dict, bytes));
 	TM_CHECK(upload(&fib->part_info, bb.part, bytes));
 	TM_CHECK(upload(&fib->part_stat, bb.stat, bytes));
-	fib->n_parts = n_parts;
+	fib->n_parts = np;
 	fib->n_blocks = bb.nb;
 	fib->n_keys = K;
 	fib->n_filters = n;
@@ -426,6 +447,14 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 	TM_CHECK(require_device());
 	bool bump = false;   // device arrays, P or K change: rings must re-read the FIB
 	const uint32_t e = fib->epoch + 1;
+	if (!fib->slot_epoch) {
+		// lane epochs from now on, so that later inserts can run live
+		TM_CHECK(dalloc(&fib->slot_epoch, (size_t)std::max<uint32_t>(fib->n_blocks, 1) * 64));
+		hipLaunchKernelGGL(k_init_epochs, dim3((uint32_t)((fib->n_blocks * 64ull + 255) / 256)), dim3(256), 0, 0,
+						   fib->blk_info, 0u, fib->n_blocks, fib->slot_epoch, 0u);
+		TM_HIP(hipGetLastError());
+		TM_HIP(hipDeviceSynchronize());
+	}
 
 	// 1. keys new to the dictionary: grow it and remap dense ids
 	const uint64_t nkeys = key_offsets[n];
@@ -669,6 +698,11 @@ int tm_fib_insert_live(tm_fib * fib, const uint32_t * words, uint32_t n, const u
 		target[i] = (uint32_t)best;
 		size[best]++;
 	}
+	// Submissions made before lane epochs existed do not filter lanes by
+	// epoch: while one runs, a lane could become valid before its key slot
+	// is written.  Epochs are created only with nothing in flight.
+	if (!fib->slot_epoch && fib->in_flight.load() > 0)
+		return set_error(TM_ERR_BUSY, "the first live insert needs no submission in flight: use tm_fib_insert");
 	// 2. device state for epochs and the insert stream
 	if (!fib->ins_stream) TM_HIP(hipStreamCreateWithFlags(&fib->ins_stream, hipStreamNonBlocking));
 	hipStream_t s = fib->ins_stream;

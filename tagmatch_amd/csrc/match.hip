@@ -152,9 +152,19 @@ k_build_items(const uint32_t * __restrict__ pcount, const uint64_t * __restrict_
 
 // ------------------------------------------------------------- match
 
+// The overflow list a slot points to, bounded by the table's overflow words:
+// whatever a slot holds, nothing outside the table is read (n = 0 then).
+__device__ inline uint32_t ovf_list(uint64_t kv, const uint32_t * __restrict__ ovf, uint32_t ovf_n, uint32_t & off) {
+	off = (uint32_t)(kv >> 16);
+	if (off >= ovf_n) return 0;
+	const uint32_t n = ovf[off];
+	return n < ovf_n - off ? n : 0;
+}
+
 // Number of keys in a slot (inline ids or an overflow list).
-__device__ inline uint32_t key_count(uint64_t kv, const uint32_t * __restrict__ ovf) {
-	if ((kv & 0xFFFFu) == KEY_NONE) return ovf[(uint32_t)(kv >> 16)];
+__device__ inline uint32_t key_count(uint64_t kv, const uint32_t * __restrict__ ovf, uint32_t ovf_n) {
+	uint32_t off;
+	if ((kv & 0xFFFFu) == KEY_NONE) return ovf_list(kv, ovf, ovf_n, off);
 	uint32_t n = 0;
 	for (int x = 0; x < 4; ++x) n += ((kv >> (16 * x)) & 0xFFFFu) != KEY_NONE;
 	return n;
@@ -220,7 +230,8 @@ constexpr int HBUF = 256;                     // staged hits per wave (HITS mode
 // Dense key ids of a hit filter into a bitmap row; `kv` inline form has up to
 // four 16-bit ids, KEY_NONE for an unused field (ORs 0 into word 0).
 template <bool LDS_BM>
-__device__ inline void or_keys(uint32_t * row, uint64_t kv, const uint32_t * __restrict__ ovf) {
+__device__ inline void or_keys(uint32_t * row, uint64_t kv, const uint32_t * __restrict__ ovf, uint32_t ovf_n,
+							   uint32_t n_keys) {
 	if ((kv & 0xFFFFu) != KEY_NONE) {
 #pragma unroll
 		for (int i = 0; i < 4; ++i) {
@@ -231,9 +242,11 @@ __device__ inline void or_keys(uint32_t * row, uint64_t kv, const uint32_t * __r
 			else __hip_atomic_fetch_or(&row[w], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 		}
 	} else {
-		const uint32_t off = (uint32_t)(kv >> 16), n = ovf[off];
+		uint32_t off;
+		const uint32_t n = ovf_list(kv, ovf, ovf_n, off);
 		for (uint32_t i = 0; i < n; ++i) {
 			const uint32_t id = ovf[off + 1 + i];
+			if (id >= n_keys) continue;
 			if (LDS_BM) atomicOr(&row[id >> 5], 1u << (id & 31));
 			else __hip_atomic_fetch_or(&row[id >> 5], 1u << (id & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 		}
@@ -269,6 +282,7 @@ struct hit_sink {
 	uint4 * klog;             // KLOG: this wave's key log (64 entries)
 	uint32_t hcnt;
 	const uint32_t * ovf;
+	uint32_t ovf_n, n_keys;
 	unsigned long long * counter;
 	uint2 * hits;
 	uint64_t cap;
@@ -276,7 +290,7 @@ struct hit_sink {
 	__device__ inline void drain(uint32_t lane) {
 		if (lane < hcnt) {
 			const uint4 e = klog[lane];
-			or_keys<LDS_BM>(bm_base + (size_t)e.x * kw, (uint64_t)e.y | ((uint64_t)e.z << 32), ovf);
+			or_keys<LDS_BM>(bm_base + (size_t)e.x * kw, (uint64_t)e.y | ((uint64_t)e.z << 32), ovf, ovf_n, n_keys);
 		}
 		hcnt = 0;
 	}
@@ -299,7 +313,7 @@ struct hit_sink {
 			if (mine) klog[hcnt + mbcnt64(h)] = make_uint4(jq, (uint32_t)kv, (uint32_t)(kv >> 32), 0u);
 			hcnt += n;
 		} else if (MODE == 0) {
-			if (mine) or_keys<LDS_BM>(bm_base + (size_t)jq * kw, kv, ovf);
+			if (mine) or_keys<LDS_BM>(bm_base + (size_t)jq * kw, kv, ovf, ovf_n, n_keys);
 		} else {
 			const uint32_t n = (uint32_t)__popcll(h);
 			if (hcnt + n > (uint32_t)HBUF) flush(lane);
@@ -409,6 +423,8 @@ k_match_q(mq_args a) {
 		sink.kw = kw;
 		sink.hcnt = 0;
 		sink.ovf = a.f.ovf;
+		sink.ovf_n = a.f.ovf_n;
+		sink.n_keys = a.f.n_keys;
 		sink.counter = &a.counters[0];
 		sink.hits = a.hits;
 		sink.cap = a.hit_cap;
@@ -443,7 +459,7 @@ k_match_q(mq_args a) {
 			hit_acc += (uint32_t)__popcll(h);
 			const uint32_t ql = q0 + jl;
 			const uint32_t gq = (MODE == 1 && a.qmap) ? sld1(a.qmap + ql) : ql;
-			if (a.log && ((h >> lane) & 1)) key_acc += key_count(kv, a.f.ovf);
+			if (a.log && ((h >> lane) & 1)) key_acc += key_count(kv, a.f.ovf, a.f.ovf_n);
 			sink.add(h, jl, gq, b, kv, lane);
 		};
 		for (uint32_t r = r_begin; r < r_end; ++r) {
@@ -738,8 +754,9 @@ k_bm_write(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const ui
 	}
 }
 
-__device__ inline uint32_t slot_key_count(uint64_t kv, const uint32_t * __restrict__ ovf) {
-	if ((kv & 0xFFFFu) == KEY_NONE) return ovf[(uint32_t)(kv >> 16)];
+__device__ inline uint32_t slot_key_count(uint64_t kv, const uint32_t * __restrict__ ovf, uint32_t ovf_n) {
+	uint32_t off;
+	if ((kv & 0xFFFFu) == KEY_NONE) return ovf_list(kv, ovf, ovf_n, off);
 	uint32_t n = 1;
 	if (((kv >> 16) & 0xFFFFu) != KEY_NONE) ++n; else return n;
 	if (((kv >> 32) & 0xFFFFu) != KEY_NONE) ++n; else return n;
@@ -747,9 +764,12 @@ __device__ inline uint32_t slot_key_count(uint64_t kv, const uint32_t * __restri
 	return n;
 }
 
-__device__ inline uint32_t slot_key_id(uint64_t kv, const uint32_t * __restrict__ ovf, uint32_t i) {
-	if ((kv & 0xFFFFu) == KEY_NONE) return ovf[(uint32_t)(kv >> 16) + 1 + i];
-	return (uint32_t)((kv >> (16 * i)) & 0xFFFFu);
+// Key i < slot_key_count(kv) of a slot (a dense id, clamped into the
+// dictionary).
+__device__ inline uint32_t slot_key_id(uint64_t kv, const uint32_t * __restrict__ ovf, uint32_t i, uint32_t n_keys) {
+	const uint32_t id = (kv & 0xFFFFu) == KEY_NONE ? ovf[(uint32_t)(kv >> 16) + 1 + i]
+												   : (uint32_t)((kv >> (16 * i)) & 0xFFFFu);
+	return id < n_keys ? id : 0u;
 }
 
 // HITS / KEYS_ALL: values per query.
@@ -762,7 +782,7 @@ k_hits_count(const uint2 * __restrict__ hits, const unsigned long long * __restr
 	const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
 	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 		uint2 h = hits[i];
-		uint32_t c = KEYS ? slot_key_count(f.slot_keys[h.y], f.ovf) : 1u;
+		uint32_t c = KEYS ? slot_key_count(f.slot_keys[h.y], f.ovf, f.ovf_n) : 1u;
 		atomicAdd(&qcount[h.x], c);
 	}
 }
@@ -779,10 +799,10 @@ k_hits_write(const uint2 * __restrict__ hits, const unsigned long long * __restr
 		uint2 h = hits[i];
 		if (KEYS) {
 			uint64_t kv = f.slot_keys[h.y];
-			uint32_t c = slot_key_count(kv, f.ovf);
+			uint32_t c = slot_key_count(kv, f.ovf, f.ovf_n);
 			unsigned long long p = atomicAdd(&qcur[h.x], (unsigned long long)c);
 			for (uint32_t k = 0; k < c; ++k)
-				if (p + k < vcap) vals[p + k] = f.key_dict[slot_key_id(kv, f.ovf, k)];
+				if (p + k < vcap) vals[p + k] = f.key_dict[slot_key_id(kv, f.ovf, k, f.n_keys)];
 		} else {
 			unsigned long long p = atomicAdd(&qcur[h.x], 1ULL);
 			if (p < vcap) vals[p] = f.slot_fid[h.y];

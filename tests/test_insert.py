@@ -105,6 +105,15 @@ def test_live_insert_is_seen_only_by_later_submissions(gpu):
     fib, perm, off, masks = tm.build_fib(cw, ck, cv, 8000)
     _, _, _, big, _ = tm.workload(100000, 200000, seed=421)
     bw, bko, bk, _ = gen(3000, 1, seed=422)
+    # lane epochs start with the first insert, made with nothing in flight
+    b0w, b0ko, b0k, _ = gen(50, 1, seed=423)
+    ring0 = tm.Ring(fib, len(q))
+    ring0.submit(q)
+    assert not fib.insert_live(b0w, b0ko, b0k)          # declined: a submission without epochs runs
+    ring0.wait()
+    ring0.close()
+    assert fib.insert_live(b0w, b0ko, b0k)
+    cw, ck, cv = concat(cw, ck, cv, b0w, b0ko, b0k)
     w2, ko2, k2 = concat(cw, ck, cv, bw, bko, bk)
     for mode, om in MODES:
         eo1, ev1 = orc.match(cw, ck, cv, q, mode=om)
