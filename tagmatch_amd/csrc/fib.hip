@@ -684,16 +684,26 @@ int tm_fib_insert_live(tm_fib * fib, const uint32_t * words, uint32_t n, const u
 		pmpop[p] = c;
 	}
 	std::vector<uint32_t> size = fib->h_part_size, target(n);
-	for (uint32_t i = 0; i < n; ++i) {
-		const uint32_t * w = words + (size_t)i * 6;
+	auto choose = [&](const uint32_t * w, const std::vector<uint32_t> & sz) -> int64_t {
 		int64_t best = -1;
 		for (uint32_t p = 0; p < P; ++p) {
 			const uint32_t * m = &fib->h_part_info[(size_t)p * 8];
 			if ((m[0] & ~w[0]) | (m[1] & ~w[1]) | (m[2] & ~w[2]) | (m[3] & ~w[3]) | (m[4] & ~w[4]) | (m[5] & ~w[5]))
 				continue;
-			if (size[p] >= (uint64_t)fib->h_part_cap[p] * 64) continue;
-			if (best < 0 || pmpop[p] > pmpop[best] || (pmpop[p] == pmpop[best] && size[p] < size[best])) best = p;
+			if (sz[p] >= (uint64_t)fib->h_part_cap[p] * 64) continue;
+			if (best < 0 || pmpop[p] > pmpop[best] || (pmpop[p] == pmpop[best] && sz[p] < sz[best])) best = p;
 		}
+		return best;
+	};
+	// first choices in parallel on the sizes before the batch, then in
+	// order: a choice that the batch itself has filled is made again
+	std::vector<int64_t> first(n);
+	parallel_ranges(n, [&](size_t b0, size_t b1) {
+		for (size_t i = b0; i < b1; ++i) first[i] = choose(words + i * 6, size);
+	});
+	for (uint32_t i = 0; i < n; ++i) {
+		int64_t best = first[i];
+		if (best >= 0 && size[best] >= (uint64_t)fib->h_part_cap[best] * 64) best = choose(words + (size_t)i * 6, size);
 		if (best < 0) return set_error(TM_ERR_BUSY, "filter %u fits no spare lane: use tm_fib_insert", i);
 		target[i] = (uint32_t)best;
 		size[best]++;
