@@ -211,14 +211,15 @@ typedef struct tm_match_config {
 	uint32_t log_units;      /* record per-work-item hit counts and cycles   */
 	uint32_t order_by_cost;  /* PERSISTENT: visit partitions in the order set
 	                            by tm_ring_set_order                          */
-	uint32_t variant;        /* PERSISTENT inner loop, 1..7 or 10, as bits;
-	                            bit 0: candidate query words from scalar
-	                            loads (else LDS broadcasts); bit 1:
+	uint32_t variant;        /* PERSISTENT inner loop, 1..7, 10 or 18, as
+	                            bits; bit 0: candidate query words from
+	                            scalar loads (else LDS broadcasts); bit 1:
 	                            KEYS_UNIQUE hits go through a per-wave key
 	                            log drained 64 lanes at a time; bit 2: early
 	                            exit after words 0-1; bit 3: the next
 	                            block's data is loaded while one is tested;
-	                            0 or other values: default (2)               */
+	                            bit 4: blocks in rounds of 4, whose loads
+	                            overlap; 0 or other values: default (2)      */
 	uint32_t full_exit;      /* PERSISTENT KEYS_UNIQUE: stop testing a query
 	                            once its key set holds every key of the
 	                            table (results unchanged); 0 = default (on),

@@ -40,7 +40,10 @@ namespace {
 
 constexpr int WG = 256;          // threads per match workgroup (4 waves)
 constexpr int NQUEUE = 8;        // work queues, one per XCD of an MI355X
+constexpr int QPAD = 32;         // queue cursors 256 B apart: atomics on one line serialize
 constexpr uint32_t SEG_MAX_BLOCKS = 1024;   // persistent: largest block segment (measured at 10M)
+constexpr uint32_t WSEG_BASE = 64;          // block weight: fixed part, in units of one expected hit
+constexpr uint32_t WSEG_MAX_TILES = 256;    // persistent: tiles up to which segments are cut by weight
 constexpr uint32_t NO_QUERY = 0xFFFFFFFFu;
 
 __device__ inline uint32_t mbcnt64(uint64_t m) {
@@ -147,7 +150,7 @@ k_build_items(const uint32_t * __restrict__ pcount, const uint64_t * __restrict_
 	}
 	if (tid == 0) item_prefix[P] = carry;
 	if (tid <= NQUEUE) qbeg[tid] = carry * tid / NQUEUE;
-	if (tid < NQUEUE) qnext[tid] = carry * tid / NQUEUE;
+	if (tid < NQUEUE) qnext[tid * QPAD] = carry * tid / NQUEUE;
 }
 
 // ------------------------------------------------------------- match
@@ -208,7 +211,7 @@ struct mq_args {
 	uint32_t ntiles;                          // query tiles
 	uint32_t nseg;                            // block segments per tile: item = seg * ntiles + tile
 	uint32_t seg_blocks;                      // blocks per segment (nseg > 1)
-	unsigned long long * qnext;               // NQUEUE queue cursors
+	unsigned long long * qnext;               // NQUEUE queue cursors, QPAD apart
 	const uint64_t * __restrict__ qbeg;       // NQUEUE + 1 queue bounds (items)
 	uint32_t * pcount;                        // per partition: queries selecting it
 	uint32_t * gbm;                           // KEYS_UNIQUE: nq x kw bitmap words
@@ -223,9 +226,13 @@ struct mq_args {
 	uint32_t full_exit;                       // KEYS_UNIQUE + LDS bitmaps: skip queries whose
 	                                          // bitmap holds every key
 	uint32_t last_mask;                       // valid bits of a bitmap row's last word
+	uint32_t * qfull;                         // nseg > 1 with full_exit: per query, set once its
+	                                          // merged gbm row holds every key
+	const uint32_t * __restrict__ segb;       // nseg + 1 segment bounds (NULL: seg_blocks each)
 };
 
 constexpr int HBUF = 256;                     // staged hits per wave (HITS mode)
+constexpr uint32_t COOP_KEYS = 16;            // overflow lists longer than this: walked by the whole wave
 
 // Dense key ids of a hit filter into a bitmap row; `kv` inline form has up to
 // four 16-bit ids, KEY_NONE for an unused field (ORs 0 into word 0).
@@ -268,6 +275,20 @@ __device__ inline uint2 sld2(const uint32_t * p) {
 }
 __device__ inline uint32_t sld1(const uint32_t * p) { return *(cu32_t *)(p); }
 
+// True when a bitmap row in LDS holds every key.  Each lane starts at word
+// lane % kw: rows are kw words apart, so lanes that started at word 0 would
+// all read one LDS bank.
+__device__ inline bool row_full(const uint32_t * row, uint32_t kw, uint32_t lane, uint32_t last_mask) {
+	uint32_t w = lane % kw;
+	for (uint32_t i = 0; i < kw; ++i) {
+		uint32_t v = row[w];
+		if (w + 1 == kw) v |= ~last_mask;
+		if (v != 0xFFFFFFFFu) return false;
+		w = w + 1 == kw ? 0 : w + 1;
+	}
+	return true;
+}
+
 // Per-wave hit sink of the query-major kernel.
 //
 // KEYS_UNIQUE with KLOG: a hit appends (bitmap row, packed key ids) to this
@@ -287,10 +308,37 @@ struct hit_sink {
 	uint2 * hits;
 	uint64_t cap;
 
+	// Inline ids and short overflow lists: one entry per lane.  Long lists
+	// (filters that merged many duplicates carry up to every key of the
+	// table) would keep one lane busy for thousands of iterations while the
+	// rest of the wave waits: the wave walks them together, 64 ids at a time.
 	__device__ inline void drain(uint32_t lane) {
+		uint4 e = make_uint4(0, 0, 0, 0);
+		bool coop = false;
 		if (lane < hcnt) {
-			const uint4 e = klog[lane];
-			or_keys<LDS_BM>(bm_base + (size_t)e.x * kw, (uint64_t)e.y | ((uint64_t)e.z << 32), ovf, ovf_n, n_keys);
+			e = klog[lane];
+			const uint64_t kv = (uint64_t)e.y | ((uint64_t)e.z << 32);
+			uint32_t off;
+			coop = (kv & 0xFFFFu) == KEY_NONE && ovf_list(kv, ovf, ovf_n, off) > COOP_KEYS;
+			if (!coop) or_keys<LDS_BM>(bm_base + (size_t)e.x * kw, kv, ovf, ovf_n, n_keys);
+		}
+		uint64_t m = __ballot(coop);
+		while (m) {
+			const uint32_t j = (uint32_t)__builtin_ctzll(m);
+			m &= m - 1;
+			const uint32_t jrow = __shfl(e.x, j), jlo = __shfl(e.y, j), jhi = __shfl(e.z, j);
+			uint32_t off;
+			const uint32_t n = ovf_list((uint64_t)jlo | ((uint64_t)jhi << 32), ovf, ovf_n, off);
+			uint32_t * row = bm_base + (size_t)jrow * kw;
+#pragma unroll 4
+			for (uint32_t i = lane; i < n; i += 64) {
+				const uint32_t id = ovf[off + 1 + i];
+				if (id < n_keys) {
+					if (LDS_BM) atomicOr(&row[id >> 5], 1u << (id & 31));
+					else __hip_atomic_fetch_or(&row[id >> 5], 1u << (id & 31), __ATOMIC_RELAXED,
+											   __HIP_MEMORY_SCOPE_WORKGROUP);
+				}
+			}
 		}
 		hcnt = 0;
 	}
@@ -333,7 +381,7 @@ k_match_q(mq_args a) {
 	constexpr int NQ = TILES * 64;
 	constexpr int NT = NW * 64;
 	constexpr bool KLOG = MODE == 0 && (VAR & 2);
-	__shared__ uint32_t s_q[NQ * 8];
+	__shared__ __align__(16) uint32_t s_q[NQ * 8];
 	__shared__ uint32_t s_item[2];
 	__shared__ uint32_t s_hits, s_keys;
 	__shared__ uint2 s_hbuf[MODE == 1 ? NW * HBUF : 1];
@@ -344,6 +392,10 @@ k_match_q(mq_args a) {
 	const uint32_t kw = a.f.kw;
 	const uint32_t home = blockIdx.x % NQUEUE;
 	const uint32_t P = a.f.n_parts;
+	// statistics of this wave over every item it works on, added to the
+	// global counters once at the end (all waves adding to one line per
+	// item serializes small launches)
+	uint64_t w_hits = 0, w_visit = 0, w_test = 0, w_hev = 0;
 
 	for (;;) {
 		uint64_t t_start = 0;
@@ -351,8 +403,8 @@ k_match_q(mq_args a) {
 			unsigned long long item = ~0ULL;
 			for (int k = 0; k < NQUEUE; ++k) {
 				const uint32_t x = (home + k) % NQUEUE;
-				if (a.qnext[x] >= a.qbeg[x + 1]) continue;
-				const unsigned long long i = atomicAdd(&a.qnext[x], 1ULL);
+				if (a.qnext[x * QPAD] >= a.qbeg[x + 1]) continue;
+				const unsigned long long i = atomicAdd(&a.qnext[x * QPAD], 1ULL);
 				if (i < a.qbeg[x + 1]) { item = i; break; }
 			}
 			s_item[0] = (uint32_t)item;
@@ -391,8 +443,13 @@ k_match_q(mq_args a) {
 		} else if (a.nseg > 1 || a.blk_lo != 0 || a.blk_hi < a.f.n_blocks) {
 			const uint32_t seg = item0 / a.ntiles;
 			item = item0 - seg * a.ntiles;
-			seg_b0 = a.blk_lo + seg * a.seg_blocks;
-			seg_b1 = min(seg_b0 + a.seg_blocks, a.blk_hi);
+			if (a.segb) {   // segments of equal estimated work
+				seg_b0 = sld1(a.segb + seg);
+				seg_b1 = sld1(a.segb + seg + 1);
+			} else {
+				seg_b0 = a.blk_lo + seg * a.seg_blocks;
+				seg_b1 = min(seg_b0 + a.seg_blocks, a.blk_hi);
+			}
 			if (seg_b1 > seg_b0) {
 				r_begin = sld1(a.f.blk_info + (size_t)seg_b0 * 8 + 7);
 				r_end = sld1(a.f.blk_info + (size_t)(seg_b1 - 1) * 8 + 7) + 1;
@@ -459,7 +516,9 @@ k_match_q(mq_args a) {
 			hit_acc += (uint32_t)__popcll(h);
 			const uint32_t ql = q0 + jl;
 			const uint32_t gq = (MODE == 1 && a.qmap) ? sld1(a.qmap + ql) : ql;
-			if (a.log && ((h >> lane) & 1)) key_acc += key_count(kv, a.f.ovf, a.f.ovf_n);
+			if (a.log) {   // uniform: a scalar branch around the per-lane count
+				if ((h >> lane) & 1) key_acc += key_count(kv, a.f.ovf, a.f.ovf_n);
+			}
 			sink.add(h, jl, gq, b, kv, lane);
 		};
 		for (uint32_t r = r_begin; r < r_end; ++r) {
@@ -495,104 +554,34 @@ k_match_q(mq_args a) {
 					if (qsel[t] == 0) continue;
 					// full: words 0..kw-2 all ones, word kw-1 holds the last K % 32 keys.
 					// With block segments the other segments of this tile merge
-					// their rows into gbm: the query's keys so far are the union.
-					const uint32_t * row = &s_bm[(t * 64 + lane) * kw];
-					const uint32_t qrow = q0 + t * 64 + lane;
-					const uint32_t * grow = (a.nseg > 1 && qrow < q_end) ? a.gbm + (size_t)qrow * kw : nullptr;
-					uint32_t all = 0xFFFFFFFFu;
-					if ((kw & 3) == 0) {
-						const uint4 * r4 = reinterpret_cast<const uint4 *>(row);
-						const uint4 * g4 = reinterpret_cast<const uint4 *>(grow);
-						for (uint32_t w4 = 0; w4 < (kw >> 2); ++w4) {
-							uint4 v = r4[w4];
-							if (grow) {
-								const uint4 g = g4[w4];
-								v.x |= g.x; v.y |= g.y; v.z |= g.z; v.w |= g.w;
-							}
-							if (w4 + 1 == (kw >> 2)) v.w |= ~a.last_mask;
-							all &= v.x & v.y & v.z & v.w;
-						}
+					// their rows into gbm and flag a query whose merged row is
+					// full (one coalesced load per tile here); in one segment
+					// the LDS row is the query's whole bitmap.
+					bool full;
+					if (a.nseg > 1) {
+						const uint32_t qrow = q0 + t * 64 + lane;
+						full = qrow < q_end &&
+							   __hip_atomic_load(&a.qfull[qrow], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 					} else {
-						for (uint32_t w = 0; w < kw; ++w) {
-							uint32_t v = row[w] | (grow ? grow[w] : 0u);
-							if (w + 1 == kw) v |= ~a.last_mask;
-							all &= v;
-						}
+						full = row_full(&s_bm[(t * 64 + lane) * kw], kw, lane, a.last_mask);
 					}
-					const bool full = all == 0xFFFFFFFFu;
 					qsel[t] &= ~__ballot(full);
 					anyq |= qsel[t];
 				}
 				if (anyq == 0) continue;
 			}
-			// VAR & 8: software pipeline, the next block's mask, words and
-			// key slots are loaded while this block is tested
-			constexpr bool PF = (VAR & 8) != 0;
-			uint4 p_ma = make_uint4(0, 0, 0, 0), p_mb = make_uint4(0, 0, 0, 0);
-			uint32_t p_f[6] = {0, 0, 0, 0, 0, 0};
-			uint64_t p_kv = 0;
-			if (PF && fb + wave < fb + nb) {
-				const uint32_t b = fb + wave;
-				p_ma = sld4(a.f.blk_info + (size_t)b * 8);
-				p_mb = sld4(a.f.blk_info + (size_t)b * 8 + 4);
-				const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
-#pragma unroll
-				for (int j = 0; j < 6; ++j) p_f[j] = bw[j * 64];
-				if (MODE == 0) p_kv = a.f.slot_keys[(size_t)b * 64 + lane];
-			}
-			for (uint32_t b = fb + wave; b < fb + nb; b += NW) {
-				uint4 ma, mb;
-				uint32_t c_f[6];
-				uint64_t c_kv = 0;
-				if (PF) {
-					ma = p_ma;
-					mb = p_mb;
-#pragma unroll
-					for (int j = 0; j < 6; ++j) c_f[j] = p_f[j];
-					c_kv = p_kv;
-					const uint32_t bn = b + NW;
-					if (bn < fb + nb) {
-						p_ma = sld4(a.f.blk_info + (size_t)bn * 8);
-						p_mb = sld4(a.f.blk_info + (size_t)bn * 8 + 4);
-						const uint32_t * bw = a.f.blk_words + (size_t)bn * BLOCK_WORDS + lane;
-#pragma unroll
-						for (int j = 0; j < 6; ++j) p_f[j] = bw[j * 64];
-						if (MODE == 0) p_kv = a.f.slot_keys[(size_t)bn * 64 + lane];
-					}
-				} else {
-					const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
-					ma = sld4(bi);
-					mb = sld4(bi + 4);
-				}
-				const uint32_t nv = mb.z;
-				uint64_t cand[TILES];
-				uint64_t any = 0;
-#pragma unroll
-				for (int t = 0; t < TILES; ++t) {
-					if (qsel[t] == 0) { cand[t] = 0; continue; }
-					const uint32_t x = (ma.x & qv[t][0]) | (ma.y & qv[t][1]) | (ma.z & qv[t][2]) |
-									   (ma.w & qv[t][3]) | (mb.x & qv[t][4]) | (mb.y & qv[t][5]);
-					cand[t] = __ballot(x == 0) & qsel[t];
-					any |= cand[t];
-				}
-				if (any == 0) continue;
-				++n_visit;
-
-				if (!PF) {
-					const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
-#pragma unroll
-					for (int j = 0; j < 6; ++j) c_f[j] = bw[j * 64];
-					c_kv = MODE == 0 ? a.f.slot_keys[(size_t)b * 64 + lane] : 0;
-				}
-				const uint32_t f0 = c_f[0], f1 = c_f[1], f2 = c_f[2], f3 = c_f[3], f4 = c_f[4], f5 = c_f[5];
-				const uint64_t kv = c_kv;
+			// Subset tests of block b's filters (words cf, key slots kv, nv valid
+			// lanes) against its candidate queries cand.
+			auto test_block = [&](uint32_t b, const uint32_t (&cf)[6], uint64_t kv, const uint64_t (&cand)[TILES],
+								  uint32_t nv) {
+				const uint32_t f0 = cf[0], f1 = cf[1], f2 = cf[2], f3 = cf[3], f4 = cf[4], f5 = cf[5];
 				uint64_t lmask = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
 				if (a.slot_epoch) lmask &= __ballot(a.slot_epoch[(size_t)b * 64 + lane] <= a.epoch);
-
 #pragma unroll
 				for (int t = 0; t < TILES; ++t) {
 					uint64_t c = cand[t];
-					if ((VAR & 1) == 0) {
+					n_test += (uint32_t)__popcll(c);
+					if ((VAR & 5) == 0) {
 						// candidate query words broadcast from LDS, two per iteration
 						while (c) {
 							const uint32_t j0 = (uint32_t)__builtin_ctzll(c);
@@ -600,25 +589,55 @@ k_match_q(mq_args a) {
 							const uint64_t keep1 = c ? ~0ULL : 0ULL;
 							const uint32_t j1 = c ? (uint32_t)__builtin_ctzll(c) : j0;
 							c &= c - 1;
-							n_test += keep1 ? 2 : 1;
 							const uint32_t * qq0 = &s_q[(t * 64 + j0) * 8];
 							const uint32_t * qq1 = &s_q[(t * 64 + j1) * 8];
-							const uint2 a0 = *reinterpret_cast<const uint2 *>(qq0);
-							const uint2 a1 = *reinterpret_cast<const uint2 *>(qq1);
-							uint32_t acc0 = (f0 & a0.x) | (f1 & a0.y);
-							uint32_t acc1 = (f0 & a1.x) | (f1 & a1.y);
+							const uint4 a0 = *reinterpret_cast<const uint4 *>(qq0);
+							const uint2 c0 = *reinterpret_cast<const uint2 *>(qq0 + 4);
+							const uint4 a1 = *reinterpret_cast<const uint4 *>(qq1);
+							const uint2 c1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
+							const uint32_t acc0 =
+								(f0 & a0.x) | (f1 & a0.y) | (f2 & a0.z) | (f3 & a0.w) | (f4 & c0.x) | (f5 & c0.y);
+							const uint32_t acc1 =
+								(f0 & a1.x) | (f1 & a1.y) | (f2 & a1.z) | (f3 & a1.w) | (f4 & c1.x) | (f5 & c1.y);
+							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
+							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
+							if (h0) on_hit(h0, t * 64 + j0, b, kv);
+							if (h1) on_hit(h1, t * 64 + j1, b, kv);
+						}
+					} else if ((VAR & 1) == 0) {
+						// early exit variant: words 0-1 first, two per iteration
+						while (c) {
+							const uint32_t j0 = (uint32_t)__builtin_ctzll(c);
+							c &= c - 1;
+							const uint64_t keep1 = c ? ~0ULL : 0ULL;
+							const uint32_t j1 = c ? (uint32_t)__builtin_ctzll(c) : j0;
+							c &= c - 1;
+							const uint32_t * qq0 = &s_q[(t * 64 + j0) * 8];
+							const uint32_t * qq1 = &s_q[(t * 64 + j1) * 8];
+							uint32_t acc0, acc1;
 							if (VAR & 4) {
+								const uint2 a0 = *reinterpret_cast<const uint2 *>(qq0);
+								const uint2 a1 = *reinterpret_cast<const uint2 *>(qq1);
+								acc0 = (f0 & a0.x) | (f1 & a0.y);
+								acc1 = (f0 & a1.x) | (f1 & a1.y);
 								// early exit: words 0-1 already reject every filter
 								// for both candidates (the common case)
 								const uint64_t alive = (__ballot(acc0 == 0) | (__ballot(acc1 == 0) & keep1)) & lmask;
 								if (alive == 0) continue;
+								const uint2 b0 = *reinterpret_cast<const uint2 *>(qq0 + 2);
+								const uint2 c0 = *reinterpret_cast<const uint2 *>(qq0 + 4);
+								const uint2 b1 = *reinterpret_cast<const uint2 *>(qq1 + 2);
+								const uint2 c1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
+								acc0 |= (f2 & b0.x) | (f3 & b0.y) | (f4 & c0.x) | (f5 & c0.y);
+								acc1 |= (f2 & b1.x) | (f3 & b1.y) | (f4 & c1.x) | (f5 & c1.y);
+							} else {
+								const uint4 a0 = *reinterpret_cast<const uint4 *>(qq0);
+								const uint2 c0 = *reinterpret_cast<const uint2 *>(qq0 + 4);
+								const uint4 a1 = *reinterpret_cast<const uint4 *>(qq1);
+								const uint2 c1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
+								acc0 = (f0 & a0.x) | (f1 & a0.y) | (f2 & a0.z) | (f3 & a0.w) | (f4 & c0.x) | (f5 & c0.y);
+								acc1 = (f0 & a1.x) | (f1 & a1.y) | (f2 & a1.z) | (f3 & a1.w) | (f4 & c1.x) | (f5 & c1.y);
 							}
-							const uint2 b0 = *reinterpret_cast<const uint2 *>(qq0 + 2);
-							const uint2 c0 = *reinterpret_cast<const uint2 *>(qq0 + 4);
-							const uint2 b1 = *reinterpret_cast<const uint2 *>(qq1 + 2);
-							const uint2 c1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
-							acc0 |= (f2 & b0.x) | (f3 & b0.y) | (f4 & c0.x) | (f5 & c0.y);
-							acc1 |= (f2 & b1.x) | (f3 & b1.y) | (f4 & c1.x) | (f5 & c1.y);
 							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
 							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
 							if (h0) on_hit(h0, t * 64 + j0, b, kv);
@@ -633,7 +652,6 @@ k_match_q(mq_args a) {
 							const uint64_t keep1 = c ? ~0ULL : 0ULL;
 							const uint32_t j1 = c ? (uint32_t)__builtin_ctzll(c) : j0;
 							c &= c - 1;
-							n_test += keep1 ? 2 : 1;
 							const uint2 a0 = sld2(qbase + j0 * 8), a1 = sld2(qbase + j1 * 8);
 							uint32_t acc0 = (f0 & a0.x) | (f1 & a0.y);
 							uint32_t acc1 = (f0 & a1.x) | (f1 & a1.y);
@@ -651,37 +669,160 @@ k_match_q(mq_args a) {
 						}
 					}
 				}
+			};
+			// candidate queries of a block from its AND mask (ma, mb: info words 0-7)
+			auto block_cands = [&](const uint4 & ma, const uint4 & mb, uint64_t (&cand)[TILES]) -> uint64_t {
+				uint64_t any = 0;
+#pragma unroll
+				for (int t = 0; t < TILES; ++t) {
+					if (qsel[t] == 0) { cand[t] = 0; continue; }
+					const uint32_t x = (ma.x & qv[t][0]) | (ma.y & qv[t][1]) | (ma.z & qv[t][2]) |
+									   (ma.w & qv[t][3]) | (mb.x & qv[t][4]) | (mb.y & qv[t][5]);
+					cand[t] = __ballot(x == 0) & qsel[t];
+					any |= cand[t];
+				}
+				return any;
+			};
+			const uint32_t be = fb + nb;
+			if (VAR & 16) {
+				// GB blocks per round: their info with one scalar-load wait, then
+				// the words of every block that has candidates with one vector
+				// wait, then the tests; the loads of a round overlap.
+				constexpr int GB = 4;
+				for (uint32_t b0 = fb + wave; b0 < be; b0 += NW * GB) {
+					uint64_t cand[GB][TILES];
+					uint64_t any[GB];
+					uint32_t nv[GB];
+#pragma unroll
+					for (int g = 0; g < GB; ++g) {
+						// past the end: read the last block, use nothing
+						const uint32_t b = min(b0 + g * NW, be - 1);
+						const uint4 ma = sld4(a.f.blk_info + (size_t)b * 8);
+						const uint4 mb = sld4(a.f.blk_info + (size_t)b * 8 + 4);
+						nv[g] = mb.z;
+						any[g] = block_cands(ma, mb, cand[g]);
+						if (b0 + g * NW >= be) any[g] = 0;
+					}
+					uint64_t anyall = 0;
+#pragma unroll
+					for (int g = 0; g < GB; ++g) anyall |= any[g];
+					if (anyall == 0) continue;
+					uint32_t cf[GB][6];
+					uint64_t kv[GB];
+#pragma unroll
+					for (int g = 0; g < GB; ++g) {
+						kv[g] = 0;
+#pragma unroll
+						for (int j = 0; j < 6; ++j) cf[g][j] = 0;
+						if (any[g]) {
+							const uint32_t b = b0 + g * NW;
+							const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
+#pragma unroll
+							for (int j = 0; j < 6; ++j) cf[g][j] = bw[j * 64];
+							if (MODE == 0) kv[g] = a.f.slot_keys[(size_t)b * 64 + lane];
+						}
+					}
+#pragma unroll
+					for (int g = 0; g < GB; ++g) {
+						if (!any[g]) continue;
+						++n_visit;
+						test_block(b0 + g * NW, cf[g], kv[g], cand[g], nv[g]);
+					}
+				}
+			} else if (VAR & 8) {
+				// software pipeline: the next block's mask, words and key slots
+				// are loaded while this block is tested
+				uint4 p_ma = make_uint4(0, 0, 0, 0), p_mb = make_uint4(0, 0, 0, 0);
+				uint32_t p_f[6] = {0, 0, 0, 0, 0, 0};
+				uint64_t p_kv = 0;
+				if (fb + wave < be) {
+					const uint32_t b = fb + wave;
+					p_ma = sld4(a.f.blk_info + (size_t)b * 8);
+					p_mb = sld4(a.f.blk_info + (size_t)b * 8 + 4);
+					const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
+#pragma unroll
+					for (int j = 0; j < 6; ++j) p_f[j] = bw[j * 64];
+					if (MODE == 0) p_kv = a.f.slot_keys[(size_t)b * 64 + lane];
+				}
+				for (uint32_t b = fb + wave; b < be; b += NW) {
+					const uint4 ma = p_ma, mb = p_mb;
+					uint32_t c_f[6];
+#pragma unroll
+					for (int j = 0; j < 6; ++j) c_f[j] = p_f[j];
+					const uint64_t c_kv = p_kv;
+					const uint32_t bn = b + NW;
+					if (bn < be) {
+						p_ma = sld4(a.f.blk_info + (size_t)bn * 8);
+						p_mb = sld4(a.f.blk_info + (size_t)bn * 8 + 4);
+						const uint32_t * bw = a.f.blk_words + (size_t)bn * BLOCK_WORDS + lane;
+#pragma unroll
+						for (int j = 0; j < 6; ++j) p_f[j] = bw[j * 64];
+						if (MODE == 0) p_kv = a.f.slot_keys[(size_t)bn * 64 + lane];
+					}
+					uint64_t cand[TILES];
+					if (block_cands(ma, mb, cand) == 0) continue;
+					++n_visit;
+					test_block(b, c_f, c_kv, cand, mb.z);
+				}
+			} else {
+				for (uint32_t b = fb + wave; b < be; b += NW) {
+					const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
+					const uint4 ma = sld4(bi), mb = sld4(bi + 4);
+					uint64_t cand[TILES];
+					if (block_cands(ma, mb, cand) == 0) continue;
+					++n_visit;
+					uint32_t c_f[6];
+					const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
+#pragma unroll
+					for (int j = 0; j < 6; ++j) c_f[j] = bw[j * 64];
+					const uint64_t c_kv = MODE == 0 ? a.f.slot_keys[(size_t)b * 64 + lane] : 0;
+					test_block(b, c_f, c_kv, cand, mb.z);
+				}
 			}
 		}
 		if (MODE == 1 && sink.hcnt) sink.flush(lane);
 		if (KLOG && sink.hcnt) sink.drain(lane);
-		if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
+		w_hits += hit_acc;
+		w_visit += n_visit;
+		w_test += n_test;
+		w_hev += n_hev;
 		if (a.log) {
+			if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
 			for (int d = 32; d > 0; d >>= 1) key_acc += __shfl_xor(key_acc, d, 64);
 			if (lane == 0 && key_acc) atomicAdd(&s_keys, key_acc);
 		}
-		if (lane == 0 && n_visit) {
-			atomicAdd(&a.counters[2], (unsigned long long)n_visit);
-			atomicAdd(&a.counters[3], (unsigned long long)n_test);
-			atomicAdd(&a.counters[4], (unsigned long long)n_hev);
-		}
 		__syncthreads();
 		if (MODE == 0 && LDS_BM) {
-			const uint32_t nw = (min(q_end, q0 + NQ) - q0) * kw;
+			const uint32_t nrows = min(q_end, q0 + NQ) - q0;
+			const uint32_t nw = nrows * kw;
 			uint32_t * dst = a.gbm + (size_t)q0 * kw;
-			if (a.nseg > 1) {   // other segments of the tile write the same rows
-				for (uint32_t w = tid; w < nw; w += NT) {
-					const uint32_t v = s_bm[w];
-					// an atomic only for bits the row does not hold yet: rows
-					// fill up, and most words are then complete
-					if (v && (v & ~dst[w])) atomicOr(&dst[w], v);
+			if (a.nseg > 1) {
+				// Other segments of the tile write the same rows: an atomic
+				// only for bits a row does not hold yet (rows fill up, and most
+				// words are then complete).  One wave per row; a row that is
+				// now full flags its query for the full-key-set exit.
+				for (uint32_t j = wave; j < nrows; j += NW) {
+					const uint32_t * src = s_bm + (size_t)j * kw;
+					uint32_t * drow = dst + (size_t)j * kw;
+					uint32_t any = 0;
+					for (uint32_t w = lane; w < kw; w += 64) any |= src[w];
+					if (__ballot(any != 0) == 0) continue;   // nothing from this segment
+					uint32_t all = 0xFFFFFFFFu;
+					for (uint32_t w = lane; w < kw; w += 64) {
+						const uint32_t v = src[w], d = drow[w];
+						if (v & ~d) atomicOr(&drow[w], v);
+						uint32_t u = v | d;
+						if (w + 1 == kw) u |= ~a.last_mask;
+						all &= u;
+					}
+					if (a.qfull && __ballot(all != 0xFFFFFFFFu) == 0 && lane == 0)
+						__hip_atomic_store(&a.qfull[q0 + j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 				}
 			} else {
 				for (uint32_t w = tid; w < nw; w += NT) dst[w] = s_bm[w];
 			}
 		}
 		if (tid == 0) {
-			if (s_hits) atomicAdd(&a.counters[1], (unsigned long long)s_hits);
 			if (a.log && a.log_unit >= 0) {   // plain: every item of the launch adds up
 				atomicAdd(&a.log[(size_t)a.log_unit * 3], s_hits);
 				atomicAdd(&a.log[(size_t)a.log_unit * 3 + 1], s_keys);
@@ -693,6 +834,12 @@ k_match_q(mq_args a) {
 			}
 		}
 		__syncthreads();
+	}
+	if (lane == 0 && (w_hits || w_visit)) {
+		atomicAdd(&a.counters[1], (unsigned long long)w_hits);
+		atomicAdd(&a.counters[2], (unsigned long long)w_visit);
+		atomicAdd(&a.counters[3], (unsigned long long)w_test);
+		atomicAdd(&a.counters[4], (unsigned long long)w_hev);
 	}
 }
 
@@ -708,7 +855,7 @@ k_invert_queries(const uint32_t * __restrict__ q, uint32_t nq, uint32_t * __rest
 __global__ void k_init_queues(uint64_t items, uint64_t * __restrict__ qbeg, unsigned long long * __restrict__ qnext) {
 	const uint32_t t = threadIdx.x;
 	if (t <= NQUEUE) qbeg[t] = items * t / NQUEUE;
-	if (t < NQUEUE) qnext[t] = items * t / NQUEUE;
+	if (t < NQUEUE) qnext[t * QPAD] = items * t / NQUEUE;
 }
 
 // ------------------------------------------------------------- finalize
@@ -835,7 +982,7 @@ k_init_unit_queues(const uint32_t * __restrict__ items, uint32_t n_units, uint64
 	if (u >= n_units) return;
 	const uint64_t n = items[u];
 	for (int x = 0; x <= NQUEUE; ++x) qbeg[(size_t)u * (NQUEUE + 1) + x] = n * x / NQUEUE;
-	for (int x = 0; x < NQUEUE; ++x) qnext[(size_t)u * NQUEUE + x] = n * x / NQUEUE;
+	for (int x = 0; x < NQUEUE; ++x) qnext[((size_t)u * NQUEUE + x) * QPAD] = n * x / NQUEUE;
 }
 
 // OR one plain unit's staging rows into its queries' rows and clear them
@@ -851,6 +998,48 @@ k_or_rows(uint32_t * __restrict__ stage, uint32_t kw, const uint32_t * __restric
 	const uint32_t t = w / kw;
 	gbm[(size_t)qmap[t] * kw + (w - t * kw)] |= v;
 	stage[w] = 0;
+}
+
+// Estimated work of each block for a 64-query tile whose queries have bit
+// density rho (l2rho = log2 rho): a fixed part for the candidate pass and
+// the subset tests, plus the expected hits, 64 x the sum over valid lanes of
+// rho^popcount (a random query of density rho contains a filter of
+// popcount k with probability rho^k).  One wave per block.
+__global__ void __launch_bounds__(256)
+k_block_weights(const uint32_t * __restrict__ blk_words, const uint32_t * __restrict__ blk_info, uint32_t nb,
+				float l2rho, uint32_t * __restrict__ w) {
+	const uint32_t lane = threadIdx.x & 63;
+	const uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (b >= nb) return;
+	const uint32_t nv = blk_info[(size_t)b * 8 + 6];
+	float e = 0.f;
+	if (lane < nv) {
+		uint32_t pop = 0;
+		for (int j = 0; j < 6; ++j) pop += __popc(blk_words[(size_t)b * BLOCK_WORDS + j * 64 + lane]);
+		e = exp2f(l2rho * (float)pop);
+	}
+	for (int d = 32; d > 0; d >>= 1) e += __shfl_xor(e, d, 64);
+	if (lane == 0) w[b] = WSEG_BASE + (uint32_t)fminf(64.f * e, 4096.f);
+}
+
+// Bounds of nseg segments of about equal weight: segb[k] is the first block
+// whose exclusive weight prefix reaches k/nseg of the total; segb[0] = 0,
+// segb[nseg] = nb.  Non-decreasing in k; a segment may be empty.
+__global__ void __launch_bounds__(256)
+k_seg_bounds(const uint64_t * __restrict__ prefix, uint32_t nb, uint32_t nseg, uint32_t * __restrict__ segb) {
+	const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+	if (k > nseg) return;
+	if (k == 0 || k == nseg) {
+		segb[k] = k == 0 ? 0u : nb;
+		return;
+	}
+	const uint64_t target = (prefix[nb] * k + nseg - 1) / nseg;
+	uint32_t lo = 0, hi = nb;
+	while (lo < hi) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (prefix[mid] >= target) hi = mid; else lo = mid + 1;
+	}
+	segb[k] = lo;
 }
 
 // ------------------------------------------------------------- kernel table
@@ -891,6 +1080,7 @@ mq_fn pick_match_q(int mode, bool lds_bm, int tiles, int nw, int var) {
 	case 6: return pick_q_mode<6>(mode, lds_bm, tiles, nw);
 	case 7: return pick_q_mode<7>(mode, lds_bm, tiles, nw);
 	case 10: return pick_q_mode<10>(mode, lds_bm, tiles, nw);
+	case 18: return pick_q_mode<18>(mode, lds_bm, tiles, nw);
 	default: return pick_q_mode<2>(mode, lds_bm, tiles, nw);
 	}
 }
@@ -922,6 +1112,15 @@ struct tm_ring {
 	uint32_t * d_porder = nullptr;
 	bool have_order = false;
 	uint32_t * d_gbm = nullptr;
+	uint32_t * d_qfull = nullptr;   // max_q: full-key-set flags of merged rows
+	uint32_t * d_bw = nullptr;      // per block: work estimate (weighted segments)
+	uint64_t * d_bprefix = nullptr; // n_blocks + 1: exclusive prefix of d_bw
+	uint64_t * d_bscan = nullptr;   // scan scratch for d_bprefix
+	uint32_t * d_segb = nullptr;    // n_blocks + 1: segment bounds
+	uint32_t bw_cap = 0, bw_nb = 0;
+	uint64_t bw_version = ~0ull;
+	float bw_l2 = 0.f;              // log2 density the weights were computed for
+	float rho_l2 = -1.f;            // log2 mean bit density of the last host queries
 	uint2 * d_hits = nullptr;
 	uint64_t hit_cap = 0;
 	uint32_t * d_qcount = nullptr;
@@ -1004,7 +1203,7 @@ void tm_ring::release() {
 	h_stat = nullptr;
 	dfree(d_q); dfree(d_qinv); dfree(d_pcount); dfree(d_pbase); dfree(d_pcur); dfree(d_list);
 	dfree(d_sched); dfree(d_iprefix); dfree(d_qbeg); dfree(d_qnext); dfree(d_porder);
-	dfree(d_gbm); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
+	dfree(d_gbm); dfree(d_qfull); dfree(d_bw); dfree(d_bprefix); dfree(d_bscan); dfree(d_segb); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
 	dfree(d_vals); dfree(d_cnt); dfree(d_scan); dfree(d_stat); dfree(d_log);
 	dfree(d_units); dfree(d_ufeat); dfree(d_stage); dfree(d_uitems); dfree(d_ubeg); dfree(d_unext);
 	dfree(d_xunits); dfree(d_xprefix); dfree(d_iota);
@@ -1046,7 +1245,7 @@ int resolve_params(tm_ring * r) {
 	r->chunk = 1u << 30;   // work items of the lookup: one per (partition, batch)
 	const size_t max_lds = (size_t)dev().max_lds;
 	const size_t nq_grp = (size_t)r->tiles * 64;
-	r->variant = ((c.variant >= 1 && c.variant <= 7) || c.variant == 10) ? (int)c.variant : 2;
+	r->variant = ((c.variant >= 1 && c.variant <= 7) || c.variant == 10 || c.variant == 18) ? (int)c.variant : 2;
 	int nw = c.waves_per_group ? (int)c.waves_per_group : (c.schedule == TM_SCHED_PLAIN ? 8 : 4);
 	if (nw != 4 && nw != 8 && nw != 16)
 		return set_error(TM_ERR_INVALID, "waves_per_group must be 4, 8 or 16 (got %d)", nw);
@@ -1151,8 +1350,29 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	const uint32_t seg_blocks = nseg > 1 ? (nblk + nseg - 1) / nseg : std::max<uint32_t>(nblk, 1);
 	if (nseg > 1) nseg = (nblk + seg_blocks - 1) / seg_blocks;
 	const uint64_t items = tiles * nseg;
+	// Few tiles: segments of equal estimated work rather than equal block
+	// counts.  Blocks of low-popcount filters are matched by most dense
+	// queries; uniform segments put them in a few long work items, which
+	// then set the step's time.
+	const bool weighted = nseg > 1 && tiles <= WSEG_MAX_TILES && !r->cfg.chunk_blocks;
+	if (weighted) {
+		const float l2 = std::round(r->rho_l2 * 16.f) / 16.f;
+		if (r->bw_version != r->fib->version || r->bw_nb != nblk || r->bw_l2 != l2) {
+			hipLaunchKernelGGL(k_block_weights, dim3((nblk + 3) / 4), dim3(256), 0, s, r->fib->blk_words,
+							   r->fib->blk_info, nblk, l2, r->d_bw);
+			scan_u32_to_u64(r->d_bw, r->d_bprefix, nblk, r->d_bscan, s);
+			r->bw_version = r->fib->version;
+			r->bw_nb = nblk;
+			r->bw_l2 = l2;
+		}
+		hipLaunchKernelGGL(k_seg_bounds, dim3((nseg + 256) / 256), dim3(256), 0, s, r->d_bprefix, nblk, nseg,
+						   r->d_segb);
+	}
 	r->last_nseg = nseg;
-	if (nseg > 1 && r->cfg.mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
+	if (nseg > 1 && r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
+		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
+		fill_u32(r->d_qfull, 0, nq, s);
+	}
 	hipLaunchKernelGGL(k_init_queues, dim3(1), dim3(64), 0, s, items, r->d_qbeg, r->d_qnext);
 	mq_args a;
 	std::memset(&a, 0, sizeof(a));
@@ -1168,6 +1388,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.ntiles = (uint32_t)tiles;
 	a.nseg = nseg;
 	a.seg_blocks = seg_blocks;
+	a.segb = weighted ? r->d_segb : nullptr;
 	a.blk_lo = 0;
 	a.blk_hi = nblk;
 	a.log_unit = -1;
@@ -1176,6 +1397,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.qbeg = r->d_qbeg;
 	a.pcount = r->d_pcount;
 	a.gbm = r->d_gbm;
+	a.qfull = r->d_qfull;
 	a.hits = r->d_hits;
 	a.hit_cap = r->hit_cap;
 	a.counters = r->d_cnt;
@@ -1279,7 +1501,7 @@ int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit
 		dfree(r->d_unext);
 		r->uq_cap = 0;
 		TM_CHECK(dalloc(&r->d_ubeg, (size_t)std::max<uint32_t>(nu, 1) * (NQUEUE + 1)));
-		TM_CHECK(dalloc(&r->d_unext, (size_t)std::max<uint32_t>(nu, 1) * NQUEUE));
+		TM_CHECK(dalloc(&r->d_unext, (size_t)std::max<uint32_t>(nu, 1) * NQUEUE * QPAD));
 		r->uq_cap = nu;
 	}
 	if (mode == TM_MODE_KEYS_UNIQUE && r->stage_cap < (uint64_t)bq * r->kw) {
@@ -1342,7 +1564,7 @@ int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit
 		a.nseg = u.nseg;
 		a.seg_blocks = u.seg_blocks;
 		a.qbeg = r->d_ubeg + (size_t)i * (NQUEUE + 1);
-		a.qnext = r->d_unext + (size_t)i * NQUEUE;
+		a.qnext = r->d_unext + (size_t)i * NQUEUE * QPAD;
 		a.log = (r->cfg.log_units && i < r->n_units) ? r->d_log : nullptr;
 		a.log_unit = a.log ? (int32_t)i : -1;
 		const uint64_t nit = (uint64_t)u.ntiles * u.nseg;
@@ -1361,6 +1583,17 @@ int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit
 
 int ensure_capacity(tm_ring * r) {
 	const uint32_t P = r->P;
+	const uint32_t nblk = r->fib->n_blocks;
+	if (r->bw_cap < nblk || !r->d_bw) {   // weighted segments: sized by the table
+		dfree(r->d_bw); dfree(r->d_bprefix); dfree(r->d_bscan); dfree(r->d_segb);
+		const uint32_t cap = std::max<uint32_t>(nblk, 1);
+		TM_CHECK(dalloc(&r->d_bw, cap));
+		TM_CHECK(dalloc(&r->d_bprefix, (size_t)cap + 1));
+		TM_CHECK(dalloc(&r->d_bscan, scan_tmp_words(cap)));
+		TM_CHECK(dalloc(&r->d_segb, (size_t)cap + 1));
+		r->bw_cap = cap;
+		r->bw_version = ~0ull;
+	}
 	if (!r->d_pcount) {
 		TM_CHECK(dalloc(&r->d_pcount, P + 1));
 		TM_CHECK(dalloc(&r->d_pbase, P + 1));
@@ -1368,7 +1601,7 @@ int ensure_capacity(tm_ring * r) {
 		TM_CHECK(dalloc(&r->d_sched, P + 1));
 		TM_CHECK(dalloc(&r->d_iprefix, P + 1));
 		TM_CHECK(dalloc(&r->d_qbeg, NQUEUE + 1));
-		TM_CHECK(dalloc(&r->d_qnext, NQUEUE));
+		TM_CHECK(dalloc(&r->d_qnext, NQUEUE * QPAD));
 		TM_CHECK(dalloc(&r->d_porder, P + 1));
 		TM_CHECK(dalloc(&r->d_cnt, 8));
 		TM_CHECK(dalloc(&r->d_stat, 8));
@@ -1383,6 +1616,7 @@ int ensure_capacity(tm_ring * r) {
 		TM_CHECK(dalloc(&r->d_gbm, n));
 		TM_HIP(hipMemset(r->d_gbm, 0, std::max<size_t>(n, 1) * 4));
 	}
+	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->d_qfull) TM_CHECK(dalloc(&r->d_qfull, std::max<uint32_t>(r->max_q, 1)));
 	if (r->cfg.mode != TM_MODE_KEYS_UNIQUE && !r->d_hits) {
 		uint64_t cap = r->cfg.hit_capacity ? r->cfg.hit_capacity : std::max<uint64_t>(1 << 20, (uint64_t)r->max_q * 64);
 		TM_CHECK(dalloc(&r->d_hits, cap));
@@ -1417,7 +1651,10 @@ int enqueue_units(tm_ring * r, uint32_t nq) {
 	const int mode = r->cfg.mode;
 	const tm_fib * f = r->fib;
 	const uint32_t n = (uint32_t)r->u_part.size();
-	if (mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
+	if (mode == TM_MODE_KEYS_UNIQUE) {
+		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
+		fill_u32(r->d_qfull, 0, nq, s);
+	}
 	fill_u32(r->d_pcount, 0, r->P, s);
 	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 8, s);
 	scan_u32_to_u64(r->d_pcount, r->d_pbase, r->P, r->d_scan, s);
@@ -1500,6 +1737,7 @@ int enqueue_units(tm_ring * r, uint32_t nq) {
 	a.qnext = r->d_qnext;
 	a.qbeg = r->d_qbeg;
 	a.gbm = r->d_gbm;
+	a.qfull = r->d_qfull;
 	a.hits = r->d_hits;
 	a.hit_cap = r->hit_cap;
 	a.counters = r->d_cnt;
@@ -1817,10 +2055,17 @@ int tm_submit(tm_ring * r, const uint32_t * words, uint32_t nq) {
 	if (nq && !words) return set_error(TM_ERR_INVALID, "query words are NULL");
 	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
 	TM_CHECK(require_device());
+	uint64_t bits = 0;
 	for (uint32_t i = 0; i < nq; ++i) {
 		std::memcpy(r->h_q + (size_t)i * 8, words + (size_t)i * 6, 24);
 		r->h_q[(size_t)i * 8 + 6] = 0;
 		r->h_q[(size_t)i * 8 + 7] = 0;
+		if (i < 4096)
+			for (int j = 0; j < 6; ++j) bits += (uint64_t)__builtin_popcount(words[(size_t)i * 6 + j]);
+	}
+	if (nq) {   // mean bit density of the queries, for weighted segments
+		const double rho = (double)bits / (192.0 * std::min<uint32_t>(nq, 4096));
+		r->rho_l2 = (float)std::max(-16.0, std::log2(std::max(rho, 1e-6)));
 	}
 	if (nq) TM_HIP(hipMemcpyAsync(r->d_q, r->h_q, (size_t)nq * 32, hipMemcpyHostToDevice, r->s));
 	r->units_mode = false;

@@ -160,7 +160,7 @@ def test_batch_sizes_and_chunks(gpu, batch):
             assert_same(o, v, eo, ev, sort_got=False, what=f"b{batch} s{sched} c{chunk}")
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 10])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 10, 18])
 def test_inner_loop_variants_vs_oracle(gpu, variant):
     """Every persistent inner-loop variant (scalar or LDS query words, key
     log, early exit), large submissions (one tile per item) and small ones
@@ -190,13 +190,15 @@ def test_full_key_set_exit(gpu, n_keys):
     # one segment (bitmaps accumulate in LDS), and 8-block segments (the
     # rows other segments merged into HBM count too)
     for seg in (1 << 30, 8):
-        res = {}
-        for fe in (1, 2):
-            o, v, st = tm.match(fib, q, full_exit=fe, chunk_blocks=seg)
-            assert_same(o, v, eo, ev, sort_got=False, what=f"full_exit {fe} segment {seg}")
-            res[fe] = st["n_hits"]
-        log(f"n_keys {n_keys} segment {seg}: hits processed {res[1]} with the exit, {res[2]} without")
-        assert res[1] < res[2]
+        for var in (2, 18):
+            res = {}
+            for fe in (1, 2):
+                o, v, st = tm.match(fib, q, full_exit=fe, chunk_blocks=seg, variant=var)
+                assert_same(o, v, eo, ev, sort_got=False, what=f"full_exit {fe} segment {seg} variant {var}")
+                res[fe] = st["n_hits"]
+            log(f"n_keys {n_keys} segment {seg} variant {var}: hits processed {res[1]} with the exit, "
+                f"{res[2]} without")
+            assert res[1] < res[2]
 
 
 def bitf(*bits):
