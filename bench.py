@@ -67,6 +67,9 @@ def parse():
                     help="batches per partition in the uneven-batching leg (0: skip the leg)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds per serving load level (0: skip)")
     ap.add_argument("--serve-request", type=int, default=64, help="queries per serving request")
+    ap.add_argument("--serve-slots", type=int, default=4, help="serving slots (device steps in flight)")
+    ap.add_argument("--serve-slot-queries", type=int, default=1024,
+                    help="queries per serving step at most (small steps keep the latency low)")
     ap.add_argument("--serve-insert", type=int, default=10000, help="filters per insert (0: none)")
     ap.add_argument("--serve-insert-ms", type=float, default=100.0, help="interval between inserts")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
@@ -242,7 +245,7 @@ def serving_leg(fib, args, n_filters):
     g = tm.Generator(seed=args.seed ^ 0x5E, superset_frac=0.2)
     pool, _ = g.queries(1 << 17, n_filters)
     gi = tm.Generator(seed=args.seed)
-    srv = tm.Server(fib, n_slots=4, slot_queries=16384)
+    srv = tm.Server(fib, n_slots=args.serve_slots, slot_queries=args.serve_slot_queries)
     rng = np.random.default_rng(11)
 
     def take(i):
@@ -287,6 +290,7 @@ def serving_leg(fib, args, n_filters):
             srv.wait(rid)
         wall = time.perf_counter() - t0
         lat = np.array([srv.info(rid).latency_ms for rid in ids])
+        ins_lat = np.array([srv.info(rid).latency_ms for rid in inserts]) if inserts else np.zeros(1)
         for rid in ids + inserts:
             srv.release(rid)
         st_l = srv.stats()
@@ -294,11 +298,13 @@ def serving_leg(fib, args, n_filters):
             "offered_qps": load * peak, "achieved_qps": n * req / wall, "requests": n,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "p999_ms": float(np.percentile(lat, 99.9)), "max_ms": float(lat.max()),
-            "inserts": len(inserts), "live_inserts_so_far": st_l["n_live_inserts"]}
+            "inserts": len(inserts), "live_inserts_so_far": st_l["n_live_inserts"],
+            "insert_p50_ms": float(np.percentile(ins_lat, 50)), "insert_max_ms": float(ins_lat.max())}
         log(f"serving {load:.0%}: {levels[f'{int(load * 100)}%']}")
     st = srv.stats()
     srv.close()
-    return {"peak_qps": peak, "request_queries": req, "slots": 4, "slot_queries": 16384,
+    return {"peak_qps": peak, "request_queries": req, "slots": args.serve_slots,
+            "slot_queries": args.serve_slot_queries,
             "device_ms_per_step": st["device_ms_sum"] / max(st["n_steps"], 1),
             "insert_filters": args.serve_insert, "insert_every_ms": args.serve_insert_ms,
             "levels": levels, "steps": st["n_batches"], "budget_waits": st["n_budget_waits"],

@@ -202,8 +202,9 @@ typedef struct tm_match_config {
 	                            the table (0 = automatic: at most 1024, and
 	                            smaller for submissions too small to fill
 	                            the GPU)                                     */
-	uint32_t waves_per_group;/* waves per workgroup (4, 8 or 16); 0 = 4
-	                            (PERSISTENT) or 8 (PLAIN)                    */
+	uint32_t waves_per_group;/* waves per workgroup (4, 8 or 16); 0 = by
+	                            step size for PERSISTENT (16 up to 16 query
+	                            tiles, 8 up to 256, else 4), 8 for PLAIN     */
 	uint32_t groups_per_cu;  /* PERSISTENT: resident workgroups per CU; 0 = auto */
 	uint64_t hit_capacity;   /* HITS / KEYS_ALL: hit slots; 0 = default      */
 	uint32_t use_graph;      /* PERSISTENT: replay steady-state steps from a

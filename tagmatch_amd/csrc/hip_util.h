@@ -15,6 +15,8 @@ namespace tmx {
 // Thread-local text of the last error (tm_last_error()).
 std::string & last_error();
 int set_error(int code, const char * fmt, ...);
+void * pinned_alloc(size_t bytes);   // hipHostMalloc; NULL on failure
+void pinned_free(void * p);
 int hip_error(hipError_t e, const char * what, const char * file, int line);
 
 #define TM_HIP(expr)                                                         \

@@ -170,6 +170,16 @@ void copy_u64(uint64_t * d, const uint64_t * src, uint64_t n, hipStream_t s) {
 	if (n) hipLaunchKernelGGL(k_copy_u64, dim3(fill_grid(n)), dim3(256), 0, s, d, src, n);
 }
 
+// Pinned host memory (hipHostMalloc) for host code compiled without the HIP
+// headers: result buffers that device copies fill by DMA.
+void * pinned_alloc(size_t bytes) {
+	void * p = nullptr;
+	return hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) == hipSuccess ? p : nullptr;
+}
+void pinned_free(void * p) {
+	if (p) (void)hipHostFree(p);
+}
+
 } // namespace tmx
 
 using namespace tmx;

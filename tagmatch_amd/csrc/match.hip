@@ -304,6 +304,7 @@ struct hit_sink {
 	uint32_t hcnt;
 	const uint32_t * ovf;
 	uint32_t ovf_n, n_keys;
+	uint32_t last_mask;       // valid bits of a row's last word
 	unsigned long long * counter;
 	uint2 * hits;
 	uint64_t cap;
@@ -330,6 +331,10 @@ struct hit_sink {
 			uint32_t off;
 			const uint32_t n = ovf_list((uint64_t)jlo | ((uint64_t)jhi << 32), ovf, ovf_n, off);
 			uint32_t * row = bm_base + (size_t)jrow * kw;
+			// a row that already holds every key gains nothing (rows only grow)
+			uint32_t all = 0xFFFFFFFFu;
+			for (uint32_t w = lane; w < kw; w += 64) all &= row[w] | (w + 1 == kw ? ~last_mask : 0u);
+			if (__ballot(all != 0xFFFFFFFFu) == 0) continue;
 #pragma unroll 4
 			for (uint32_t i = lane; i < n; i += 64) {
 				const uint32_t id = ovf[off + 1 + i];
@@ -482,6 +487,7 @@ k_match_q(mq_args a) {
 		sink.ovf = a.f.ovf;
 		sink.ovf_n = a.f.ovf_n;
 		sink.n_keys = a.f.n_keys;
+		sink.last_mask = a.last_mask;
 		sink.counter = &a.counters[0];
 		sink.hits = a.hits;
 		sink.cap = a.hit_cap;
@@ -1006,20 +1012,21 @@ k_or_rows(uint32_t * __restrict__ stage, uint32_t kw, const uint32_t * __restric
 // rho^popcount (a random query of density rho contains a filter of
 // popcount k with probability rho^k).  One wave per block.
 __global__ void __launch_bounds__(256)
-k_block_weights(const uint32_t * __restrict__ blk_words, const uint32_t * __restrict__ blk_info, uint32_t nb,
-				float l2rho, uint32_t * __restrict__ w) {
+k_block_weights(tmx::dev_fib_view f, uint32_t nb, float l2rho, uint32_t * __restrict__ w) {
 	const uint32_t lane = threadIdx.x & 63;
 	const uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
 	if (b >= nb) return;
-	const uint32_t nv = blk_info[(size_t)b * 8 + 6];
+	const uint32_t nv = f.blk_info[(size_t)b * 8 + 6];
 	float e = 0.f;
 	if (lane < nv) {
 		uint32_t pop = 0;
-		for (int j = 0; j < 6; ++j) pop += __popc(blk_words[(size_t)b * BLOCK_WORDS + j * 64 + lane]);
-		e = exp2f(l2rho * (float)pop);
+		for (int j = 0; j < 6; ++j) pop += __popc(f.blk_words[(size_t)b * BLOCK_WORDS + j * 64 + lane]);
+		// a hit costs one step, plus one per 64 keys of a long overflow list
+		const uint32_t nk = key_count(f.slot_keys[(size_t)b * 64 + lane], f.ovf, f.ovf_n);
+		e = exp2f(l2rho * (float)pop) * (1.f + (nk > COOP_KEYS ? (float)nk / 64.f : 0.f));
 	}
 	for (int d = 32; d > 0; d >>= 1) e += __shfl_xor(e, d, 64);
-	if (lane == 0) w[b] = WSEG_BASE + (uint32_t)fminf(64.f * e, 4096.f);
+	if (lane == 0) w[b] = WSEG_BASE + (uint32_t)fminf(64.f * e, 1048576.f);
 }
 
 // Bounds of nseg segments of about equal weight: segb[k] is the first block
@@ -1162,6 +1169,7 @@ struct tm_ring {
 
 	// effective launch parameters
 	int tiles = 4, wgs_per_cu = 2, waves = 8, variant = 1;
+	int wgs_for[3] = {1, 1, 1};     // resident workgroups per CU with 4, 8, 16 waves
 	uint32_t chunk = 256;
 	uint32_t plain_bq = 256;   // PLAIN: queries per (partition, batch) unit
 	bool lds_bm = false;       // key bitmaps in LDS
@@ -1255,12 +1263,32 @@ int resolve_params(tm_ring * r) {
 	size_t bm = nq_grp * r->kw * 4;
 	r->lds_bm = (mode_of(c) == 0) && bm + static_lds <= max_lds;
 	r->stage_lds = r->lds_bm ? bm : 0;
-	int by_lds = (int)(std::min<size_t>(160 * 1024, max_lds) / (r->stage_lds + static_lds));
-	int by_waves = 32 / nw;
-	int want = c.groups_per_cu ? (int)c.groups_per_cu : 64;
-	r->wgs_per_cu = std::max(1, std::min(want, std::min(by_lds, by_waves)));
+	const int want = c.groups_per_cu ? (int)c.groups_per_cu : 64;
+	for (int i = 0; i < 3; ++i) {
+		// resident workgroups per CU from the kernel's registers and LDS
+		const int w = 4 << i;
+		const mq_fn fn = pick_match_q(mode_of(c), r->lds_bm, r->tiles, w, r->variant);
+		int n = 0;
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(fn), w * 64,
+														  r->stage_lds) != hipSuccess || n < 1) {
+			const size_t st = nq_grp * 32 + 64 + (mode_of(c) == 1 ? (size_t)w * HBUF * 8 : 8) +
+							  ((mode_of(c) == 0 && (r->variant & 2)) ? (size_t)w * 64 * 16 : 16);
+			n = std::max(1, std::min((int)(std::min<size_t>(160 * 1024, max_lds) / (r->stage_lds + st)), 28 / w));
+		}
+		r->wgs_for[i] = std::max(1, std::min(want, n));
+	}
+	r->wgs_per_cu = r->wgs_for[nw == 4 ? 0 : nw == 8 ? 1 : 2];
 	return TM_OK;
 }
+
+// Waves per workgroup of a persistent step of `tiles` query tiles: the
+// configured value, or by default more waves for fewer tiles (a small step
+// has few items in flight per tile, and more waves share each item's blocks).
+int step_waves(const tm_ring * r, uint64_t tiles) {
+	if (r->cfg.waves_per_group) return r->waves;
+	return tiles <= 16 ? 16 : tiles <= 256 ? 8 : 4;
+}
+int wgs_for_waves(const tm_ring * r, int nw) { return r->wgs_for[nw == 4 ? 0 : nw == 8 ? 1 : 2]; }
 
 int grow_u32(uint32_t ** p, uint64_t & cap, uint64_t need) {
 	if (need <= cap && *p) return TM_OK;
@@ -1330,10 +1358,11 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 8, s);
 	const uint64_t nqg = (uint64_t)r->tiles * 64;
 	const uint64_t tiles = (nq + nqg - 1) / nqg;
+	const int nw = step_waves(r, tiles);
 	// Small submissions: cut each tile's work into block segments so that
 	// the items fill the GPU (about two per resident workgroup).  Needs the
 	// key bitmaps in LDS (flushed with atomicOr) or the hit-list modes.
-	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	const uint64_t resident = (uint64_t)dev().num_cu * wgs_for_waves(r, nw);
 	// Segments also bound the blocks a work item streams: with segment-major
 	// items, the workgroups on an XCD share one segment's blocks (at most
 	// SEG_MAX_BLOCKS x 2 KiB) in its L2 instead of each streaming the whole
@@ -1358,8 +1387,8 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	if (weighted) {
 		const float l2 = std::round(r->rho_l2 * 16.f) / 16.f;
 		if (r->bw_version != r->fib->version || r->bw_nb != nblk || r->bw_l2 != l2) {
-			hipLaunchKernelGGL(k_block_weights, dim3((nblk + 3) / 4), dim3(256), 0, s, r->fib->blk_words,
-							   r->fib->blk_info, nblk, l2, r->d_bw);
+			hipLaunchKernelGGL(k_block_weights, dim3((nblk + 3) / 4), dim3(256), 0, s, r->fib->view(), nblk, l2,
+							   r->d_bw);
 			scan_u32_to_u64(r->d_bw, r->d_bprefix, nblk, r->d_bscan, s);
 			r->bw_version = r->fib->version;
 			r->bw_nb = nblk;
@@ -1409,10 +1438,10 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 		hipLaunchKernelGGL(k_invert_queries, dim3((uint32_t)std::max<uint64_t>(1, (nw + 255) / 256)), dim3(256), 0, s,
 						   r->d_q, nq, r->d_qinv);
 	}
-	mq_fn fn = pick_match_q(mode_of(r->cfg), r->lds_bm, r->tiles, r->waves, r->variant);
-	uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)dev().num_cu * r->wgs_per_cu, std::max<uint64_t>(items, 1));
+	mq_fn fn = pick_match_q(mode_of(r->cfg), r->lds_bm, r->tiles, nw, r->variant);
+	uint32_t grid = (uint32_t)std::min<uint64_t>(resident, std::max<uint64_t>(items, 1));
 	if (events) TM_HIP(hipEventRecord(r->em0, s));
-	hipLaunchKernelGGL(fn, dim3(grid), dim3(r->waves * 64), r->stage_lds, s, a);
+	hipLaunchKernelGGL(fn, dim3(grid), dim3(nw * 64), r->stage_lds, s, a);
 	TM_HIP(hipGetLastError());
 	if (events) TM_HIP(hipEventRecord(r->em1, s));
 	scan_u32_to_u64(r->d_pcount, r->d_pbase, P, r->d_scan, s);

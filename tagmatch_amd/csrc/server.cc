@@ -19,14 +19,17 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
-namespace tmx {
-int set_error(int code, const char * fmt, ...);   // hip_util.hip: text for tm_last_error()
+namespace tmx {   // hip_util.hip
+int set_error(int code, const char * fmt, ...);   // text for tm_last_error()
+void * pinned_alloc(size_t bytes);                // pinned host memory, NULL on failure
+void pinned_free(void * p);
 }
 
 namespace {
@@ -46,15 +49,36 @@ struct request {
 	std::vector<uint32_t> ins_words, ins_koff, ins_keys, ins_fid;
 	uint32_t ins_n = 0, ins_maxp = 0;
 	bool ins_tried_live = false;      // the live insert declined: apply at a barrier
-	// results
+	// results: offsets of its own, values a slice of its step's buffer
 	std::vector<uint64_t> off;
-	std::vector<uint32_t> vals;
+	std::shared_ptr<struct result_buf> rbuf;
+	uint64_t vbase = 0, vn = 0;
 	int status = TM_OK;
 	std::string error;
 	uint64_t reserved = 0;            // bytes of the HBM budget held
 	clock_t_::time_point t_submit, t_admit, t_done;
 	uint32_t step_queries = 0;        // queries of the device step it ran in
 	bool done = false;
+};
+
+// Result values of one step in pinned host memory: filled by one DMA copy
+// and shared by the step's requests until the last is released, then kept
+// in the pool for later steps.
+struct result_buf {
+	uint32_t * p = nullptr;
+	uint64_t cap = 0;   // values
+};
+
+struct buf_pool {
+	std::mutex mu;
+	std::vector<result_buf *> free;
+	size_t keep = 8;
+	~buf_pool() {
+		for (result_buf * b : free) {
+			tmx::pinned_free(b->p);
+			delete b;
+		}
+	}
 };
 
 struct slot {
@@ -64,7 +88,6 @@ struct slot {
 	bool busy = false;
 	std::vector<uint32_t> words;
 	std::vector<uint64_t> off;
-	std::vector<uint32_t> vals;
 };
 
 } // namespace
@@ -88,6 +111,7 @@ struct tm_server {
 	std::atomic<bool> stop{false};
 	int fatal = TM_OK;
 	std::string fatal_msg;
+	std::shared_ptr<buf_pool> pool = std::make_shared<buf_pool>();
 };
 
 namespace {
@@ -110,18 +134,53 @@ void finish(tm_server * s, request * r) {
 	}
 }
 
+// A pinned buffer of at least n values, from the pool or new; it returns to
+// the pool when the last request holding it is released.
+std::shared_ptr<result_buf> take_buf(tm_server * s, uint64_t n) {
+	std::shared_ptr<buf_pool> pool = s->pool;
+	result_buf * b = nullptr;
+	{
+		std::lock_guard<std::mutex> lk(pool->mu);
+		for (size_t i = 0; i < pool->free.size(); ++i)
+			if (pool->free[i]->cap >= n) {
+				b = pool->free[i];
+				pool->free.erase(pool->free.begin() + i);
+				break;
+			}
+	}
+	if (!b) {
+		const uint64_t cap = std::max<uint64_t>(n + n / 4, 1u << 18);
+		void * p = tmx::pinned_alloc(cap * 4);
+		if (!p) return nullptr;
+		b = new result_buf();
+		b->p = static_cast<uint32_t *>(p);
+		b->cap = cap;
+	}
+	return std::shared_ptr<result_buf>(b, [pool](result_buf * x) {
+		std::lock_guard<std::mutex> lk(pool->mu);
+		if (pool->free.size() < pool->keep) {
+			pool->free.push_back(x);
+		} else {
+			tmx::pinned_free(x->p);
+			delete x;
+		}
+	});
+}
+
 // Collect a finished step: split results per request, release the budget.
 void complete(tm_server * s, slot & sl) {
 	tm_status st;
 	int rc = tm_wait(sl.ring, &st);
 	const uint32_t nq = (uint32_t)(sl.words.size() / 6);
+	std::shared_ptr<result_buf> buf;
 	if (rc == TM_OK) {
 		sl.off.resize((size_t)nq + 1);
 		rc = tm_read_results(sl.ring, sl.off.data(), nullptr, 0);
 	}
 	if (rc == TM_OK) {
-		sl.vals.resize(sl.off[nq]);
-		rc = tm_read_results(sl.ring, sl.off.data(), sl.vals.data(), sl.vals.size());
+		buf = take_buf(s, sl.off[nq]);
+		rc = buf ? tm_read_results(sl.ring, sl.off.data(), buf->p, buf->cap)
+				 : tmx::set_error(TM_ERR_NOMEM, "pinned result buffer of %llu values", (unsigned long long)sl.off[nq]);
 	}
 	std::string err = rc == TM_OK ? std::string() : std::string(tm_last_error());
 	std::lock_guard<std::mutex> lk(s->mu);
@@ -133,7 +192,9 @@ void complete(tm_server * s, slot & sl) {
 			const uint64_t base = sl.off[q0];
 			r->off.resize((size_t)r->nq + 1);
 			for (uint32_t i = 0; i <= r->nq; ++i) r->off[i] = sl.off[q0 + i] - base;
-			r->vals.assign(sl.vals.begin() + base, sl.vals.begin() + sl.off[q0 + r->nq]);
+			r->rbuf = buf;
+			r->vbase = base;
+			r->vn = sl.off[q0 + r->nq] - base;
 		} else {
 			r->error = err;
 		}
@@ -314,6 +375,7 @@ int tm_server_create(tm_fib * fib, const tm_server_config * cfg, const tm_match_
 	// key bitmap rows are per query and independent of hits
 	s->fixed_per_query = s->mcfg.mode == TM_MODE_KEYS_UNIQUE ? 4 * ((fs.n_distinct_keys + 31) / 32) : 8;
 	s->slots.resize(s->cfg.n_slots);
+	s->pool->keep = 2 * (size_t)s->cfg.n_slots + 4;
 	for (auto & sl : s->slots) {
 		int rc = tm_ring_create(fib, s->cfg.slot_queries, &s->mcfg, &sl.ring);
 		if (rc != TM_OK) {
@@ -442,16 +504,17 @@ int tm_server_result(tm_server * s, uint64_t id, tm_request_info * info, uint64_
 	if (info) {
 		info->status = r->status;
 		info->n_queries = r->nq;
-		info->n_values = r->vals.size();
+		info->n_values = r->vn;
 		info->step_queries = r->step_queries;
 		info->latency_ms = ms_since(r->t_submit, r->t_done);
 		info->queue_ms = r->is_insert ? 0.0 : ms_since(r->t_submit, r->t_admit);
 	}
 	if (offsets && !r->off.empty()) std::memcpy(offsets, r->off.data(), r->off.size() * sizeof(uint64_t));
 	if (values) {
-		if (r->vals.size() > capacity)
-			return tmx::set_error(TM_ERR_OVERFLOW, "%zu values, capacity %llu", r->vals.size(), (unsigned long long)capacity);
-		if (!r->vals.empty()) std::memcpy(values, r->vals.data(), r->vals.size() * 4);
+		if (r->vn > capacity)
+			return tmx::set_error(TM_ERR_OVERFLOW, "%llu values, capacity %llu", (unsigned long long)r->vn,
+								  (unsigned long long)capacity);
+		if (r->vn) std::memcpy(values, r->rbuf->p + r->vbase, r->vn * 4);
 	}
 	return TM_OK;
 }
