@@ -5,7 +5,10 @@
 #include "hip_util.h"
 
 #include <atomic>
+#include <mutex>
 #include <vector>
+
+struct tm_fib;
 
 namespace tmx {
 
@@ -46,6 +49,16 @@ struct dev_fib_view {
 
 constexpr uint32_t BLOCK_WORDS = 6 * 64;
 constexpr uint32_t KEY_NONE = 0xFFFFu;
+constexpr uint32_t MAX_COMPLETE = 64;
+
+// A live insert prepared on the host (partition choice, lane records, host
+// copies updated), committed to the device later by the thread that makes
+// submissions.  Commits follow prepares in order.
+struct live_plan;
+int live_prepare(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+				 const uint32_t * keys, const uint32_t * filter_ids, live_plan ** out);
+int live_commit(tm_fib * fib, live_plan * plan);   // frees the plan
+void live_discard(live_plan * plan);
 
 } // namespace tmx
 
@@ -79,6 +92,17 @@ struct tm_fib {
 	void * d_recs = nullptr;             // insert records staging
 	size_t recs_cap = 0;
 	std::atomic<int> in_flight{0};       // ring submissions not yet waited for
+	// live inserts prepared on the host ahead of their device commit
+	// (tmx::live_prepare / live_commit): host copies change at prepare,
+	// under host_mu; the device and `epoch` at commit, in prepare order
+	std::mutex host_mu;
+	// filters whose key set is the whole dictionary (at most MAX_COMPLETE,
+	// from the upload; cleared when an insert grows the dictionary): a query
+	// that contains one matches every key
+	uint32_t * d_complete = nullptr;     // n_complete x 8 words (6 + padding)
+	uint32_t n_complete = 0;
+	uint32_t prep_epoch = 0;             // epoch of the last prepared insert
+	uint32_t ovf_valid = 1;              // words of ovf written on the device
 
 	tmx::dev_fib_view view() const {
 		tmx::dev_fib_view v;
@@ -87,7 +111,7 @@ struct tm_fib {
 		v.slot_keys = slot_keys;
 		v.slot_fid = slot_fid;
 		v.ovf = ovf;
-		v.ovf_n = (uint32_t)h_ovf.size();
+		v.ovf_n = ovf_valid;
 		v.key_dict = key_dict;
 		v.part_info = part_info;
 		v.part_stat = part_stat;

@@ -32,6 +32,8 @@ void tm_fib::release() {
 	dfree(part_info);
 	dfree(part_stat);
 	dfree(slot_epoch);
+	dfree(d_complete);
+	n_complete = 0;
 	epoch = 0;
 	n_parts = n_blocks = n_keys = 0;
 	n_filters = n_key_instances = device_bytes = max_part = 0;
@@ -327,6 +329,7 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 				  const uint32_t * key_offsets, const uint32_t * keys,
 				  const uint32_t * filter_ids) {
 	if (!fib) return set_error(TM_ERR_INVALID, "fib is NULL");
+	std::lock_guard<std::mutex> lk(fib->host_mu);
 	if (n && (!words || !key_offsets || !keys)) return set_error(TM_ERR_INVALID, "NULL input array");
 	if (!part_offsets || (n_parts && !part_masks)) return set_error(TM_ERR_INVALID, "NULL partition array");
 	if (part_offsets[0] != 0 || part_offsets[n_parts] != n)
@@ -405,6 +408,18 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 		maxp = std::max<uint64_t>(maxp, h_psize[p]);
 	}
 
+	// complete filters: a key list of K distinct keys is the whole dictionary
+	std::vector<uint32_t> complete;
+	for (uint32_t i = 0; i < n && complete.size() < MAX_COMPLETE * 8; ++i) {
+		if (key_offsets[i + 1] - key_offsets[i] != K || K == 0) continue;
+		std::vector<uint32_t> ks(keys + key_offsets[i], keys + key_offsets[i + 1]);
+		std::sort(ks.begin(), ks.end());
+		if (std::unique(ks.begin(), ks.end()) != ks.end()) continue;
+		for (int j = 0; j < 6; ++j) complete.push_back(words[(size_t)i * 6 + j]);
+		complete.push_back(0);
+		complete.push_back(0);
+	}
+
 	fib->release();
 	uint64_t bytes = 0;
 	TM_CHECK(upload(&fib->blk_words, bb.words, bytes));
@@ -418,6 +433,8 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	TM_CHECK(upload(&fib->key_dict, dict, bytes));
 	TM_CHECK(upload(&fib->part_info, bb.part, bytes));
 	TM_CHECK(upload(&fib->part_stat, bb.stat, bytes));
+	if (!complete.empty()) TM_CHECK(upload(&fib->d_complete, complete, bytes));
+	fib->n_complete = (uint32_t)(complete.size() / 8);
 	fib->n_parts = np;
 	fib->n_blocks = bb.nb;
 	fib->n_keys = K;
@@ -432,13 +449,204 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	fib->h_ovf = std::move(h_ovf);
 	fib->h_dict = std::move(dict);
 	fib->h_order = std::move(order);
+	fib->ovf_valid = (uint32_t)fib->h_ovf.size();
+	fib->prep_epoch = fib->epoch;
 	fib->version++;
 	return TM_OK;
 }
 
+} // extern "C"
+
+namespace tmx {
+
+struct live_plan {
+	std::vector<ins_rec> recs;
+	size_t ovf_from = 0, ovf_to = 0;              // overflow words this insert appended
+	std::vector<uint32_t> part_info, part_stat;   // P x 8, as after this insert
+	uint32_t epoch = 0;
+};
+
+namespace {
+
+int create_epochs(tm_fib * fib) {
+	if (!fib->ins_stream) TM_HIP(hipStreamCreateWithFlags(&fib->ins_stream, hipStreamNonBlocking));
+	TM_CHECK(dalloc(&fib->slot_epoch, (size_t)std::max<uint32_t>(fib->n_blocks, 1) * 64));
+	hipLaunchKernelGGL(k_init_epochs, dim3((uint32_t)((fib->n_blocks * 64ull + 255) / 256)), dim3(256), 0,
+					   fib->ins_stream, fib->blk_info, 0u, fib->n_blocks, fib->slot_epoch, 0u);
+	TM_HIP(hipGetLastError());
+	TM_HIP(hipStreamSynchronize(fib->ins_stream));
+	return TM_OK;
+}
+
+} // namespace
+
+// Host half of a live insert, with fib->host_mu held: checks that nothing
+// but spare lanes, known keys and room in the overflow lists is needed
+// (TM_ERR_BUSY and no change otherwise), chooses each filter's partition,
+// builds the lane records and updates the host copies.
+int live_prepare_locked(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+						const uint32_t * keys, const uint32_t * filter_ids, live_plan ** out) {
+	*out = nullptr;
+	const uint64_t nkeys = key_offsets[n];
+	for (uint64_t x = 0; x < nkeys; ++x)
+		if (!std::binary_search(fib->h_dict.begin(), fib->h_dict.end(), keys[x]))
+			return set_error(TM_ERR_BUSY, "key %u is new to the table: use tm_fib_insert", keys[x]);
+	const bool inline_ok = fib->n_keys < KEY_NONE;
+	size_t ovf_need = 0;
+	for (uint32_t i = 0; i < n; ++i) {
+		const uint32_t kn = key_offsets[i + 1] - key_offsets[i];
+		if (!inline_ok || kn > 4) ovf_need += 1 + kn;
+	}
+	if (fib->h_ovf.size() + ovf_need > fib->ovf_cap)
+		return set_error(TM_ERR_BUSY, "overflow key lists are full: use tm_fib_insert");
+	const uint32_t P = fib->n_parts;
+	std::vector<uint32_t> pmpop(P);
+	for (uint32_t p = 0; p < P; ++p) {
+		uint32_t c = 0;
+		for (int j = 0; j < 6; ++j) c += __builtin_popcount(fib->h_part_info[(size_t)p * 8 + j]);
+		pmpop[p] = c;
+	}
+	std::vector<uint32_t> size = fib->h_part_size, target(n);
+	auto choose = [&](const uint32_t * w, const std::vector<uint32_t> & sz) -> int64_t {
+		int64_t best = -1;
+		for (uint32_t p = 0; p < P; ++p) {
+			const uint32_t * m = &fib->h_part_info[(size_t)p * 8];
+			if ((m[0] & ~w[0]) | (m[1] & ~w[1]) | (m[2] & ~w[2]) | (m[3] & ~w[3]) | (m[4] & ~w[4]) | (m[5] & ~w[5]))
+				continue;
+			if (sz[p] >= (uint64_t)fib->h_part_cap[p] * 64) continue;
+			if (best < 0 || pmpop[p] > pmpop[best] || (pmpop[p] == pmpop[best] && sz[p] < sz[best])) best = p;
+		}
+		return best;
+	};
+	// first choices in parallel on the sizes before the batch, then in
+	// order: a choice that the batch itself has filled is made again
+	std::vector<int64_t> first(n);
+	parallel_ranges(n, [&](size_t b0, size_t b1) {
+		for (size_t i = b0; i < b1; ++i) first[i] = choose(words + i * 6, size);
+	});
+	for (uint32_t i = 0; i < n; ++i) {
+		int64_t best = first[i];
+		if (best >= 0 && size[best] >= (uint64_t)fib->h_part_cap[best] * 64) best = choose(words + (size_t)i * 6, size);
+		if (best < 0) return set_error(TM_ERR_BUSY, "filter %u fits no spare lane: use tm_fib_insert", i);
+		target[i] = (uint32_t)best;
+		size[best]++;
+	}
+	// every check passed: lanes and host copies
+	live_plan * pl = new live_plan();
+	pl->epoch = fib->prep_epoch + 1;
+	pl->ovf_from = fib->h_ovf.size();
+	pl->recs.resize(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		const uint32_t p = target[i];
+		const uint32_t * w = words + (size_t)i * 6;
+		const uint32_t k = fib->h_part_size[p]++;
+		ins_rec & r = pl->recs[i];
+		r.block = fib->h_part_info[(size_t)p * 8 + 6] + k / 64;
+		r.lane = k % 64;
+		r.fid = filter_ids ? filter_ids[i] : (uint32_t)(fib->n_filters + i);
+		r.pad = 0;
+		for (int j = 0; j < 6; ++j) r.w[j] = w[j];
+		const int rc = key_slot(keys, key_offsets[i], key_offsets[i + 1], fib->h_dict, inline_ok, fib->h_ovf, r.kv);
+		if (rc != TM_OK) {   // cannot happen after the capacity check above
+			delete pl;
+			return rc;
+		}
+		fib->h_part_info[(size_t)p * 8 + 7] = (fib->h_part_size[p] + 63) / 64;
+		uint32_t * ps = &fib->h_part_stat[(size_t)p * 8];
+		for (int j = 0; j < 6; ++j) {
+			ps[j] |= w[j];
+			ps[7] += (uint32_t)__builtin_popcount(w[j]);
+		}
+		ps[6] += 1;
+		fib->max_part = std::max<uint64_t>(fib->max_part, fib->h_part_size[p]);
+	}
+	pl->ovf_to = fib->h_ovf.size();
+	pl->part_info = fib->h_part_info;
+	pl->part_stat = fib->h_part_stat;
+	fib->prep_epoch = pl->epoch;
+	fib->n_filters += n;
+	fib->n_key_instances += nkeys;
+	*out = pl;
+	return TM_OK;
+}
+
+// Device half, with fib->host_mu held, on the insert stream: the new
+// overflow words, the lanes (written with the plan's epoch), then the block
+// counts and statistics.  A submission that sees a new block count sees
+// lanes that its epoch hides.
+int live_commit_locked(tm_fib * fib, live_plan * pl) {
+	const uint32_t n = (uint32_t)pl->recs.size();
+	if (!fib->ins_stream) TM_HIP(hipStreamCreateWithFlags(&fib->ins_stream, hipStreamNonBlocking));
+	hipStream_t s = fib->ins_stream;
+	if (fib->recs_cap < n) {
+		if (fib->d_recs) {
+			TM_HIP(hipStreamSynchronize(s));
+			(void)hipFree(fib->d_recs);
+			fib->d_recs = nullptr;
+			fib->recs_cap = 0;
+		}
+		TM_HIP(hipMalloc(&fib->d_recs, sizeof(ins_rec) * (size_t)n * 2));
+		fib->recs_cap = (size_t)n * 2;
+	}
+	if (pl->ovf_to > pl->ovf_from)
+		TM_HIP(hipMemcpyAsync(fib->ovf + pl->ovf_from, fib->h_ovf.data() + pl->ovf_from,
+							  (pl->ovf_to - pl->ovf_from) * 4, hipMemcpyHostToDevice, s));
+	if (n) {
+		TM_HIP(hipMemcpyAsync(fib->d_recs, pl->recs.data(), sizeof(ins_rec) * n, hipMemcpyHostToDevice, s));
+		hipLaunchKernelGGL(k_insert_lanes, dim3((n + 255) / 256), dim3(256), 0, s, (const ins_rec *)fib->d_recs, n,
+						   fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid, fib->slot_epoch, pl->epoch);
+		TM_HIP(hipGetLastError());
+	}
+	TM_HIP(hipMemcpyAsync(fib->part_info, pl->part_info.data(), pl->part_info.size() * 4, hipMemcpyHostToDevice, s));
+	TM_HIP(hipMemcpyAsync(fib->part_stat, pl->part_stat.data(), pl->part_stat.size() * 4, hipMemcpyHostToDevice, s));
+	TM_HIP(hipStreamSynchronize(s));
+	fib->ovf_valid = (uint32_t)std::max<size_t>(fib->ovf_valid, pl->ovf_to);
+	fib->epoch = pl->epoch;
+	return TM_OK;
+}
+
+int live_prepare(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+				 const uint32_t * keys, const uint32_t * filter_ids, live_plan ** out) {
+	*out = nullptr;
+	if (!fib) return set_error(TM_ERR_INVALID, "fib is NULL");
+	TM_CHECK(check_table(words, n, key_offsets, keys));
+	if (fib->h_order.size() != 192) return set_error(TM_ERR_STATE, "no table uploaded");
+	std::lock_guard<std::mutex> lk(fib->host_mu);
+	// lane epochs are created by a commit-side call only (tm_fib_insert or
+	// tm_fib_insert_live), in the thread that makes the submissions
+	if (!fib->slot_epoch) return set_error(TM_ERR_BUSY, "no lane epochs yet");
+	if (n == 0) {
+		*out = new live_plan();
+		(*out)->epoch = ++fib->prep_epoch;
+		(*out)->ovf_from = (*out)->ovf_to = fib->h_ovf.size();
+		(*out)->part_info = fib->h_part_info;
+		(*out)->part_stat = fib->h_part_stat;
+		return TM_OK;
+	}
+	return live_prepare_locked(fib, words, n, key_offsets, keys, filter_ids, out);
+}
+
+int live_commit(tm_fib * fib, live_plan * plan) {
+	int rc = TM_OK;
+	{
+		std::lock_guard<std::mutex> lk(fib->host_mu);
+		rc = require_device();
+		if (rc == TM_OK) rc = live_commit_locked(fib, plan);
+	}
+	delete plan;
+	return rc;
+}
+
+void live_discard(live_plan * plan) { delete plan; }
+
+} // namespace tmx
+
+extern "C" {
+
 int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
 				  const uint32_t * keys, const uint32_t * filter_ids, uint32_t maxp) {
 	if (!fib) return set_error(TM_ERR_INVALID, "fib is NULL");
+	std::lock_guard<std::mutex> lk(fib->host_mu);
 	TM_CHECK(check_table(words, n, key_offsets, keys));
 	if (maxp == 0) return set_error(TM_ERR_INVALID, "maxp must be > 0");
 	if (fib->in_flight.load() > 0) return set_error(TM_ERR_BUSY, "a ring on this FIB has a submission in flight");
@@ -446,7 +654,7 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 	if (fib->h_order.size() != 192) return set_error(TM_ERR_STATE, "no table uploaded");
 	TM_CHECK(require_device());
 	bool bump = false;   // device arrays, P or K change: rings must re-read the FIB
-	const uint32_t e = fib->epoch + 1;
+	const uint32_t e = std::max(fib->epoch, fib->prep_epoch) + 1;
 	if (!fib->slot_epoch) {
 		// lane epochs from now on, so that later inserts can run live
 		TM_CHECK(dalloc(&fib->slot_epoch, (size_t)std::max<uint32_t>(fib->n_blocks, 1) * 64));
@@ -500,6 +708,8 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 		TM_CHECK(upload(&fib->key_dict, dict, b));
 		fib->h_dict = std::move(dict);
 		fib->n_keys = (uint32_t)fib->h_dict.size();
+		dfree(fib->d_complete);   // no filter holds the new keys yet
+		fib->n_complete = 0;
 		bump = true;
 	}
 	const bool inline_ok = fib->n_keys < KEY_NONE;
@@ -651,6 +861,8 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 	fib->n_filters += n;
 	fib->n_key_instances += nkeys;
 	fib->epoch = e;
+	fib->prep_epoch = e;
+	fib->ovf_valid = (uint32_t)fib->h_ovf.size();
 	if (bump) fib->version++;
 	return TM_OK;
 }
@@ -662,116 +874,20 @@ int tm_fib_insert_live(tm_fib * fib, const uint32_t * words, uint32_t n, const u
 	if (n == 0) return TM_OK;
 	if (fib->h_order.size() != 192) return set_error(TM_ERR_STATE, "no table uploaded");
 	TM_CHECK(require_device());
-	// 1. dry run: known keys, room in the overflow lists, a spare lane for
-	//    every filter; nothing changes unless all of them hold
-	const uint64_t nkeys = key_offsets[n];
-	for (uint64_t x = 0; x < nkeys; ++x)
-		if (!std::binary_search(fib->h_dict.begin(), fib->h_dict.end(), keys[x]))
-			return set_error(TM_ERR_BUSY, "key %u is new to the table: use tm_fib_insert", keys[x]);
-	const bool inline_ok = fib->n_keys < KEY_NONE;
-	size_t ovf_need = 0;
-	for (uint32_t i = 0; i < n; ++i) {
-		const uint32_t kn = key_offsets[i + 1] - key_offsets[i];
-		if (!inline_ok || kn > 4) ovf_need += 1 + kn;
-	}
-	if (fib->h_ovf.size() + ovf_need > fib->ovf_cap)
-		return set_error(TM_ERR_BUSY, "overflow key lists are full: use tm_fib_insert");
-	const uint32_t P = fib->n_parts;
-	std::vector<uint32_t> pmpop(P);
-	for (uint32_t p = 0; p < P; ++p) {
-		uint32_t c = 0;
-		for (int j = 0; j < 6; ++j) c += __builtin_popcount(fib->h_part_info[(size_t)p * 8 + j]);
-		pmpop[p] = c;
-	}
-	std::vector<uint32_t> size = fib->h_part_size, target(n);
-	auto choose = [&](const uint32_t * w, const std::vector<uint32_t> & sz) -> int64_t {
-		int64_t best = -1;
-		for (uint32_t p = 0; p < P; ++p) {
-			const uint32_t * m = &fib->h_part_info[(size_t)p * 8];
-			if ((m[0] & ~w[0]) | (m[1] & ~w[1]) | (m[2] & ~w[2]) | (m[3] & ~w[3]) | (m[4] & ~w[4]) | (m[5] & ~w[5]))
-				continue;
-			if (sz[p] >= (uint64_t)fib->h_part_cap[p] * 64) continue;
-			if (best < 0 || pmpop[p] > pmpop[best] || (pmpop[p] == pmpop[best] && sz[p] < sz[best])) best = p;
-		}
-		return best;
-	};
-	// first choices in parallel on the sizes before the batch, then in
-	// order: a choice that the batch itself has filled is made again
-	std::vector<int64_t> first(n);
-	parallel_ranges(n, [&](size_t b0, size_t b1) {
-		for (size_t i = b0; i < b1; ++i) first[i] = choose(words + i * 6, size);
-	});
-	for (uint32_t i = 0; i < n; ++i) {
-		int64_t best = first[i];
-		if (best >= 0 && size[best] >= (uint64_t)fib->h_part_cap[best] * 64) best = choose(words + (size_t)i * 6, size);
-		if (best < 0) return set_error(TM_ERR_BUSY, "filter %u fits no spare lane: use tm_fib_insert", i);
-		target[i] = (uint32_t)best;
-		size[best]++;
-	}
+	std::lock_guard<std::mutex> lk(fib->host_mu);
 	// Submissions made before lane epochs existed do not filter lanes by
 	// epoch: while one runs, a lane could become valid before its key slot
 	// is written.  Epochs are created only with nothing in flight.
-	if (!fib->slot_epoch && fib->in_flight.load() > 0)
-		return set_error(TM_ERR_BUSY, "the first live insert needs no submission in flight: use tm_fib_insert");
-	// 2. device state for epochs and the insert stream
-	if (!fib->ins_stream) TM_HIP(hipStreamCreateWithFlags(&fib->ins_stream, hipStreamNonBlocking));
-	hipStream_t s = fib->ins_stream;
 	if (!fib->slot_epoch) {
-		TM_CHECK(dalloc(&fib->slot_epoch, (size_t)std::max<uint32_t>(fib->n_blocks, 1) * 64));
-		hipLaunchKernelGGL(k_init_epochs, dim3((uint32_t)((fib->n_blocks * 64ull + 255) / 256)), dim3(256), 0, s,
-						   fib->blk_info, 0u, fib->n_blocks, fib->slot_epoch, 0u);
-		TM_HIP(hipGetLastError());
+		if (fib->in_flight.load() > 0)
+			return set_error(TM_ERR_BUSY, "the first live insert needs no submission in flight: use tm_fib_insert");
+		TM_CHECK(create_epochs(fib));
 	}
-	if (fib->recs_cap < n) {
-		if (fib->d_recs) {
-			TM_HIP(hipStreamSynchronize(s));
-			(void)hipFree(fib->d_recs);
-			fib->d_recs = nullptr;
-			fib->recs_cap = 0;
-		}
-		TM_HIP(hipMalloc(&fib->d_recs, sizeof(ins_rec) * (size_t)n * 2));
-		fib->recs_cap = (size_t)n * 2;
-	}
-	// 3. commit: lanes, host mirrors, then the device copies
-	const uint32_t e = fib->epoch + 1;
-	const size_t ovf_before = fib->h_ovf.size();
-	std::vector<ins_rec> recs(n);
-	for (uint32_t i = 0; i < n; ++i) {
-		const uint32_t p = target[i];
-		const uint32_t * w = words + (size_t)i * 6;
-		const uint32_t k = fib->h_part_size[p]++;
-		ins_rec & r = recs[i];
-		r.block = fib->h_part_info[(size_t)p * 8 + 6] + k / 64;
-		r.lane = k % 64;
-		r.fid = filter_ids ? filter_ids[i] : (uint32_t)(fib->n_filters + i);
-		r.pad = 0;
-		for (int j = 0; j < 6; ++j) r.w[j] = w[j];
-		TM_CHECK(key_slot(keys, key_offsets[i], key_offsets[i + 1], fib->h_dict, inline_ok, fib->h_ovf, r.kv));
-		fib->h_part_info[(size_t)p * 8 + 7] = (fib->h_part_size[p] + 63) / 64;
-		uint32_t * ps = &fib->h_part_stat[(size_t)p * 8];
-		for (int j = 0; j < 6; ++j) {
-			ps[j] |= w[j];
-			ps[7] += (uint32_t)__builtin_popcount(w[j]);
-		}
-		ps[6] += 1;
-		fib->max_part = std::max<uint64_t>(fib->max_part, fib->h_part_size[p]);
-	}
-	if (fib->h_ovf.size() > ovf_before)
-		TM_HIP(hipMemcpyAsync(fib->ovf + ovf_before, fib->h_ovf.data() + ovf_before,
-							  (fib->h_ovf.size() - ovf_before) * 4, hipMemcpyHostToDevice, s));
-	TM_HIP(hipMemcpyAsync(fib->d_recs, recs.data(), sizeof(ins_rec) * n, hipMemcpyHostToDevice, s));
-	hipLaunchKernelGGL(k_insert_lanes, dim3((n + 255) / 256), dim3(256), 0, s, (const ins_rec *)fib->d_recs, n,
-					   fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid, fib->slot_epoch, e);
-	TM_HIP(hipGetLastError());
-	// block counts and statistics after the lanes: a submission that sees a
-	// new block count sees lanes that its epoch hides
-	TM_HIP(hipMemcpyAsync(fib->part_info, fib->h_part_info.data(), (size_t)P * 32, hipMemcpyHostToDevice, s));
-	TM_HIP(hipMemcpyAsync(fib->part_stat, fib->h_part_stat.data(), (size_t)P * 32, hipMemcpyHostToDevice, s));
-	TM_HIP(hipStreamSynchronize(s));
-	fib->n_filters += n;
-	fib->n_key_instances += nkeys;
-	fib->epoch = e;
-	return TM_OK;
+	tmx::live_plan * plan = nullptr;
+	TM_CHECK(tmx::live_prepare_locked(fib, words, n, key_offsets, keys, filter_ids, &plan));
+	const int rc = tmx::live_commit_locked(fib, plan);
+	delete plan;
+	return rc;
 }
 
 } // extern "C"

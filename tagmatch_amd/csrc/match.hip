@@ -563,14 +563,12 @@ k_match_q(mq_args a) {
 					// their rows into gbm and flag a query whose merged row is
 					// full (one coalesced load per tile here); in one segment
 					// the LDS row is the query's whole bitmap.
-					bool full;
-					if (a.nseg > 1) {
-						const uint32_t qrow = q0 + t * 64 + lane;
-						full = qrow < q_end &&
-							   __hip_atomic_load(&a.qfull[qrow], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-					} else {
-						full = row_full(&s_bm[(t * 64 + lane) * kw], kw, lane, a.last_mask);
-					}
+					// Flags also mark the queries that contain a complete filter
+					// (k_complete_flags), whose rows are already full in gbm.
+					const uint32_t qrow = q0 + t * 64 + lane;
+					bool full = a.qfull && qrow < q_end &&
+								__hip_atomic_load(&a.qfull[qrow], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+					if (a.nseg <= 1) full = full || row_full(&s_bm[(t * 64 + lane) * kw], kw, lane, a.last_mask);
 					qsel[t] &= ~__ballot(full);
 					anyq |= qsel[t];
 				}
@@ -822,10 +820,14 @@ k_match_q(mq_args a) {
 						all &= u;
 					}
 					if (a.qfull && __ballot(all != 0xFFFFFFFFu) == 0 && lane == 0)
-						__hip_atomic_store(&a.qfull[q0 + j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+						atomicMax(&a.qfull[q0 + j], 1u);
 				}
 			} else {
-				for (uint32_t w = tid; w < nw; w += NT) dst[w] = s_bm[w];
+				// rows of complete queries (flag 2) were filled before the launch
+				for (uint32_t w = tid; w < nw; w += NT) {
+					if (a.qfull && a.qfull[q0 + w / kw] == 2u) continue;
+					dst[w] = s_bm[w];
+				}
 			}
 		}
 		if (tid == 0) {
@@ -1047,6 +1049,36 @@ k_seg_bounds(const uint64_t * __restrict__ prefix, uint32_t nb, uint32_t nseg, u
 		if (prefix[mid] >= target) hi = mid; else lo = mid + 1;
 	}
 	segb[k] = lo;
+}
+
+// Queries that contain a complete filter (its key set is the whole
+// dictionary) match every key, whatever else they match: flag them with 2,
+// clear every other flag.  Exact for KEYS_UNIQUE.
+__global__ void __launch_bounds__(256)
+k_complete_flags(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ cf, uint32_t ncf,
+				 uint32_t * __restrict__ qfull) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= nq) return;
+	const uint4 x = *reinterpret_cast<const uint4 *>(q + (size_t)i * 8);
+	const uint2 y = *reinterpret_cast<const uint2 *>(q + (size_t)i * 8 + 4);
+	bool hit = false;
+	for (uint32_t c = 0; c < ncf; ++c) {
+		const uint4 f = sld4(cf + (size_t)c * 8);
+		const uint2 g = sld2(cf + (size_t)c * 8 + 4);
+		hit = hit || ((f.x & ~x.x) | (f.y & ~x.y) | (f.z & ~x.z) | (f.w & ~x.w) | (g.x & ~y.x) | (g.y & ~y.y)) == 0;
+	}
+	qfull[i] = hit ? 2u : 0u;
+}
+
+// Bitmap rows of the flagged queries: every key (the last word's valid bits).
+__global__ void __launch_bounds__(256)
+k_complete_rows(uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t last_mask,
+				const uint32_t * __restrict__ qfull) {
+	const uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+	if (w >= (uint64_t)nq * kw) return;
+	const uint32_t qi = (uint32_t)(w / kw);
+	if (qfull[qi] != 2u) return;
+	gbm[w] = w - (uint64_t)qi * kw == kw - 1 ? last_mask : 0xFFFFFFFFu;
 }
 
 // ------------------------------------------------------------- kernel table
@@ -1398,9 +1430,20 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 						   r->d_segb);
 	}
 	r->last_nseg = nseg;
+	const bool exit_on = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
+	const uint32_t last_mask = (r->fib->n_keys & 31) ? (1u << (r->fib->n_keys & 31)) - 1u : 0xFFFFFFFFu;
+	const bool complete_pass = exit_on && r->fib->n_complete > 0 && nq > 0;
 	if (nseg > 1 && r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
 		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
-		fill_u32(r->d_qfull, 0, nq, s);
+		if (!complete_pass) fill_u32(r->d_qfull, 0, nq, s);
+	}
+	if (complete_pass) {
+		// queries that contain a complete filter: every key, nothing to test
+		hipLaunchKernelGGL(k_complete_flags, dim3((nq + 255) / 256), dim3(256), 0, s, r->d_q, nq, r->fib->d_complete,
+						   r->fib->n_complete, r->d_qfull);
+		const uint64_t nw = (uint64_t)nq * r->kw;
+		hipLaunchKernelGGL(k_complete_rows, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, r->d_gbm, r->kw, nq,
+						   last_mask, r->d_qfull);
 	}
 	hipLaunchKernelGGL(k_init_queues, dim3(1), dim3(64), 0, s, items, r->d_qbeg, r->d_qnext);
 	mq_args a;
@@ -1410,8 +1453,8 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.nq = nq;
 	a.porder = r->have_order ? r->d_porder : nullptr;
 	a.iorder = r->use_iorder ? r->d_iorder : nullptr;
-	a.full_exit = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
-	a.last_mask = (r->fib->n_keys & 31) ? (1u << (r->fib->n_keys & 31)) - 1u : 0xFFFFFFFFu;
+	a.full_exit = exit_on;
+	a.last_mask = last_mask;
 	a.slot_epoch = r->fib->slot_epoch;
 	a.epoch = r->epoch;
 	a.ntiles = (uint32_t)tiles;
@@ -1426,7 +1469,8 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.qbeg = r->d_qbeg;
 	a.pcount = r->d_pcount;
 	a.gbm = r->d_gbm;
-	a.qfull = r->d_qfull;
+	// flags hold this submission's state: cleared or set by the pass above
+	a.qfull = (nseg > 1 || complete_pass) ? r->d_qfull : nullptr;
 	a.hits = r->d_hits;
 	a.hit_cap = r->hit_cap;
 	a.counters = r->d_cnt;

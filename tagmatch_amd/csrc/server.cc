@@ -26,10 +26,15 @@
 #include <unordered_map>
 #include <vector>
 
-namespace tmx {   // hip_util.hip
+namespace tmx {   // hip_util.hip, fib.hip
 int set_error(int code, const char * fmt, ...);   // text for tm_last_error()
 void * pinned_alloc(size_t bytes);                // pinned host memory, NULL on failure
 void pinned_free(void * p);
+struct live_plan;                                 // a live insert prepared on the host
+int live_prepare(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32_t * key_offsets,
+				 const uint32_t * keys, const uint32_t * filter_ids, live_plan ** out);
+int live_commit(tm_fib * fib, live_plan * plan);  // frees the plan
+void live_discard(live_plan * plan);
 }
 
 namespace {
@@ -49,6 +54,7 @@ struct request {
 	std::vector<uint32_t> ins_words, ins_koff, ins_keys, ins_fid;
 	uint32_t ins_n = 0, ins_maxp = 0;
 	bool ins_tried_live = false;      // the live insert declined: apply at a barrier
+	tmx::live_plan * plan = nullptr;  // prepared when submitted: the engine only commits it
 	// results: offsets of its own, values a slice of its step's buffer
 	std::vector<uint64_t> off;
 	std::shared_ptr<struct result_buf> rbuf;
@@ -112,6 +118,8 @@ struct tm_server {
 	int fatal = TM_OK;
 	std::string fatal_msg;
 	std::shared_ptr<buf_pool> pool = std::make_shared<buf_pool>();
+	std::mutex ins_mu;                // orders the preparation of inserts
+	uint32_t unprepared = 0;          // queued inserts without a plan (they change the layout)
 };
 
 namespace {
@@ -285,7 +293,26 @@ void engine(tm_server * s) {
 		// inserts: live (beside the steps in flight, which do not see it) when
 		// the batch fits spare lanes, else at a barrier once every earlier
 		// request has completed
-		if (!s->waiting.empty() && s->waiting.front()->is_insert && !s->waiting.front()->ins_tried_live) {
+		if (!s->waiting.empty() && s->waiting.front()->is_insert && s->waiting.front()->plan) {
+			// prepared when submitted: only the device half is left
+			request * r = s->waiting.front();
+			s->waiting.pop_front();
+			tmx::live_plan * plan = r->plan;
+			r->plan = nullptr;
+			lk.unlock();
+			int rc = tmx::live_commit(s->fib, plan);
+			const std::string err = rc == TM_OK ? std::string() : std::string(tm_last_error());
+			lk.lock();
+			r->status = rc;
+			r->error = err;
+			s->st.n_inserts++;
+			s->st.n_live_inserts += rc == TM_OK;
+			finish(s, r);
+			s->cv_done.notify_all();
+			progressed = true;
+		}
+		if (!s->waiting.empty() && s->waiting.front()->is_insert && !s->waiting.front()->plan &&
+			!s->waiting.front()->ins_tried_live) {
 			request * r = s->waiting.front();
 			lk.unlock();
 			int rc = tm_fib_insert_live(s->fib, r->ins_words.data(), r->ins_n, r->ins_koff.data(),
@@ -296,6 +323,7 @@ void engine(tm_server * s) {
 				r->ins_tried_live = true;   // needs the barrier
 			} else {
 				s->waiting.pop_front();
+				s->unprepared--;
 				r->status = rc;
 				r->error = err;
 				s->st.n_inserts++;
@@ -316,6 +344,7 @@ void engine(tm_server * s) {
 									   r->ins_fid.empty() ? nullptr : r->ins_fid.data(), r->ins_maxp);
 				const std::string err = rc == TM_OK ? std::string() : std::string(tm_last_error());
 				lk.lock();
+				s->unprepared--;
 				r->status = rc;
 				r->error = err;
 				s->st.n_inserts++;
@@ -397,7 +426,10 @@ int tm_server_destroy(tm_server * s) {
 	if (s->th.joinable()) s->th.join();
 	for (auto & sl : s->slots)
 		if (sl.ring) tm_ring_destroy(sl.ring);
-	for (auto & kv : s->reqs) delete kv.second;
+	for (auto & kv : s->reqs) {
+		if (kv.second->plan) tmx::live_discard(kv.second->plan);
+		delete kv.second;
+	}
 	delete s;
 	return TM_OK;
 }
@@ -456,11 +488,27 @@ int tm_server_insert(tm_server * s, const uint32_t * words, uint32_t n, const ui
 	if (filter_ids) r->ins_fid.assign(filter_ids, filter_ids + n);
 	if (r->ins_keys.empty()) r->ins_keys.push_back(0);
 	r->t_submit = clock_t_::now();
+	// Prepare the insert now, in the caller's thread (partition choice and
+	// lane records on the host), unless an insert that may change the
+	// table's layout is queued ahead of it; the engine then only commits it
+	// to the device when it reaches the front of the queue.
+	std::lock_guard<std::mutex> og(s->ins_mu);
+	bool can_prepare;
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		can_prepare = s->unprepared == 0;
+	}
+	if (can_prepare) {
+		int rc = tmx::live_prepare(s->fib, r->ins_words.data(), n, r->ins_koff.data(), r->ins_keys.data(),
+								   r->ins_fid.empty() ? nullptr : r->ins_fid.data(), &r->plan);
+		if (rc != TM_OK) r->plan = nullptr;   // declined (TM_ERR_BUSY) or invalid: the engine path decides
+	}
 	{
 		std::lock_guard<std::mutex> lk(s->mu);
 		r->id = s->next_id++;
 		s->reqs[r->id] = r;
 		s->waiting.push_back(r);
+		if (!r->plan) s->unprepared++;
 	}
 	s->cv_engine.notify_one();
 	*id = r->id;

@@ -173,6 +173,39 @@ def test_inner_loop_variants_vs_oracle(gpu, variant):
                       batch_size=64 if variant % 2 else 128)
 
 
+def test_complete_filter_pass(gpu):
+    """A filter whose key set is the whole dictionary answers every query
+    that contains it: such queries skip the match.  Results must equal the
+    oracle's with the pass (one segment and 8-block segments) and without it
+    (full_exit 2); the pass must also cut the hits processed."""
+    cw, cko, ck = tm.workload(60000, 1, seed=971)[:3]
+    K = np.unique(ck)
+    # two complete filters with 3 and 5 bits, the rest unchanged
+    extra = np.zeros((2, 6), np.uint32)
+    extra[0, 0] = 0b111
+    extra[1, 2] = 0b11111 << 7
+    words = np.concatenate([cw, extra])
+    klists = [ck[cko[i]:cko[i + 1]] for i in range(len(cw))] + [K, K[::-1]]
+    ko = np.zeros(len(words) + 1, np.uint64)
+    np.cumsum([len(k) for k in klists], out=ko[1:])
+    keys = np.concatenate(klists).astype(np.uint32)
+    cw2, cko2, ck2 = tm.canonicalize(words, ko, keys)
+    rng = np.random.default_rng(972)
+    q = rng.integers(0, 2**32, size=(3000, 6), dtype=np.uint64).astype(np.uint32)
+    q[::3, 0] |= 0b111                   # a third contain the first complete filter
+    q[1::7, 2] |= 0b11111 << 7
+    fib, *_ = tm.build_fib(cw2, cko2, ck2, 8000)
+    eo, ev = orc.match(cw2, cko2, ck2, q, mode=0)
+    for seg in (1 << 30, 8):
+        res = {}
+        for fe in (1, 2):
+            o, v, st = tm.match(fib, q, full_exit=fe, chunk_blocks=seg)
+            assert_same(o, v, eo, ev, sort_got=False, what=f"full_exit {fe} segment {seg}")
+            res[fe] = st["n_hits"]
+        log(f"complete pass, segment {seg}: hits processed {res[1]} with it, {res[2]} without")
+        assert res[1] < res[2]
+
+
 @pytest.mark.parametrize("n_keys", [5, 37, 64, 100])
 def test_full_key_set_exit(gpu, n_keys):
     """With few distinct keys most queries collect every key early; the exit
