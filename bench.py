@@ -67,7 +67,7 @@ def parse():
                     help="batches per partition in the uneven-batching leg (0: skip the leg)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds per serving load level (0: skip)")
     ap.add_argument("--serve-request", type=int, default=64, help="queries per serving request")
-    ap.add_argument("--serve-slots", type=int, default=4, help="serving slots (device steps in flight)")
+    ap.add_argument("--serve-slots", type=int, default=3, help="serving slots (device steps in flight)")
     ap.add_argument("--serve-slot-queries", type=int, default=1024,
                     help="queries per serving step at most (small steps keep the latency low)")
     ap.add_argument("--serve-insert", type=int, default=10000, help="filters per insert (0: none)")
@@ -118,14 +118,20 @@ def cpu_baseline(cw, off_parts, masks, perm_words, q, budget_s):
             "hits_per_query": hits / n2}
 
 
-def traffic_from_profiles():
-    p = os.path.join(ROOT, "profiles", "match_traffic.json")
-    if os.path.exists(p):
-        try:
-            return json.load(open(p)).get("bytes_per_launch")
-        except Exception:
-            return None
-    return None
+def profile_figures():
+    """HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)
+    and VALU instructions per launch (SQ_INSTS_VALU) of the match kernel in
+    the default step and in the step without the full-key-set exit, from
+    the summaries under profiles/ (empty when absent)."""
+    out = {}
+    for name in ("match_traffic.json", "match_counters.json"):
+        p = os.path.join(ROOT, "profiles", name)
+        if os.path.exists(p):
+            try:
+                out.update(json.load(open(p)))
+            except Exception:
+                pass
+    return out
 
 
 def unit_logs(fib, q, batches):
@@ -409,10 +415,16 @@ def main():
     kernel_s = (np.mean(match_ms) / 1e3) if match_ms[-1] > 0 else ms_per_step / 1e3
     hbm_gbs = bytes_alg / kernel_s / 1e9
     valu_tops = OPS_PER_PAIR * pair_tests / kernel_s / 1e12
+    prof = profile_figures()
+    traffic = prof.get("bytes_per_launch")
     if no_exit["match_ms"] > 0:
-        no_exit["roofline_hbm_frac"] = bytes_alg / (no_exit["match_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
-        no_exit["roofline_valu_frac"] = OPS_PER_PAIR * pair_tests / (no_exit["match_ms"] / 1e3) / 1e12 / VALU_PEAK_TOPS
-    traffic = traffic_from_profiles()
+        ne_s = no_exit["match_ms"] / 1e3
+        no_exit["roofline_hbm_frac"] = bytes_alg / ne_s / 1e9 / HBM_PEAK_GBS
+        no_exit["roofline_valu_frac"] = OPS_PER_PAIR * pair_tests / ne_s / 1e12 / VALU_PEAK_TOPS
+        if prof.get("no_full_exit_bytes_per_launch"):
+            no_exit["measured_hbm_frac"] = prof["no_full_exit_bytes_per_launch"] / ne_s / 1e9 / HBM_PEAK_GBS
+        if prof.get("no_full_exit_valu_insts"):
+            no_exit["measured_valu_frac"] = prof["no_full_exit_valu_insts"] * 64 / ne_s / 1e12 / VALU_PEAK_TOPS
 
     # the plain single-launch form on the same queries (reference dispatch shape)
     plain = None
@@ -482,8 +494,16 @@ def main():
         "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_match", "kernel_ms": kernel_s * 1e3,
-                     "bytes_per_launch": bytes_alg},
+                     "kernel": "k_match_q", "kernel_ms": kernel_s * 1e3,
+                     "bytes_per_launch": bytes_alg,
+                     "measured_frac": (traffic / kernel_s / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                     "measured_valu_frac": (prof["valu_insts"] * 64 / kernel_s / 1e12 / VALU_PEAK_TOPS)
+                     if prof.get("valu_insts") else None,
+                     "note": "achieved = algorithmic bytes of the (partition, 256-query batch) units the launch "
+                             "covers, as brute force reads them; block masks, the full-key-set exit and complete "
+                             "filters skip most of that work exactly, so frac above 1 is work avoided. "
+                             "measured_frac = HBM traffic (rocprofv3) / kernel time / peak; measured_valu_frac "
+                             "= SQ_INSTS_VALU x 64 / kernel time / 39.3 T"},
         "roofline_valu": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
                           "frac": valu_tops / VALU_PEAK_TOPS, "pair_tests_per_launch": pair_tests,
                           "note": "algorithmic int ops (7 per pair-test over selected partitions); "

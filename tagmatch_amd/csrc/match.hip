@@ -42,6 +42,8 @@ constexpr int WG = 256;          // threads per match workgroup (4 waves)
 constexpr int NQUEUE = 8;        // work queues, one per XCD of an MI355X
 constexpr int QPAD = 32;         // queue cursors 256 B apart: atomics on one line serialize
 constexpr uint32_t SEG_MAX_BLOCKS = 1024;   // persistent: largest block segment (measured at 10M)
+constexpr uint32_t SEG_MAX_BLOCKS_EXIT = 4096;   // the same with the full-key-set exit: most items
+                                                 // skip their blocks, and fewer items cost less
 constexpr uint32_t WSEG_BASE = 64;          // block weight: fixed part, in units of one expected hit
 constexpr uint32_t WSEG_MAX_TILES = 256;    // persistent: tiles up to which segments are cut by weight
 constexpr uint32_t NO_QUERY = 0xFFFFFFFFu;
@@ -880,32 +882,30 @@ k_bm_count(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t
 }
 
 // KEYS_UNIQUE: one wave per query writes its keys in ascending id order,
-// which is ascending key order (the dictionary is sorted).
+// which is ascending key order (the dictionary is sorted).  Lanes take 64
+// consecutive key ids at a time: a set bit's output slot is the count of set
+// bits below it (mbcnt), so each store writes consecutive values (rows are
+// mostly dense) and the dictionary reads are coalesced.
 __global__ void __launch_bounds__(256)
 k_bm_write(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint64_t * __restrict__ off,
 		   const uint32_t * __restrict__ dict, uint32_t * __restrict__ vals, uint64_t cap) {
 	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
 	if (qi >= nq) return;
+	const uint32_t * row = gbm + (size_t)qi * kw;
 	uint64_t pos = off[qi];
-	for (uint32_t wb = 0; wb < kw; wb += 64) {
-		const uint32_t w = wb + lane;
-		uint32_t v = w < kw ? gbm[(size_t)qi * kw + w] : 0;
-		uint32_t c = __popc(v), x = c;
-		for (int o = 1; o < 64; o <<= 1) {
-			uint32_t y = __shfl_up(x, o, 64);
-			if (lane >= (uint32_t)o) x += y;
-		}
-		uint32_t tot = __shfl(x, 63, 64);
-		if (v) {
-			uint64_t p = pos + x - c;
-			while (v) {
-				uint32_t bit = __builtin_ctz(v);
-				v &= v - 1;
-				if (p < cap) vals[p] = dict[w * 32 + bit];
-				++p;
+	for (uint32_t wb = 0; wb < kw; wb += 64) {   // 64 words (2048 ids) per pass
+		const uint32_t v = wb + lane < kw ? row[wb + lane] : 0u;
+		const uint32_t nw = min(64u, kw - wb);
+		for (uint32_t c = 0; c < nw; c += 2) {
+			const uint64_t m = (uint64_t)(uint32_t)__shfl(v, (int)c, 64) |
+							   ((uint64_t)(uint32_t)__shfl(v, (int)c + 1, 64) << 32);
+			if (m == 0) continue;
+			if ((m >> lane) & 1) {
+				const uint64_t p = pos + mbcnt64(m);
+				if (p < cap) vals[p] = dict[(wb + c) * 32 + lane];
 			}
+			pos += (uint64_t)__popcll(m);
 		}
-		pos += tot;
 	}
 }
 
@@ -1404,7 +1404,9 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	const bool can_cut = (r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE) && !r->use_iorder && !r->cfg.log_units;
 	if (can_cut && tiles > 0) {
 		nseg = (uint32_t)std::min<uint64_t>((2 * resident + tiles - 1) / tiles, std::max<uint32_t>(nblk / 16, 1));
-		nseg = std::max<uint32_t>(nseg, (nblk + SEG_MAX_BLOCKS - 1) / SEG_MAX_BLOCKS);
+		const uint32_t seg_max = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->cfg.full_exit != 2
+									 ? SEG_MAX_BLOCKS_EXIT : SEG_MAX_BLOCKS;
+		nseg = std::max<uint32_t>(nseg, (nblk + seg_max - 1) / seg_max);
 	}
 	if (r->cfg.chunk_blocks && (r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE))   // fixed segment size
 		nseg = std::max<uint32_t>(1, (nblk + r->cfg.chunk_blocks - 1) / r->cfg.chunk_blocks);
@@ -1700,7 +1702,11 @@ int ensure_capacity(tm_ring * r) {
 		TM_CHECK(grow_u32(&r->d_list, r->list_cap, cap));
 	}
 	if (!r->d_vals) {
+		// KEYS_UNIQUE: every key for every query when that is at most 64M
+		// values (no overflow retry then), else a first guess that tm_wait grows
 		uint64_t per = r->cfg.mode == TM_MODE_KEYS_UNIQUE ? std::min<uint32_t>(r->fib->n_keys, 256) : 64;
+		if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && (uint64_t)r->max_q * r->fib->n_keys <= (1ull << 26))
+			per = std::max<uint32_t>(r->fib->n_keys, 1);
 		TM_CHECK(grow_u32(&r->d_vals, r->val_cap, (uint64_t)r->max_q * std::max<uint64_t>(per, 1)));
 	}
 	if (r->cfg.log_units && !r->d_log) {

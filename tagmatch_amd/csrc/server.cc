@@ -119,6 +119,7 @@ struct tm_server {
 	std::string fatal_msg;
 	std::shared_ptr<buf_pool> pool = std::make_shared<buf_pool>();
 	std::mutex ins_mu;                // orders the preparation of inserts
+	std::atomic<uint64_t> arrivals{0};   // requests and inserts submitted so far
 	uint32_t unprepared = 0;          // queued inserts without a plan (they change the layout)
 };
 
@@ -361,8 +362,27 @@ void engine(tm_server * s) {
 		if (!progressed) {
 			bool any_busy = false;
 			for (auto & sl : s->slots) any_busy = any_busy || sl.busy;
-			// poll device completions quickly; sleep on the queue when idle
-			s->cv_engine.wait_for(lk, any_busy ? std::chrono::microseconds(20) : std::chrono::milliseconds(5));
+			if (any_busy) {
+				// steps in flight: poll their completions for a short while
+				// without sleeping (a timed wait oversleeps by tens of
+				// microseconds), then yield the lock briefly
+				lk.unlock();
+				bool any_free = false;
+				for (auto & sl : s->slots) any_free = any_free || !sl.busy;
+				const uint64_t seen = s->arrivals.load();
+				const auto until = clock_t_::now() + std::chrono::microseconds(200);
+				bool ready = false;
+				while (!ready && clock_t_::now() < until) {
+					for (auto & sl : s->slots)
+						if (sl.busy && tm_poll(sl.ring) != TM_ERR_BUSY) ready = true;
+					if (any_free && s->arrivals.load() != seen) ready = true;   // a request for a free slot
+					if (!ready) std::this_thread::yield();
+				}
+				lk.lock();
+				if (!ready && s->waiting.empty()) s->cv_engine.wait_for(lk, std::chrono::microseconds(20));
+			} else {
+				s->cv_engine.wait_for(lk, std::chrono::milliseconds(5));   // idle: sleep on the queue
+			}
 		}
 	}
 }
@@ -470,6 +490,7 @@ int tm_server_submit(tm_server * s, const uint32_t * queries, uint32_t nq, uint6
 		s->waiting.push_back(r);
 		s->st.n_requests++;
 	}
+	s->arrivals.fetch_add(1);
 	s->cv_engine.notify_one();
 	*id = r->id;
 	return TM_OK;
@@ -510,6 +531,7 @@ int tm_server_insert(tm_server * s, const uint32_t * words, uint32_t n, const ui
 		s->waiting.push_back(r);
 		if (!r->plan) s->unprepared++;
 	}
+	s->arrivals.fetch_add(1);
 	s->cv_engine.notify_one();
 	*id = r->id;
 	return TM_OK;

@@ -206,7 +206,12 @@ def test_presize_avoids_retry_and_keeps_results(trained, mode):
     om, vm = ring.results()
     log(f"mode {mode}: no model overflow={sb.overflow} {sb.device_ms:.2f} ms; "
         f"model overflow={sm.overflow} {sm.device_ms:.2f} ms pred {sm.pred_hits:.0f} actual {sm.n_hits}")
-    assert sb.overflow == 1
+    # a KEYS_UNIQUE ring whose worst case (queries x keys) fits 64M values
+    # is sized for it and never retries; the hit-list modes need the model
+    if mode == tm.MODE_KEYS_UNIQUE and len(q) * d["fib"].stats()["n_distinct_keys"] <= (1 << 26):
+        assert sb.overflow == 0
+    else:
+        assert sb.overflow == 1
     assert sm.overflow == 0
     assert np.array_equal(ob, om)
     if mode == tm.MODE_KEYS_UNIQUE:
