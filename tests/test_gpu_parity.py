@@ -173,6 +173,29 @@ def test_inner_loop_variants_vs_oracle(gpu, variant):
                       batch_size=64 if variant % 2 else 128)
 
 
+def test_long_key_lists(gpu):
+    """Filters with 17 to 300 keys (overflow lists that the whole wave walks
+    together in KEYS_UNIQUE) in every output mode, one segment and 8-block
+    segments, against the oracle."""
+    cw, cko, ck = tm.workload(40000, 1, seed=981)[:3]
+    rng = np.random.default_rng(982)
+    pool = np.unique(rng.integers(0, 2**31, 5000).astype(np.uint32))
+    klists = []
+    for i in range(len(cw)):
+        n = int(rng.integers(17, 301)) if i % 5 == 0 else int(rng.integers(1, 5))
+        klists.append(np.sort(rng.choice(pool, n, replace=False)))
+    ko = np.zeros(len(cw) + 1, np.uint64)
+    np.cumsum([len(k) for k in klists], out=ko[1:])
+    keys = np.concatenate(klists).astype(np.uint32)
+    q = rng.integers(0, 2**32, size=(1500, 6), dtype=np.uint64).astype(np.uint32)
+    fib, *_ = tm.build_fib(cw, ko, keys, 5000)
+    for mode in (0, 1, 2):
+        eo, ev = orc.match(cw, ko, keys, q, mode=mode)
+        for seg in (1 << 30, 8):
+            o, v, st = tm.match(fib, q, mode=mode, chunk_blocks=seg, full_exit=2)
+            assert_same(o, v, eo, ev, sort_got=mode != 0, what=f"mode {mode} segment {seg}")
+
+
 def test_complete_filter_pass(gpu):
     """A filter whose key set is the whole dictionary answers every query
     that contains it: such queries skip the match.  Results must equal the
