@@ -49,7 +49,8 @@ struct dev_fib_view {
 
 constexpr uint32_t BLOCK_WORDS = 6 * 64;
 constexpr uint32_t KEY_NONE = 0xFFFFu;
-constexpr uint32_t MAX_COMPLETE = 64;
+constexpr uint32_t MAX_RICH = 1024;
+constexpr uint32_t RICH_MIN_KEYS = 32;
 
 // A live insert prepared on the host (partition choice, lane records, host
 // copies updated), committed to the device later by the thread that makes
@@ -96,11 +97,13 @@ struct tm_fib {
 	// (tmx::live_prepare / live_commit): host copies change at prepare,
 	// under host_mu; the device and `epoch` at commit, in prepare order
 	std::mutex host_mu;
-	// filters whose key set is the whole dictionary (at most MAX_COMPLETE,
-	// from the upload; cleared when an insert grows the dictionary): a query
-	// that contains one matches every key
-	uint32_t * d_complete = nullptr;     // n_complete x 8 words (6 + padding)
-	uint32_t n_complete = 0;
+	// rich filters: those with the most keys (at least RICH_MIN_KEYS, at most
+	// MAX_RICH, from the upload; dropped when an insert grows the dictionary),
+	// each with its key set as a dense bitmap.  A query that contains one
+	// has all of its keys, whatever else it matches.
+	uint32_t * d_rich_words = nullptr;   // n_rich x 8 words (6 + padding)
+	uint32_t * d_rich_bm = nullptr;      // n_rich x kw words
+	uint32_t n_rich = 0;
 	uint32_t prep_epoch = 0;             // epoch of the last prepared insert
 	uint32_t ovf_valid = 1;              // words of ovf written on the device
 

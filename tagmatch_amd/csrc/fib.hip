@@ -32,8 +32,9 @@ void tm_fib::release() {
 	dfree(part_info);
 	dfree(part_stat);
 	dfree(slot_epoch);
-	dfree(d_complete);
-	n_complete = 0;
+	dfree(d_rich_words);
+	dfree(d_rich_bm);
+	n_rich = 0;
 	epoch = 0;
 	n_parts = n_blocks = n_keys = 0;
 	n_filters = n_key_instances = device_bytes = max_part = 0;
@@ -408,16 +409,29 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 		maxp = std::max<uint64_t>(maxp, h_psize[p]);
 	}
 
-	// complete filters: a key list of K distinct keys is the whole dictionary
-	std::vector<uint32_t> complete;
-	for (uint32_t i = 0; i < n && complete.size() < MAX_COMPLETE * 8; ++i) {
-		if (key_offsets[i + 1] - key_offsets[i] != K || K == 0) continue;
-		std::vector<uint32_t> ks(keys + key_offsets[i], keys + key_offsets[i + 1]);
-		std::sort(ks.begin(), ks.end());
-		if (std::unique(ks.begin(), ks.end()) != ks.end()) continue;
-		for (int j = 0; j < 6; ++j) complete.push_back(words[(size_t)i * 6 + j]);
-		complete.push_back(0);
-		complete.push_back(0);
+	// rich filters: the MAX_RICH with the most keys (at least RICH_MIN_KEYS),
+	// their words and their key sets as dense bitmaps
+	std::vector<uint32_t> rich_words, rich_bm;
+	{
+		std::vector<std::pair<uint32_t, uint32_t>> cand;   // (keys, filter)
+		for (uint32_t i = 0; i < n; ++i) {
+			const uint32_t kn = key_offsets[i + 1] - key_offsets[i];
+			if (kn >= RICH_MIN_KEYS) cand.push_back({kn, i});
+		}
+		const size_t nr = std::min<size_t>(cand.size(), MAX_RICH);
+		std::partial_sort(cand.begin(), cand.begin() + nr, cand.end(),
+						  [](const auto & x, const auto & y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });
+		const uint32_t kwr = (K + 31) / 32;
+		rich_words.assign(nr * 8, 0);
+		rich_bm.assign(nr * kwr, 0);
+		for (size_t r = 0; r < nr; ++r) {
+			const uint32_t i = cand[r].second;
+			for (int j = 0; j < 6; ++j) rich_words[r * 8 + j] = words[(size_t)i * 6 + j];
+			for (uint32_t x = key_offsets[i]; x < key_offsets[i + 1]; ++x) {
+				const uint32_t id = (uint32_t)(std::lower_bound(dict.begin(), dict.end(), keys[x]) - dict.begin());
+				rich_bm[r * kwr + id / 32] |= 1u << (id % 32);
+			}
+		}
 	}
 
 	fib->release();
@@ -433,8 +447,11 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	TM_CHECK(upload(&fib->key_dict, dict, bytes));
 	TM_CHECK(upload(&fib->part_info, bb.part, bytes));
 	TM_CHECK(upload(&fib->part_stat, bb.stat, bytes));
-	if (!complete.empty()) TM_CHECK(upload(&fib->d_complete, complete, bytes));
-	fib->n_complete = (uint32_t)(complete.size() / 8);
+	if (!rich_words.empty()) {
+		TM_CHECK(upload(&fib->d_rich_words, rich_words, bytes));
+		TM_CHECK(upload(&fib->d_rich_bm, rich_bm, bytes));
+	}
+	fib->n_rich = (uint32_t)(rich_words.size() / 8);
 	fib->n_parts = np;
 	fib->n_blocks = bb.nb;
 	fib->n_keys = K;
@@ -708,8 +725,9 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 		TM_CHECK(upload(&fib->key_dict, dict, b));
 		fib->h_dict = std::move(dict);
 		fib->n_keys = (uint32_t)fib->h_dict.size();
-		dfree(fib->d_complete);   // no filter holds the new keys yet
-		fib->n_complete = 0;
+		dfree(fib->d_rich_words);   // the bitmaps hold the old dense ids
+		dfree(fib->d_rich_bm);
+		fib->n_rich = 0;
 		bump = true;
 	}
 	const bool inline_ok = fib->n_keys < KEY_NONE;

@@ -565,8 +565,8 @@ k_match_q(mq_args a) {
 					// their rows into gbm and flag a query whose merged row is
 					// full (one coalesced load per tile here); in one segment
 					// the LDS row is the query's whole bitmap.
-					// Flags also mark the queries that contain a complete filter
-					// (k_complete_flags), whose rows are already full in gbm.
+					// Flags also mark the queries whose rows the rich-filter pass
+					// (k_rich_rows) already filled.
 					const uint32_t qrow = q0 + t * 64 + lane;
 					bool full = a.qfull && qrow < q_end &&
 								__hip_atomic_load(&a.qfull[qrow], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
@@ -825,10 +825,12 @@ k_match_q(mq_args a) {
 						atomicMax(&a.qfull[q0 + j], 1u);
 				}
 			} else {
-				// rows of complete queries (flag 2) were filled before the launch
-				for (uint32_t w = tid; w < nw; w += NT) {
-					if (a.qfull && a.qfull[q0 + w / kw] == 2u) continue;
-					dst[w] = s_bm[w];
+				// with flags (rich-filter pass, explicit units) the rows hold
+				// keys already: merge
+				if (a.qfull) {
+					for (uint32_t w = tid; w < nw; w += NT) dst[w] |= s_bm[w];
+				} else {
+					for (uint32_t w = tid; w < nw; w += NT) dst[w] = s_bm[w];
 				}
 			}
 		}
@@ -1051,34 +1053,57 @@ k_seg_bounds(const uint64_t * __restrict__ prefix, uint32_t nb, uint32_t nseg, u
 	segb[k] = lo;
 }
 
-// Queries that contain a complete filter (its key set is the whole
-// dictionary) match every key, whatever else they match: flag them with 2,
-// clear every other flag.  Exact for KEYS_UNIQUE.
+// Rich-filter pass, one wave per query: the key bitmaps of the rich
+// filters that the query contains, ORed into its row, which this writes
+// whole; flag 2 when the row then holds every key (the match kernel's
+// full-key-set exit drops the query), else 0.  Exact: a filter contained in
+// the query matches it, whatever the partition lookup.
 __global__ void __launch_bounds__(256)
-k_complete_flags(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ cf, uint32_t ncf,
-				 uint32_t * __restrict__ qfull) {
-	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-	if (i >= nq) return;
-	const uint4 x = *reinterpret_cast<const uint4 *>(q + (size_t)i * 8);
-	const uint2 y = *reinterpret_cast<const uint2 *>(q + (size_t)i * 8 + 4);
-	bool hit = false;
-	for (uint32_t c = 0; c < ncf; ++c) {
-		const uint4 f = sld4(cf + (size_t)c * 8);
-		const uint2 g = sld2(cf + (size_t)c * 8 + 4);
-		hit = hit || ((f.x & ~x.x) | (f.y & ~x.y) | (f.z & ~x.z) | (f.w & ~x.w) | (g.x & ~y.x) | (g.y & ~y.y)) == 0;
+k_rich_rows(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ rw,
+			const uint32_t * __restrict__ rbm, uint32_t nr, uint32_t kw, uint32_t last_mask,
+			uint32_t * __restrict__ gbm, uint32_t * __restrict__ qfull) {
+	const uint32_t lane = threadIdx.x & 63;
+	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (qi >= nq) return;
+	const uint4 x = *reinterpret_cast<const uint4 *>(q + (size_t)qi * 8);
+	const uint2 y = *reinterpret_cast<const uint2 *>(q + (size_t)qi * 8 + 4);
+	uint32_t * row = gbm + (size_t)qi * kw;
+	uint32_t all = 0xFFFFFFFFu;
+	for (uint32_t wb = 0; wb < kw; wb += 64) {   // 64 words (2048 keys) per pass
+		const uint32_t w = wb + lane;
+		uint32_t acc = 0;
+		for (uint32_t fb = 0; fb < nr; fb += 64) {
+			const uint32_t f = fb + lane;
+			bool hit = false;
+			if (f < nr) {
+				const uint4 a = *reinterpret_cast<const uint4 *>(rw + (size_t)f * 8);
+				const uint2 b = *reinterpret_cast<const uint2 *>(rw + (size_t)f * 8 + 4);
+				hit = ((a.x & ~x.x) | (a.y & ~x.y) | (a.z & ~x.z) | (a.w & ~x.w) | (b.x & ~y.x) | (b.y & ~y.y)) == 0;
+			}
+			uint64_t m = __ballot(hit);
+			while (m) {   // eight bitmaps in flight per round (a repeat ORs twice)
+				uint32_t j[8];
+#pragma unroll
+				for (int k = 0; k < 8; ++k) {
+					j[k] = m ? (uint32_t)__builtin_ctzll(m) : j[0];
+					m &= m - 1;
+				}
+				if (w < kw) {
+					uint32_t v[8];
+#pragma unroll
+					for (int k = 0; k < 8; ++k) v[k] = rbm[(size_t)(fb + j[k]) * kw + w];
+#pragma unroll
+					for (int k = 0; k < 8; ++k) acc |= v[k];
+				}
+			}
+		}
+		if (w < kw) {
+			row[w] = acc;
+			all &= w + 1 == kw ? (acc | ~last_mask) : acc;
+		}
 	}
-	qfull[i] = hit ? 2u : 0u;
-}
-
-// Bitmap rows of the flagged queries: every key (the last word's valid bits).
-__global__ void __launch_bounds__(256)
-k_complete_rows(uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t last_mask,
-				const uint32_t * __restrict__ qfull) {
-	const uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-	if (w >= (uint64_t)nq * kw) return;
-	const uint32_t qi = (uint32_t)(w / kw);
-	if (qfull[qi] != 2u) return;
-	gbm[w] = w - (uint64_t)qi * kw == kw - 1 ? last_mask : 0xFFFFFFFFu;
+	const bool full = __ballot(all != 0xFFFFFFFFu) == 0;
+	if (lane == 0) qfull[qi] = full ? 2u : 0u;
 }
 
 // ------------------------------------------------------------- kernel table
@@ -1434,18 +1459,15 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	r->last_nseg = nseg;
 	const bool exit_on = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
 	const uint32_t last_mask = (r->fib->n_keys & 31) ? (1u << (r->fib->n_keys & 31)) - 1u : 0xFFFFFFFFu;
-	const bool complete_pass = exit_on && r->fib->n_complete > 0 && nq > 0;
-	if (nseg > 1 && r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
+	const bool rich_pass = exit_on && r->fib->n_rich > 0 && nq > 0;
+	if (rich_pass) {
+		// rows start with the keys of the rich filters each query contains
+		// (written whole, with the flags); full rows need no test at all
+		hipLaunchKernelGGL(k_rich_rows, dim3((nq + 3) / 4), dim3(256), 0, s, r->d_q, nq, r->fib->d_rich_words,
+						   r->fib->d_rich_bm, r->fib->n_rich, r->kw, last_mask, r->d_gbm, r->d_qfull);
+	} else if (nseg > 1 && r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
 		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
-		if (!complete_pass) fill_u32(r->d_qfull, 0, nq, s);
-	}
-	if (complete_pass) {
-		// queries that contain a complete filter: every key, nothing to test
-		hipLaunchKernelGGL(k_complete_flags, dim3((nq + 255) / 256), dim3(256), 0, s, r->d_q, nq, r->fib->d_complete,
-						   r->fib->n_complete, r->d_qfull);
-		const uint64_t nw = (uint64_t)nq * r->kw;
-		hipLaunchKernelGGL(k_complete_rows, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, r->d_gbm, r->kw, nq,
-						   last_mask, r->d_qfull);
+		fill_u32(r->d_qfull, 0, nq, s);
 	}
 	hipLaunchKernelGGL(k_init_queues, dim3(1), dim3(64), 0, s, items, r->d_qbeg, r->d_qnext);
 	mq_args a;
@@ -1472,7 +1494,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.pcount = r->d_pcount;
 	a.gbm = r->d_gbm;
 	// flags hold this submission's state: cleared or set by the pass above
-	a.qfull = (nseg > 1 || complete_pass) ? r->d_qfull : nullptr;
+	a.qfull = (nseg > 1 || rich_pass) ? r->d_qfull : nullptr;
 	a.hits = r->d_hits;
 	a.hit_cap = r->hit_cap;
 	a.counters = r->d_cnt;
