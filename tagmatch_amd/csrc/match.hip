@@ -403,6 +403,7 @@ k_match_q(mq_args a) {
 	// global counters once at the end (all waves adding to one line per
 	// item serializes small launches)
 	uint64_t w_hits = 0, w_visit = 0, w_test = 0, w_hev = 0;
+	bool bm_dirty = true;   // LDS bitmap rows hold bits of an earlier item (uniform)
 
 	for (;;) {
 		uint64_t t_start = 0;
@@ -497,7 +498,9 @@ k_match_q(mq_args a) {
 		sink.bm_base = nullptr;
 		if (MODE == 0) {
 			if (LDS_BM) {
-				for (uint32_t w = tid; w < (uint32_t)NQ * kw; w += NT) s_bm[w] = 0;
+				if (bm_dirty)   // an item without hits leaves the rows at zero
+					for (uint32_t w = tid; w < (uint32_t)NQ * kw; w += NT) s_bm[w] = 0;
+				bm_dirty = false;
 				sink.bm_base = s_bm;
 			} else {
 				sink.bm_base = a.gbm + (size_t)q0 * kw;
@@ -517,6 +520,17 @@ k_match_q(mq_args a) {
 			qok[t] = __ballot(d[6] != NO_QUERY);
 		}
 
+		// full-key-set flags of the item's queries, reloaded at a partition
+		// only while a selected query is not known to be full
+		uint64_t qfullm[TILES];
+		const bool use_flags = MODE == 0 && LDS_BM && a.full_exit && a.qfull;
+		auto load_flags = [&](int t) -> uint64_t {
+			const uint32_t qrow = q0 + t * 64 + lane;
+			return __ballot(qrow < q_end &&
+							__hip_atomic_load(&a.qfull[qrow], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u);
+		};
+#pragma unroll
+		for (int t = 0; t < TILES; ++t) qfullm[t] = use_flags ? load_flags(t) : 0ULL;
 		uint32_t hit_acc = 0, n_visit = 0, n_test = 0, n_hev = 0, key_acc = 0;
 		// a hit event: lanes h of block b matched local query jl of the item
 		auto on_hit = [&](uint64_t h, uint32_t jl, uint32_t b, uint64_t kv) {
@@ -567,11 +581,10 @@ k_match_q(mq_args a) {
 					// the LDS row is the query's whole bitmap.
 					// Flags also mark the queries whose rows the rich-filter pass
 					// (k_rich_rows) already filled.
-					const uint32_t qrow = q0 + t * 64 + lane;
-					bool full = a.qfull && qrow < q_end &&
-								__hip_atomic_load(&a.qfull[qrow], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-					if (a.nseg <= 1) full = full || row_full(&s_bm[(t * 64 + lane) * kw], kw, lane, a.last_mask);
-					qsel[t] &= ~__ballot(full);
+					if (use_flags && (qsel[t] & ~qfullm[t])) qfullm[t] |= load_flags(t);
+					qsel[t] &= ~qfullm[t];
+					if (a.nseg <= 1 && qsel[t])
+						qsel[t] &= ~__ballot(row_full(&s_bm[(t * 64 + lane) * kw], kw, lane, a.last_mask));
 					anyq |= qsel[t];
 				}
 				if (anyq == 0) continue;
@@ -792,13 +805,17 @@ k_match_q(mq_args a) {
 		w_visit += n_visit;
 		w_test += n_test;
 		w_hev += n_hev;
+		if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
 		if (a.log) {
-			if (lane == 0 && hit_acc) atomicAdd(&s_hits, hit_acc);
 			for (int d = 32; d > 0; d >>= 1) key_acc += __shfl_xor(key_acc, d, 64);
 			if (lane == 0 && key_acc) atomicAdd(&s_keys, key_acc);
 		}
 		__syncthreads();
-		if (MODE == 0 && LDS_BM) {
+		// An item without hits set no bit: rows that are merged into gbm
+		// (several segments, or rows filled before the launch) need no flush.
+		const bool item_hits = s_hits != 0;
+		if (LDS_BM) bm_dirty = item_hits;
+		if (MODE == 0 && LDS_BM && (item_hits || (a.nseg <= 1 && !a.qfull))) {
 			const uint32_t nrows = min(q_end, q0 + NQ) - q0;
 			const uint32_t nw = nrows * kw;
 			uint32_t * dst = a.gbm + (size_t)q0 * kw;
