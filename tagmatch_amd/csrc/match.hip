@@ -46,6 +46,7 @@ constexpr uint32_t SEG_MAX_BLOCKS_EXIT = 4096;   // the same with the full-key-s
                                                  // skip their blocks, and fewer items cost less
 constexpr uint32_t WSEG_BASE = 64;          // block weight: fixed part, in units of one expected hit
 constexpr uint32_t WSEG_MAX_TILES = 256;    // persistent: tiles up to which segments are cut by weight
+constexpr uint64_t COMPACT_MIN_TILES = 128;   // persistent: tiles above which steps are compacted
 constexpr uint32_t NO_QUERY = 0xFFFFFFFFu;
 
 __device__ inline uint32_t mbcnt64(uint64_t m) {
@@ -231,6 +232,10 @@ struct mq_args {
 	uint32_t * qfull;                         // nseg > 1 with full_exit: per query, set once its
 	                                          // merged gbm row holds every key
 	const uint32_t * __restrict__ segb;       // nseg + 1 segment bounds (NULL: seg_blocks each)
+	const uint32_t * __restrict__ nq_dev;     // device query count (NULL: nq), with
+	const uint32_t * __restrict__ ntiles_dev; // device tile count, segments, blocks per segment
+	                                          // (NULL: ntiles, nseg, seg_blocks)
+	uint32_t map_rows;                        // KEYS_UNIQUE: local query i is row and flag qmap[i]
 };
 
 constexpr int HBUF = 256;                     // staged hits per wave (HITS mode)
@@ -404,6 +409,13 @@ k_match_q(mq_args a) {
 	// item serializes small launches)
 	uint64_t w_hits = 0, w_visit = 0, w_test = 0, w_hev = 0;
 	bool bm_dirty = true;   // LDS bitmap rows hold bits of an earlier item (uniform)
+	// counts written by earlier kernels of the step (compacted submissions)
+	const uint32_t nq_all = a.nq_dev ? sld1(a.nq_dev) : a.nq;
+	const uint32_t ntiles_all = a.ntiles_dev ? sld1(a.ntiles_dev) : a.ntiles;
+	const uint32_t nseg_all = a.ntiles_dev ? sld1(a.ntiles_dev + 1) : a.nseg;
+	const uint32_t seg_blocks_all = a.ntiles_dev ? sld1(a.ntiles_dev + 2) : a.seg_blocks;
+	// bitmap row (and flag) of local query i
+	auto row_of = [&](uint32_t i) -> uint32_t { return a.map_rows ? a.qmap[i] : i; };
 
 	for (;;) {
 		uint64_t t_start = 0;
@@ -428,7 +440,7 @@ k_match_q(mq_args a) {
 		// one contiguous range of blocks (small submissions spread over the
 		// whole GPU); segment-major, so concurrent items share filter blocks.
 		uint32_t item, seg_b0 = 0, seg_b1 = 0xFFFFFFFFu, r_begin = 0, r_end = P;
-		uint32_t q0, q_end = a.nq;
+		uint32_t q0, q_end = nq_all;
 		if (a.units) {
 			// explicit (partition, query range) units: the item's unit by
 			// binary search over the item prefix; segment-major in the unit
@@ -448,15 +460,15 @@ k_match_q(mq_args a) {
 			seg_b1 = min(seg_b0 + u1.y, u1.w);
 			r_begin = u0.x;
 			r_end = seg_b1 > seg_b0 ? u0.x + 1 : u0.x;
-		} else if (a.nseg > 1 || a.blk_lo != 0 || a.blk_hi < a.f.n_blocks) {
-			const uint32_t seg = item0 / a.ntiles;
-			item = item0 - seg * a.ntiles;
+		} else if (nseg_all > 1 || a.blk_lo != 0 || a.blk_hi < a.f.n_blocks) {
+			const uint32_t seg = item0 / ntiles_all;
+			item = item0 - seg * ntiles_all;
 			if (a.segb) {   // segments of equal estimated work
 				seg_b0 = sld1(a.segb + seg);
 				seg_b1 = sld1(a.segb + seg + 1);
 			} else {
-				seg_b0 = a.blk_lo + seg * a.seg_blocks;
-				seg_b1 = min(seg_b0 + a.seg_blocks, a.blk_hi);
+				seg_b0 = a.blk_lo + seg * seg_blocks_all;
+				seg_b1 = min(seg_b0 + seg_blocks_all, a.blk_hi);
 			}
 			if (seg_b1 > seg_b0) {
 				r_begin = sld1(a.f.blk_info + (size_t)seg_b0 * 8 + 7);
@@ -526,8 +538,8 @@ k_match_q(mq_args a) {
 		const bool use_flags = MODE == 0 && LDS_BM && a.full_exit && a.qfull;
 		auto load_flags = [&](int t) -> uint64_t {
 			const uint32_t qrow = q0 + t * 64 + lane;
-			return __ballot(qrow < q_end &&
-							__hip_atomic_load(&a.qfull[qrow], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u);
+			return __ballot(qrow < q_end && __hip_atomic_load(&a.qfull[row_of(qrow)], __ATOMIC_RELAXED,
+															  __HIP_MEMORY_SCOPE_AGENT) != 0u);
 		};
 #pragma unroll
 		for (int t = 0; t < TILES; ++t) qfullm[t] = use_flags ? load_flags(t) : 0ULL;
@@ -544,7 +556,7 @@ k_match_q(mq_args a) {
 			sink.add(h, jl, gq, b, kv, lane);
 		};
 		for (uint32_t r = r_begin; r < r_end; ++r) {
-			const uint32_t p = a.porder && a.nseg <= 1 ? sld1(a.porder + r) : r;
+			const uint32_t p = a.porder && nseg_all <= 1 ? sld1(a.porder + r) : r;
 			const uint32_t * pi = a.f.part_info + (size_t)p * 8;
 			const uint4 pa = sld4(pi);
 			const uint4 pb = sld4(pi + 4);
@@ -583,7 +595,7 @@ k_match_q(mq_args a) {
 					// (k_rich_rows) already filled.
 					if (use_flags && (qsel[t] & ~qfullm[t])) qfullm[t] |= load_flags(t);
 					qsel[t] &= ~qfullm[t];
-					if (a.nseg <= 1 && qsel[t])
+					if (nseg_all <= 1 && qsel[t])
 						qsel[t] &= ~__ballot(row_full(&s_bm[(t * 64 + lane) * kw], kw, lane, a.last_mask));
 					anyq |= qsel[t];
 				}
@@ -815,18 +827,19 @@ k_match_q(mq_args a) {
 		// (several segments, or rows filled before the launch) need no flush.
 		const bool item_hits = s_hits != 0;
 		if (LDS_BM) bm_dirty = item_hits;
-		if (MODE == 0 && LDS_BM && (item_hits || (a.nseg <= 1 && !a.qfull))) {
+		if (MODE == 0 && LDS_BM && (item_hits || (nseg_all <= 1 && !a.qfull))) {
 			const uint32_t nrows = min(q_end, q0 + NQ) - q0;
 			const uint32_t nw = nrows * kw;
 			uint32_t * dst = a.gbm + (size_t)q0 * kw;
-			if (a.nseg > 1) {
+			if (nseg_all > 1) {
 				// Other segments of the tile write the same rows: an atomic
 				// only for bits a row does not hold yet (rows fill up, and most
 				// words are then complete).  One wave per row; a row that is
 				// now full flags its query for the full-key-set exit.
 				for (uint32_t j = wave; j < nrows; j += NW) {
 					const uint32_t * src = s_bm + (size_t)j * kw;
-					uint32_t * drow = dst + (size_t)j * kw;
+					const uint32_t grow = row_of(q0 + j);
+					uint32_t * drow = a.gbm + (size_t)grow * kw;
 					uint32_t any = 0;
 					for (uint32_t w = lane; w < kw; w += 64) any |= src[w];
 					if (__ballot(any != 0) == 0) continue;   // nothing from this segment
@@ -839,12 +852,17 @@ k_match_q(mq_args a) {
 						all &= u;
 					}
 					if (a.qfull && __ballot(all != 0xFFFFFFFFu) == 0 && lane == 0)
-						atomicMax(&a.qfull[q0 + j], 1u);
+						atomicMax(&a.qfull[grow], 1u);
 				}
 			} else {
 				// with flags (rich-filter pass, explicit units) the rows hold
 				// keys already: merge
-				if (a.qfull) {
+				if (a.map_rows) {
+					for (uint32_t w = tid; w < nw; w += NT) {
+						const uint32_t j = w / kw;
+						a.gbm[(size_t)row_of(q0 + j) * kw + (w - j * kw)] |= s_bm[w];
+					}
+				} else if (a.qfull) {
 					for (uint32_t w = tid; w < nw; w += NT) dst[w] |= s_bm[w];
 				} else {
 					for (uint32_t w = tid; w < nw; w += NT) dst[w] = s_bm[w];
@@ -1123,6 +1141,76 @@ k_rich_rows(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __res
 	if (lane == 0) qfull[qi] = full ? 2u : 0u;
 }
 
+// Queries whose rows are not yet full after the rich-filter pass, listed
+// (in no particular order) for a match over them alone.
+__global__ void __launch_bounds__(256)
+k_compact_active(const uint32_t * __restrict__ qfull, uint32_t nq, uint32_t * __restrict__ list,
+				 uint32_t * __restrict__ n_active) {
+	const uint32_t i = blockIdx.x * 256 + threadIdx.x, lane = threadIdx.x & 63;
+	const bool act = i < nq && qfull[i] == 0u;
+	const uint64_t m = __ballot(act);
+	if (m == 0) return;
+	uint32_t base = 0;
+	if (lane == 0) base = atomicAdd(n_active, (uint32_t)__popcll(m));
+	base = __shfl(base, 0);
+	if (act) list[base + mbcnt64(m)] = i;
+}
+
+// Shape of a compacted step from its active query count: ceil(n / nqg)
+// tiles, cut into segments as the host cuts other steps (about two items
+// per resident workgroup, at most seg_max blocks each).  act[1..3] = tiles,
+// segments, blocks per segment; queue bounds and the item count follow.
+__global__ void k_init_queues_active(uint32_t * __restrict__ act, uint32_t nqg, uint32_t nblk, uint32_t resident,
+									 uint32_t seg_max, uint64_t * __restrict__ qbeg,
+									 unsigned long long * __restrict__ qnext, uint64_t * __restrict__ n_items) {
+	const uint32_t t = threadIdx.x;
+	const uint64_t tiles = ((uint64_t)act[0] + nqg - 1) / nqg;
+	uint32_t nseg = 1;
+	if (tiles > 0) {
+		nseg = (uint32_t)min((2ull * resident + tiles - 1) / tiles, (uint64_t)max(nblk / 16, 1u));
+		nseg = max(nseg, (nblk + seg_max - 1) / seg_max);
+	}
+	const uint32_t sb = nseg > 1 ? (nblk + nseg - 1) / nseg : max(nblk, 1u);
+	if (nseg > 1) nseg = (nblk + sb - 1) / sb;
+	const uint64_t items = tiles * nseg;
+	if (t <= NQUEUE) qbeg[t] = items * t / NQUEUE;
+	if (t < NQUEUE) qnext[t * QPAD] = items * t / NQUEUE;
+	if (t == 0) {
+		act[1] = (uint32_t)tiles;
+		act[2] = nseg;
+		act[3] = sb;
+		*n_items = items;
+	}
+}
+
+// Queries per partition (the lookup's counts) with per-block counters in
+// LDS: one global atomic per partition per block.  P counters of LDS.
+__global__ void __launch_bounds__(256)
+k_lookup_hist(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ part_info, uint32_t P,
+			  uint32_t * __restrict__ pcount) {
+	extern __shared__ uint32_t s_cnt[];
+	for (uint32_t p = threadIdx.x; p < P; p += 256) s_cnt[p] = 0;
+	__syncthreads();
+	const uint32_t lane = threadIdx.x & 63;
+	for (uint32_t i0 = blockIdx.x * 256; i0 < nq; i0 += gridDim.x * 256) {
+		const uint32_t i = i0 + threadIdx.x;
+		const bool valid = i < nq;
+		uint32_t n[6] = {~0u, ~0u, ~0u, ~0u, ~0u, ~0u};
+		if (valid)
+			for (int j = 0; j < 6; ++j) n[j] = ~q[(size_t)i * 8 + j];
+		for (uint32_t p = 0; p < P; ++p) {
+			const uint4 ma = sld4(part_info + (size_t)p * 8);
+			const uint2 mb = sld2(part_info + (size_t)p * 8 + 4);
+			const uint32_t t = (ma.x & n[0]) | (ma.y & n[1]) | (ma.z & n[2]) | (ma.w & n[3]) | (mb.x & n[4]) | (mb.y & n[5]);
+			const uint64_t b = __ballot(valid && t == 0);
+			if (b && lane == 0) atomicAdd(&s_cnt[p], (uint32_t)__popcll(b));
+		}
+	}
+	__syncthreads();
+	for (uint32_t p = threadIdx.x; p < P; p += 256)
+		if (s_cnt[p]) atomicAdd(&pcount[p], s_cnt[p]);
+}
+
 // ------------------------------------------------------------- kernel table
 
 using mq_fn = void (*)(mq_args);
@@ -1194,6 +1282,8 @@ struct tm_ring {
 	bool have_order = false;
 	uint32_t * d_gbm = nullptr;
 	uint32_t * d_qfull = nullptr;   // max_q: full-key-set flags of merged rows
+	uint32_t * d_active = nullptr;  // max_q: queries left after the rich-filter pass
+	uint32_t * d_nact = nullptr;    // active queries, their tiles, segments, blocks per segment
 	uint32_t * d_bw = nullptr;      // per block: work estimate (weighted segments)
 	uint64_t * d_bprefix = nullptr; // n_blocks + 1: exclusive prefix of d_bw
 	uint64_t * d_bscan = nullptr;   // scan scratch for d_bprefix
@@ -1285,7 +1375,7 @@ void tm_ring::release() {
 	h_stat = nullptr;
 	dfree(d_q); dfree(d_qinv); dfree(d_pcount); dfree(d_pbase); dfree(d_pcur); dfree(d_list);
 	dfree(d_sched); dfree(d_iprefix); dfree(d_qbeg); dfree(d_qnext); dfree(d_porder);
-	dfree(d_gbm); dfree(d_qfull); dfree(d_bw); dfree(d_bprefix); dfree(d_bscan); dfree(d_segb); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
+	dfree(d_gbm); dfree(d_qfull); dfree(d_active); dfree(d_nact); dfree(d_bw); dfree(d_bprefix); dfree(d_bscan); dfree(d_segb); dfree(d_hits); dfree(d_qcount); dfree(d_off); dfree(d_qcur);
 	dfree(d_vals); dfree(d_cnt); dfree(d_scan); dfree(d_stat); dfree(d_log);
 	dfree(d_units); dfree(d_ufeat); dfree(d_stage); dfree(d_uitems); dfree(d_ubeg); dfree(d_unext);
 	dfree(d_xunits); dfree(d_xprefix); dfree(d_iota);
@@ -1486,12 +1576,40 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 		fill_u32(r->d_qfull, 0, nq, s);
 	}
-	hipLaunchKernelGGL(k_init_queues, dim3(1), dim3(64), 0, s, items, r->d_qbeg, r->d_qnext);
+	// Compacted step: only the queries whose rows the pass left incomplete
+	// are matched, in tiles of their own (rows and flags through the list);
+	// their count stays on the device.  The lookup counts of every query come
+	// from k_lookup_hist, as the match sees only the active ones.
+	// (large steps only: the extra kernels cost about 0.1 ms, more than a
+	// step of up to ~8k queries gains)
+	const bool compact = rich_pass && tiles > COMPACT_MIN_TILES && !r->use_iorder && !r->cfg.log_units &&
+						 (r->variant & 1) == 0;
+	if (compact) {
+		fill_u32(r->d_nact, 0, 4, s);
+		if (P <= 16384) {
+			const uint32_t gh = (uint32_t)std::min<uint64_t>((nq + 255) / 256, 512);
+			hipLaunchKernelGGL(k_lookup_hist, dim3(gh), dim3(256), (size_t)P * 4, s, r->d_q, nq, r->fib->part_info, P,
+							   r->d_pcount);
+		} else {
+			hipLaunchKernelGGL(k_lookup_count, dim3((nq + 255) / 256), dim3(256), 0, s, r->d_q, nq, r->fib->part_info,
+							   P, r->d_pcount);
+		}
+		hipLaunchKernelGGL(k_compact_active, dim3((nq + 255) / 256), dim3(256), 0, s, r->d_qfull, nq, r->d_active,
+						   r->d_nact);
+		const uint32_t seg_max = r->cfg.chunk_blocks ? r->cfg.chunk_blocks : SEG_MAX_BLOCKS_EXIT;
+		hipLaunchKernelGGL(k_init_queues_active, dim3(1), dim3(64), 0, s, r->d_nact, (uint32_t)nqg, nblk,
+						   (uint32_t)resident, seg_max, r->d_qbeg, r->d_qnext, r->d_iprefix + P);
+	} else {
+		hipLaunchKernelGGL(k_init_queues, dim3(1), dim3(64), 0, s, items, r->d_qbeg, r->d_qnext);
+	}
 	mq_args a;
 	std::memset(&a, 0, sizeof(a));
 	a.f = r->fib->view();
 	a.q = r->d_q;
 	a.nq = nq;
+	a.nq_dev = compact ? r->d_nact : nullptr;
+	a.ntiles_dev = compact ? r->d_nact + 1 : nullptr;
+	a.map_rows = compact;
 	a.porder = r->have_order ? r->d_porder : nullptr;
 	a.iorder = r->use_iorder ? r->d_iorder : nullptr;
 	a.full_exit = exit_on;
@@ -1501,14 +1619,14 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.ntiles = (uint32_t)tiles;
 	a.nseg = nseg;
 	a.seg_blocks = seg_blocks;
-	a.segb = weighted ? r->d_segb : nullptr;
+	a.segb = weighted && !compact ? r->d_segb : nullptr;
 	a.blk_lo = 0;
 	a.blk_hi = nblk;
 	a.log_unit = -1;
-	a.qmap = nullptr;
+	a.qmap = compact ? r->d_active : nullptr;
 	a.qnext = r->d_qnext;
 	a.qbeg = r->d_qbeg;
-	a.pcount = r->d_pcount;
+	a.pcount = compact ? nullptr : r->d_pcount;
 	a.gbm = r->d_gbm;
 	// flags hold this submission's state: cleared or set by the pass above
 	a.qfull = (nseg > 1 || rich_pass) ? r->d_qfull : nullptr;
@@ -1530,7 +1648,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	TM_HIP(hipGetLastError());
 	if (events) TM_HIP(hipEventRecord(r->em1, s));
 	scan_u32_to_u64(r->d_pcount, r->d_pbase, P, r->d_scan, s);
-	fill_u64(r->d_iprefix + P, items, 1, s);
+	if (!compact) fill_u64(r->d_iprefix + P, items, 1, s);
 	enqueue_finalize(r, nq);
 	return TM_OK;
 }
@@ -1730,7 +1848,11 @@ int ensure_capacity(tm_ring * r) {
 		TM_CHECK(dalloc(&r->d_gbm, n));
 		TM_HIP(hipMemset(r->d_gbm, 0, std::max<size_t>(n, 1) * 4));
 	}
-	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->d_qfull) TM_CHECK(dalloc(&r->d_qfull, std::max<uint32_t>(r->max_q, 1)));
+	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->d_qfull) {
+		TM_CHECK(dalloc(&r->d_qfull, std::max<uint32_t>(r->max_q, 1)));
+		TM_CHECK(dalloc(&r->d_active, std::max<uint32_t>(r->max_q, 1)));
+		TM_CHECK(dalloc(&r->d_nact, 4));
+	}
 	if (r->cfg.mode != TM_MODE_KEYS_UNIQUE && !r->d_hits) {
 		uint64_t cap = r->cfg.hit_capacity ? r->cfg.hit_capacity : std::max<uint64_t>(1 << 20, (uint64_t)r->max_q * 64);
 		TM_CHECK(dalloc(&r->d_hits, cap));

@@ -229,6 +229,14 @@ def test_rich_filter_pass(gpu):
             res[fe] = st["n_hits"]
         log(f"rich pass, segment {seg}: hits processed {res[1]} with it, {res[2]} without")
         assert res[1] < res[2]
+    # a step large enough to be compacted: only the queries whose rows the
+    # pass left incomplete are matched, rows and flags through the list
+    q2 = rng.integers(0, 2**32, size=(20000, 6), dtype=np.uint64).astype(np.uint32)
+    q2[::2, 0] |= 0b111
+    eo2, ev2 = orc.match(cw2, cko2, ck2, q2, mode=0)
+    for seg in (0, 8):
+        o, v, st = tm.match(fib, q2, chunk_blocks=seg)
+        assert_same(o, v, eo2, ev2, sort_got=False, what=f"compacted step, segment {seg}")
 
 
 @pytest.mark.parametrize("n_keys", [5, 37, 64, 100])
