@@ -104,6 +104,7 @@ struct tm_fib {
 	uint32_t * d_rich_words = nullptr;   // n_rich x 8 words (6 + padding)
 	uint32_t * d_rich_bm = nullptr;      // n_rich x kw words
 	uint32_t n_rich = 0;
+	uint32_t n_rich_full = 0;            // the first n_rich_full hold every key
 	uint32_t prep_epoch = 0;             // epoch of the last prepared insert
 	uint32_t ovf_valid = 1;              // words of ovf written on the device
 

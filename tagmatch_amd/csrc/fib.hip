@@ -34,7 +34,7 @@ void tm_fib::release() {
 	dfree(slot_epoch);
 	dfree(d_rich_words);
 	dfree(d_rich_bm);
-	n_rich = 0;
+	n_rich = n_rich_full = 0;
 	epoch = 0;
 	n_parts = n_blocks = n_keys = 0;
 	n_filters = n_key_instances = device_bytes = max_part = 0;
@@ -412,6 +412,7 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	// rich filters: the MAX_RICH with the most keys (at least RICH_MIN_KEYS),
 	// their words and their key sets as dense bitmaps
 	std::vector<uint32_t> rich_words, rich_bm;
+	uint32_t n_rich_full = 0;
 	{
 		std::vector<std::pair<uint32_t, uint32_t>> cand;   // (keys, filter)
 		for (uint32_t i = 0; i < n; ++i) {
@@ -422,16 +423,30 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 		std::partial_sort(cand.begin(), cand.begin() + nr, cand.end(),
 						  [](const auto & x, const auto & y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });
 		const uint32_t kwr = (K + 31) / 32;
-		rich_words.assign(nr * 8, 0);
-		rich_bm.assign(nr * kwr, 0);
+		std::vector<uint32_t> bw(nr * 8, 0), bm(nr * kwr, 0);
+		std::vector<uint8_t> full(nr, 0);
 		for (size_t r = 0; r < nr; ++r) {
 			const uint32_t i = cand[r].second;
-			for (int j = 0; j < 6; ++j) rich_words[r * 8 + j] = words[(size_t)i * 6 + j];
+			for (int j = 0; j < 6; ++j) bw[r * 8 + j] = words[(size_t)i * 6 + j];
+			uint32_t bits = 0;
 			for (uint32_t x = key_offsets[i]; x < key_offsets[i + 1]; ++x) {
 				const uint32_t id = (uint32_t)(std::lower_bound(dict.begin(), dict.end(), keys[x]) - dict.begin());
-				rich_bm[r * kwr + id / 32] |= 1u << (id % 32);
+				bits += !((bm[r * kwr + id / 32] >> (id % 32)) & 1u);
+				bm[r * kwr + id / 32] |= 1u << (id % 32);
 			}
+			full[r] = bits == K;
 		}
+		// filters holding every key first: a query that contains one needs
+		// nothing else
+		rich_words.reserve(nr * 8);
+		rich_bm.reserve(nr * kwr);
+		for (int pass = 0; pass < 2; ++pass)
+			for (size_t r = 0; r < nr; ++r) {
+				if (full[r] != (pass == 0)) continue;
+				rich_words.insert(rich_words.end(), bw.begin() + r * 8, bw.begin() + r * 8 + 8);
+				rich_bm.insert(rich_bm.end(), bm.begin() + r * kwr, bm.begin() + (r + 1) * kwr);
+				n_rich_full += pass == 0;
+			}
 	}
 
 	fib->release();
@@ -452,6 +467,7 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 		TM_CHECK(upload(&fib->d_rich_bm, rich_bm, bytes));
 	}
 	fib->n_rich = (uint32_t)(rich_words.size() / 8);
+	fib->n_rich_full = n_rich_full;
 	fib->n_parts = np;
 	fib->n_blocks = bb.nb;
 	fib->n_keys = K;
@@ -727,7 +743,7 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 		fib->n_keys = (uint32_t)fib->h_dict.size();
 		dfree(fib->d_rich_words);   // the bitmaps hold the old dense ids
 		dfree(fib->d_rich_bm);
-		fib->n_rich = 0;
+		fib->n_rich = fib->n_rich_full = 0;
 		bump = true;
 	}
 	const bool inline_ok = fib->n_keys < KEY_NONE;

@@ -1095,7 +1095,7 @@ k_seg_bounds(const uint64_t * __restrict__ prefix, uint32_t nb, uint32_t nseg, u
 // the query matches it, whatever the partition lookup.
 __global__ void __launch_bounds__(256)
 k_rich_rows(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ rw,
-			const uint32_t * __restrict__ rbm, uint32_t nr, uint32_t kw, uint32_t last_mask,
+			const uint32_t * __restrict__ rbm, uint32_t nr, uint32_t nfull, uint32_t kw, uint32_t last_mask,
 			uint32_t * __restrict__ gbm, uint32_t * __restrict__ qfull) {
 	const uint32_t lane = threadIdx.x & 63;
 	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1103,6 +1103,23 @@ k_rich_rows(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __res
 	const uint4 x = *reinterpret_cast<const uint4 *>(q + (size_t)qi * 8);
 	const uint2 y = *reinterpret_cast<const uint2 *>(q + (size_t)qi * 8 + 4);
 	uint32_t * row = gbm + (size_t)qi * kw;
+	// the first nfull rich filters hold every key: one contained is enough
+	bool any_full = false;
+	for (uint32_t fb = 0; fb < nfull && !any_full; fb += 64) {
+		const uint32_t f = fb + lane;
+		bool hit = false;
+		if (f < nfull) {
+			const uint4 a = *reinterpret_cast<const uint4 *>(rw + (size_t)f * 8);
+			const uint2 b = *reinterpret_cast<const uint2 *>(rw + (size_t)f * 8 + 4);
+			hit = ((a.x & ~x.x) | (a.y & ~x.y) | (a.z & ~x.z) | (a.w & ~x.w) | (b.x & ~y.x) | (b.y & ~y.y)) == 0;
+		}
+		any_full = __ballot(hit) != 0;
+	}
+	if (any_full) {
+		for (uint32_t w = lane; w < kw; w += 64) row[w] = w + 1 == kw ? last_mask : 0xFFFFFFFFu;
+		if (lane == 0) qfull[qi] = 2u;
+		return;
+	}
 	uint32_t all = 0xFFFFFFFFu;
 	for (uint32_t wb = 0; wb < kw; wb += 64) {   // 64 words (2048 keys) per pass
 		const uint32_t w = wb + lane;
@@ -1571,7 +1588,8 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 		// rows start with the keys of the rich filters each query contains
 		// (written whole, with the flags); full rows need no test at all
 		hipLaunchKernelGGL(k_rich_rows, dim3((nq + 3) / 4), dim3(256), 0, s, r->d_q, nq, r->fib->d_rich_words,
-						   r->fib->d_rich_bm, r->fib->n_rich, r->kw, last_mask, r->d_gbm, r->d_qfull);
+						   r->fib->d_rich_bm, r->fib->n_rich, r->fib->n_rich_full, r->kw, last_mask, r->d_gbm,
+						   r->d_qfull);
 	} else if (nseg > 1 && r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
 		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 		fill_u32(r->d_qfull, 0, nq, s);
