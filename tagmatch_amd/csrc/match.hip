@@ -1465,9 +1465,10 @@ int resolve_params(tm_ring * r) {
 // Waves per workgroup of a persistent step of `tiles` query tiles: the
 // configured value, or by default more waves for fewer tiles (a small step
 // has few items in flight per tile, and more waves share each item's blocks).
-int step_waves(const tm_ring * r, uint64_t tiles) {
+int step_waves(const tm_ring * r, uint64_t tiles, bool compacted) {
 	if (r->cfg.waves_per_group) return r->waves;
-	return tiles <= 16 ? 16 : tiles <= 256 ? 8 : 4;
+	// a compacted step has few tiles left whatever it started with
+	return tiles <= 16 ? 16 : (tiles <= 256 || compacted) ? 8 : 4;
 }
 int wgs_for_waves(const tm_ring * r, int nw) { return r->wgs_for[nw == 4 ? 0 : nw == 8 ? 1 : 2]; }
 
@@ -1539,7 +1540,14 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 8, s);
 	const uint64_t nqg = (uint64_t)r->tiles * 64;
 	const uint64_t tiles = (nq + nqg - 1) / nqg;
-	const int nw = step_waves(r, tiles);
+	const bool exit_on = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
+	const bool rich_pass = exit_on && r->fib->n_rich > 0 && nq > 0;
+	// Compacted step: only the queries whose rows the pass leaves incomplete
+	// are matched (large steps only: the extra kernels cost about 0.1 ms,
+	// more than a step of up to ~8k queries gains)
+	const bool compact = rich_pass && tiles > COMPACT_MIN_TILES && !r->use_iorder && !r->cfg.log_units &&
+						 (r->variant & 1) == 0;
+	const int nw = step_waves(r, tiles, compact);
 	// Small submissions: cut each tile's work into block segments so that
 	// the items fill the GPU (about two per resident workgroup).  Needs the
 	// key bitmaps in LDS (flushed with atomicOr) or the hit-list modes.
@@ -1581,9 +1589,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 						   r->d_segb);
 	}
 	r->last_nseg = nseg;
-	const bool exit_on = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
 	const uint32_t last_mask = (r->fib->n_keys & 31) ? (1u << (r->fib->n_keys & 31)) - 1u : 0xFFFFFFFFu;
-	const bool rich_pass = exit_on && r->fib->n_rich > 0 && nq > 0;
 	if (rich_pass) {
 		// rows start with the keys of the rich filters each query contains
 		// (written whole, with the flags); full rows need no test at all
@@ -1594,14 +1600,10 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 		fill_u32(r->d_qfull, 0, nq, s);
 	}
-	// Compacted step: only the queries whose rows the pass left incomplete
-	// are matched, in tiles of their own (rows and flags through the list);
-	// their count stays on the device.  The lookup counts of every query come
-	// from k_lookup_hist, as the match sees only the active ones.
-	// (large steps only: the extra kernels cost about 0.1 ms, more than a
-	// step of up to ~8k queries gains)
-	const bool compact = rich_pass && tiles > COMPACT_MIN_TILES && !r->use_iorder && !r->cfg.log_units &&
-						 (r->variant & 1) == 0;
+	// Compacted step: the active queries in tiles of their own (rows and
+	// flags through the list); their count stays on the device.  The lookup
+	// counts of every query come from k_lookup_hist, as the match sees only
+	// the active ones.
 	if (compact) {
 		fill_u32(r->d_nact, 0, 4, s);
 		if (P <= 16384) {
