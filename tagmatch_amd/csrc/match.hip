@@ -47,6 +47,7 @@ constexpr uint32_t SEG_MAX_BLOCKS_EXIT = 4096;   // the same with the full-key-s
 constexpr uint32_t WSEG_BASE = 64;          // block weight: fixed part, in units of one expected hit
 constexpr uint32_t WSEG_MAX_TILES = 256;    // persistent: tiles up to which segments are cut by weight
 constexpr uint64_t COMPACT_MIN_TILES = 128;   // persistent: tiles above which steps are compacted
+constexpr uint32_t PRICE_SAMPLE_TILES = 1024;  // cost model: tiles priced when only totals are needed
 constexpr uint32_t NO_QUERY = 0xFFFFFFFFu;
 
 __device__ inline uint32_t mbcnt64(uint64_t m) {
@@ -2084,7 +2085,20 @@ int price_submission(tm_ring * r, uint32_t nq) {
 	const auto t0 = std::chrono::steady_clock::now();
 	hipStream_t s = r->s;
 	const uint32_t nqt = 64 * r->tiles;
-	const uint32_t n_tiles = (nq + nqt - 1) / nqt;
+	const uint32_t n_all = (nq + nqt - 1) / nqt;
+	// ordering pays when tiles outnumber the resident workgroups and the
+	// step is not cut into block segments (small submissions and tables
+	// above SEG_MAX_BLOCKS are, in enqueue_persistent)
+	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
+	const uint32_t nblk = r->fib->n_blocks;
+	const bool one_segment = nblk <= SEG_MAX_BLOCKS || r->cfg.chunk_blocks >= nblk;
+	const bool order = (r->model_flags & TM_COST_ORDER) && n_all >= 2 * resident && one_segment;
+	// Without an order to compute, only totals are needed: price a prefix of
+	// PRICE_SAMPLE_TILES tiles (queries arrive in no particular order) and
+	// scale.  Pricing every tile of a large step costs more than the step.
+	const uint32_t n_tiles = order ? n_all : std::min<uint32_t>(n_all, PRICE_SAMPLE_TILES);
+	const uint32_t nq_priced = std::min<uint32_t>(nq, n_tiles * nqt);
+	const double scale = (double)nq / std::max<uint32_t>(nq_priced, 1);
 	const uint64_t rows = (uint64_t)n_tiles * r->P;
 	if (rows > 0xFFFFFFFFull) return set_error(TM_ERR_INVALID, "too many cost model rows");
 	if (rows > r->trows_cap) {
@@ -2109,7 +2123,7 @@ int price_submission(tm_ring * r, uint32_t nq) {
 		r->tiles_cap = n_tiles;
 		if (graph_uses) drop_graph(r);
 	}
-	launch_unit_features_tiles(r->fib->view(), r->d_q, nq, nqt, n_tiles, r->d_tfeat, s);
+	launch_unit_features_tiles(r->fib->view(), r->d_q, nq_priced, nqt, n_tiles, r->d_tfeat, s);
 	TM_CHECK(mlp_infer(r->model, r->d_tfeat, (uint32_t)rows, r->d_tpred, s));
 	launch_tile_costs(r->d_tfeat, r->d_tpred, n_tiles, r->P, r->d_tcost, r->d_thits, r->d_tkeys, s);
 	TM_HIP(hipGetLastError());
@@ -2123,15 +2137,11 @@ int price_submission(tm_ring * r, uint32_t nq) {
 		total += hits[t];
 		total_keys += keys[t];
 	}
+	total *= scale;
+	total_keys *= scale;
 	r->st.pred_hits = total;
 
-	// ordering pays when tiles outnumber the resident workgroups and the
-	// step is not cut into block segments (small submissions and tables
-	// above SEG_MAX_BLOCKS are, in enqueue_persistent)
-	const uint64_t resident = (uint64_t)dev().num_cu * r->wgs_per_cu;
-	const uint32_t nblk = r->fib->n_blocks;
-	const bool one_segment = nblk <= SEG_MAX_BLOCKS || r->cfg.chunk_blocks >= nblk;
-	if ((r->model_flags & TM_COST_ORDER) && n_tiles >= 2 * resident && one_segment) {
+	if (order) {
 		std::vector<uint32_t> idx(n_tiles);
 		std::iota(idx.begin(), idx.end(), 0u);
 		std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
@@ -2165,11 +2175,11 @@ int price_submission(tm_ring * r, uint32_t nq) {
 			const double K = std::max<double>(f->n_keys, 1.0);
 			double est = 0.0;
 			for (uint32_t t = 0; t < n_tiles; ++t) {
-				const uint32_t qn = std::min<uint32_t>(nqt, nq - t * nqt);
+				const uint32_t qn = std::min<uint32_t>(nqt, nq_priced - t * nqt);
 				const double k = keys[t] / std::max<uint32_t>(qn, 1);
 				est += qn * K * (1.0 - std::exp(-k / K));
 			}
-			results = (uint64_t)(est * margin) + 4096;
+			results = (uint64_t)(est * scale * margin) + 4096;
 		} else {
 			slots = (uint64_t)(total * margin) + 4096;
 			results = r->cfg.mode == TM_MODE_KEYS_ALL ? (uint64_t)(total_keys * margin) + 4096 : slots;
