@@ -491,6 +491,36 @@ int tm_gen_filters(const tm_gen * g, uint64_t first, uint64_t n, uint32_t * word
 int tm_gen_queries(const tm_gen * g, uint64_t n_filters, uint64_t first, uint64_t n,
 				   uint32_t * words, uint8_t * is_superset, int threads);
 
+/* ------------------------------------------------------------------------ */
+/* Reference file formats, so that datasets written by the reference's tools
+ * can be read into this engine and written back (host only, no device).
+ *   TM_IO_BINARY      FIB entries: filter (24 bytes, the six words little-
+ *                     endian), u32 key count, u32 keys (reference fib.cc:14-35,
+ *                     bitvector.hh:506-540); queries: the 24-byte filter
+ *                     (query.cc:11-19).
+ *   TM_IO_ASCII       FIB entries "+ <192 bits, bit 0 first> key ...", queries
+ *                     "! <192 bits>" (fib.cc:38-60, query.cc:21-40,
+ *                     bitvector.hh:543-580); other lines are skipped.
+ *   TM_IO_BINARY_PARTITIONED  FIB entries preceded by a u32 partition id
+ *                     (fib.hh:41-56, partition_fib_entry).
+ * Reading is two calls: scan for the counts, then read into buffers of that
+ * size (key_offsets holds n_filters + 1 entries).  Replaces the reference's
+ * fib_entry / basic_query read_binary, write_binary, read_ascii and
+ * write_ascii and the loops of its workload tools around them. */
+#define TM_IO_BINARY              0u
+#define TM_IO_ASCII               1u
+#define TM_IO_BINARY_PARTITIONED  2u
+
+int tm_io_scan_fib(const char * path, uint32_t format, uint64_t * n_filters, uint64_t * n_keys);
+int tm_io_read_fib(const char * path, uint32_t format, uint32_t * words, uint32_t * key_offsets,
+				   uint32_t * keys, uint32_t * partitions, uint64_t n_filters, uint64_t n_keys);
+/* partitions: n ids for TM_IO_BINARY_PARTITIONED (NULL: 0), else ignored. */
+int tm_io_write_fib(const char * path, uint32_t format, const uint32_t * words, uint64_t n,
+					const uint32_t * key_offsets, const uint32_t * keys, const uint32_t * partitions);
+int tm_io_scan_queries(const char * path, uint32_t format, uint64_t * n);
+int tm_io_read_queries(const char * path, uint32_t format, uint32_t * words, uint64_t n);
+int tm_io_write_queries(const char * path, uint32_t format, const uint32_t * words, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

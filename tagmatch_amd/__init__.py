@@ -30,6 +30,9 @@ SCHED_PLAIN = 0
 SCHED_PERSISTENT = 1
 
 
+IO_BINARY, IO_ASCII, IO_BINARY_PARTITIONED = 0, 1, 2
+
+
 class TagMatchError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -162,6 +165,12 @@ _SIGS = {
     "tm_gen_tag_bits": (C.c_int, [P, C.c_uint32, P]),
     "tm_gen_filters": (C.c_int, [P, C.c_uint64, C.c_uint64, P, P, P, C.c_int]),
     "tm_gen_queries": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, P, P, C.c_int]),
+    "tm_io_scan_fib": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "tm_io_read_fib": (C.c_int, [C.c_char_p, C.c_uint32, P, P, P, P, C.c_uint64, C.c_uint64]),
+    "tm_io_write_fib": (C.c_int, [C.c_char_p, C.c_uint32, P, C.c_uint64, P, P, P]),
+    "tm_io_scan_queries": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "tm_io_read_queries": (C.c_int, [C.c_char_p, C.c_uint32, P, C.c_uint64]),
+    "tm_io_write_queries": (C.c_int, [C.c_char_p, C.c_uint32, P, C.c_uint64]),
 }
 
 # Declared in include/tagmatch_amd.h; tests check every one is exported.
@@ -762,3 +771,44 @@ def mape(pred, true, floor=1.0):
 def split(off, vals):
     """Per-query lists from (offsets, values)."""
     return [vals[int(off[i]):int(off[i + 1])] for i in range(len(off) - 1)]
+
+
+# ------------------------------------------------------------------ file I/O
+
+def read_fib_file(path, fmt=IO_BINARY):
+    """Filter table in the reference's formats (tm_io_read_fib):
+    (words n x 6, key_offsets n + 1, keys, partition ids)."""
+    nf, nk = C.c_uint64(), C.c_uint64()
+    bp = os.fsencode(path)
+    check(load().tm_io_scan_fib(bp, fmt, C.byref(nf), C.byref(nk)), "tm_io_scan_fib")
+    words = np.zeros((nf.value, 6), np.uint32)
+    ko = np.zeros(nf.value + 1, np.uint32)
+    keys = np.zeros(nk.value, np.uint32)
+    parts = np.zeros(nf.value, np.uint32)
+    check(load().tm_io_read_fib(bp, fmt, _ptr(words), _ptr(ko), _ptr(keys), _ptr(parts), nf.value, nk.value),
+          "tm_io_read_fib")
+    return words, ko, keys, parts
+
+
+def write_fib_file(path, words, key_offsets, keys, fmt=IO_BINARY, partitions=None):
+    w = np.ascontiguousarray(words, np.uint32).reshape(-1, 6)
+    ko = np.ascontiguousarray(key_offsets, np.uint32)
+    kv = np.ascontiguousarray(keys, np.uint32)
+    pt = None if partitions is None else np.ascontiguousarray(partitions, np.uint32)
+    check(load().tm_io_write_fib(os.fsencode(path), fmt, _ptr(w), w.shape[0], _ptr(ko), _ptr(kv), _ptr(pt)),
+          "tm_io_write_fib")
+
+
+def read_query_file(path, fmt=IO_BINARY):
+    """Queries in the reference's formats (tm_io_read_queries): n x 6 words."""
+    n = C.c_uint64()
+    bp = os.fsencode(path)
+    check(load().tm_io_scan_queries(bp, fmt, C.byref(n)), "tm_io_scan_queries")
+    q = np.zeros((n.value, 6), np.uint32)
+    check(load().tm_io_read_queries(bp, fmt, _ptr(q), n.value), "tm_io_read_queries")
+    return q
+
+
+def write_query_file(path, words, fmt=IO_BINARY):
+    q = np.ascontiguousarray(words, np.uint32).reshape(-1, 6)
+    check(load().tm_io_write_queries(os.fsencode(path), fmt, _ptr(q), q.shape[0]), "tm_io_write_queries")
