@@ -127,6 +127,16 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 				  const uint32_t * filter_ids);
 int tm_fib_stats_get(const tm_fib * fib, tm_fib_stats * st);
 
+/* Versioned image of a resident table: blocks, key slots, overflow lists,
+ * dictionary, partitions and rich filters, as uploaded and inserted so far.
+ * Loading replaces the FIB's content without partitioning or laying out
+ * blocks again.  Replaces the reference's tagmatch.db round trip
+ * (tagmatch.cc:79-142, written as partition_fib_entry, read back as
+ * fib_entry), which re-partitions on every start.  Little-endian hosts;
+ * a file of another image version is refused. */
+int tm_fib_save(tm_fib * fib, const char * path);
+int tm_fib_load(tm_fib * fib, const char * path);
+
 /* Spare blocks reserved per partition at upload for tm_fib_insert, in
  * percent of its used blocks (plus one block); default 13, 0 = none.  With a
  * reserve, the upload also adds one empty partition with mask 0 (selected

@@ -165,6 +165,8 @@ _SIGS = {
     "tm_gen_tag_bits": (C.c_int, [P, C.c_uint32, P]),
     "tm_gen_filters": (C.c_int, [P, C.c_uint64, C.c_uint64, P, P, P, C.c_int]),
     "tm_gen_queries": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, P, P, C.c_int]),
+    "tm_fib_save": (C.c_int, [P, C.c_char_p]),
+    "tm_fib_load": (C.c_int, [P, C.c_char_p]),
     "tm_io_scan_fib": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "tm_io_read_fib": (C.c_int, [C.c_char_p, C.c_uint32, P, P, P, P, C.c_uint64, C.c_uint64]),
     "tm_io_write_fib": (C.c_int, [C.c_char_p, C.c_uint32, P, C.c_uint64, P, P, P]),
@@ -350,6 +352,21 @@ class Fib:
         check(load().tm_fib_upload(self.h, _ptr(words), n, _ptr(po), _ptr(pm), len(po) - 1, _ptr(ko), _ptr(kv),
                                    _ptr(fid)), "tm_fib_upload")
 
+    @classmethod
+    def empty(cls):
+        """A FIB with no table yet (upload() or load() gives it one)."""
+        self = cls.__new__(cls)
+        self.h = C.c_void_p()
+        check(load().tm_fib_create(C.byref(self.h)), "tm_fib_create")
+        return self
+
+    @classmethod
+    def from_image(cls, path):
+        """A FIB holding the table of an image written by save()."""
+        self = cls.empty()
+        self.load(path)
+        return self
+
     def set_reserve(self, percent):
         """Spare blocks per partition (percent) for the next upload."""
         check(load().tm_fib_set_reserve(self.h, percent), "tm_fib_set_reserve")
@@ -375,6 +392,14 @@ class Fib:
             return False
         check(rc, "tm_fib_insert_live")
         return True
+
+    def save(self, path):
+        """Write a versioned image of the resident table (tm_fib_save)."""
+        check(load().tm_fib_save(self.h, os.fsencode(path)), "tm_fib_save")
+
+    def load(self, path):
+        """Replace the table with an image from save() (tm_fib_load)."""
+        check(load().tm_fib_load(self.h, os.fsencode(path)), "tm_fib_load")
 
     def stats(self):
         s = FibStats()

@@ -925,3 +925,177 @@ int tm_fib_insert_live(tm_fib * fib, const uint32_t * words, uint32_t n, const u
 }
 
 } // extern "C"
+
+// ------------------------------------------------------------- FIB image
+
+namespace {
+
+constexpr char IMAGE_MAGIC[8] = {'T', 'M', 'F', 'I', 'B', 'I', 'M', 'G'};
+constexpr uint32_t IMAGE_VERSION = 1;
+
+struct image_header {
+	char magic[8];
+	uint32_t version, reserved;
+	uint64_t n_parts, n_blocks, n_keys, n_filters, n_key_instances, max_part, reserve_pct, n_rich, n_rich_full;
+};
+
+template <typename T>
+bool write_array(std::FILE * f, const T * p, uint64_t n) {
+	return std::fwrite(&n, 8, 1, f) == 1 && (n == 0 || std::fwrite(p, sizeof(T), n, f) == n);
+}
+
+template <typename T>
+int device_array_out(std::FILE * f, const T * d, uint64_t n) {
+	std::vector<T> h(n);
+	if (n) TM_HIP(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+	return write_array(f, h.data(), n) ? TM_OK : set_error(TM_ERR_INVALID, "image write failed");
+}
+
+// An array of exactly `want` elements.
+template <typename T>
+int read_array(std::FILE * f, std::vector<T> & v, uint64_t want, const char * what) {
+	uint64_t n = 0;
+	if (std::fread(&n, 8, 1, f) != 1) return set_error(TM_ERR_INVALID, "image truncated before %s", what);
+	if (n != want)
+		return set_error(TM_ERR_INVALID, "image %s holds %llu elements, header says %llu", what, (unsigned long long)n,
+						 (unsigned long long)want);
+	v.resize(n);
+	if (n && std::fread(v.data(), sizeof(T), n, f) != n) return set_error(TM_ERR_INVALID, "image truncated in %s", what);
+	return TM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int tm_fib_save(tm_fib * fib, const char * path) {
+	if (!fib || !path) return set_error(TM_ERR_INVALID, "NULL argument");
+	if (fib->h_order.size() != 192) return set_error(TM_ERR_STATE, "no table uploaded");
+	TM_CHECK(require_device());
+	std::lock_guard<std::mutex> lk(fib->host_mu);
+	if (fib->prep_epoch != fib->epoch) return set_error(TM_ERR_BUSY, "a prepared live insert is not committed yet");
+	const size_t P0 = fib->n_parts;
+	if (fib->h_part_info.size() != P0 * 8 || fib->h_part_stat.size() != P0 * 8 || fib->h_part_size.size() != P0 ||
+		fib->h_part_cap.size() != P0 || fib->h_dict.size() != fib->n_keys || fib->h_ovf.size() < fib->ovf_valid)
+		return set_error(TM_ERR_STATE, "host copies of the table are inconsistent");
+	std::FILE * f = std::fopen(path, "wb");
+	if (!f) return set_error(TM_ERR_INVALID, "cannot create %s", path);
+	image_header h;
+	std::memset(&h, 0, sizeof(h));
+	std::memcpy(h.magic, IMAGE_MAGIC, 8);
+	h.version = IMAGE_VERSION;
+	h.n_parts = fib->n_parts;
+	h.n_blocks = fib->n_blocks;
+	h.n_keys = fib->n_keys;
+	h.n_filters = fib->n_filters;
+	h.n_key_instances = fib->n_key_instances;
+	h.max_part = fib->max_part;
+	h.reserve_pct = fib->reserve_pct;
+	h.n_rich = fib->n_rich;
+	h.n_rich_full = fib->n_rich_full;
+	const uint64_t nb = fib->n_blocks, P = fib->n_parts, kw = (fib->n_keys + 31) / 32;
+	int rc = std::fwrite(&h, sizeof(h), 1, f) == 1 ? TM_OK : set_error(TM_ERR_INVALID, "image write failed");
+	if (rc == TM_OK) rc = device_array_out(f, fib->blk_words, nb * BLOCK_WORDS);
+	if (rc == TM_OK) rc = device_array_out(f, fib->blk_info, nb * 8);
+	if (rc == TM_OK) rc = device_array_out(f, fib->slot_keys, nb * 64);
+	if (rc == TM_OK) rc = device_array_out(f, fib->slot_fid, nb * 64);
+	if (rc == TM_OK) rc = device_array_out(f, fib->d_rich_words, (uint64_t)fib->n_rich * 8);
+	if (rc == TM_OK) rc = device_array_out(f, fib->d_rich_bm, (uint64_t)fib->n_rich * kw);
+	bool ok = rc == TM_OK;
+	ok = ok && write_array(f, fib->h_ovf.data(), fib->h_ovf.size());
+	ok = ok && write_array(f, fib->h_dict.data(), fib->h_dict.size());
+	ok = ok && write_array(f, fib->h_part_info.data(), P * 8);
+	ok = ok && write_array(f, fib->h_part_stat.data(), P * 8);
+	ok = ok && write_array(f, fib->h_part_size.data(), P);
+	ok = ok && write_array(f, fib->h_part_cap.data(), P);
+	ok = ok && write_array(f, fib->h_order.data(), (uint64_t)fib->h_order.size());
+	if (std::fclose(f) != 0) ok = false;
+	if (rc == TM_OK && !ok) rc = set_error(TM_ERR_INVALID, "image write failed: %s", path);
+	return rc;
+}
+
+int tm_fib_load(tm_fib * fib, const char * path) {
+	if (!fib || !path) return set_error(TM_ERR_INVALID, "NULL argument");
+	if (fib->in_flight.load() > 0) return set_error(TM_ERR_BUSY, "a ring on this FIB has a submission in flight");
+	TM_CHECK(require_device());
+	std::FILE * f = std::fopen(path, "rb");
+	if (!f) return set_error(TM_ERR_INVALID, "cannot open %s", path);
+	struct closer {
+		std::FILE * f;
+		~closer() { std::fclose(f); }
+	} c{f};
+	image_header h;
+	if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, IMAGE_MAGIC, 8) != 0)
+		return set_error(TM_ERR_INVALID, "%s is not a FIB image", path);
+	if (h.version != IMAGE_VERSION)
+		return set_error(TM_ERR_INVALID, "%s: image version %u, this library reads %u", path, h.version, IMAGE_VERSION);
+	if (h.n_parts > 0xFFFFFFFFull || h.n_blocks > 0xFFFFFFFFull || h.n_keys > 0xFFFFFFFFull || h.n_rich > MAX_RICH ||
+		h.n_rich_full > h.n_rich)
+		return set_error(TM_ERR_INVALID, "%s: sizes out of range", path);
+	const uint64_t nb = h.n_blocks, P = h.n_parts, kw = (h.n_keys + 31) / 32;
+	std::vector<uint32_t> words, info, fid, rich_w, rich_bm, ovf, dict, pinfo, pstat, psize, pcap;
+	std::vector<uint64_t> skeys;
+	std::vector<int> order;
+	TM_CHECK(read_array(f, words, nb * BLOCK_WORDS, "block words"));
+	TM_CHECK(read_array(f, info, nb * 8, "block info"));
+	TM_CHECK(read_array(f, skeys, nb * 64, "key slots"));
+	TM_CHECK(read_array(f, fid, nb * 64, "filter ids"));
+	TM_CHECK(read_array(f, rich_w, h.n_rich * 8, "rich filters"));
+	TM_CHECK(read_array(f, rich_bm, h.n_rich * kw, "rich key bitmaps"));
+	uint64_t n_ovf = 0;
+	{   // overflow lists: any length
+		const long at = std::ftell(f);
+		if (std::fread(&n_ovf, 8, 1, f) != 1 || std::fseek(f, at, SEEK_SET) != 0)
+			return set_error(TM_ERR_INVALID, "image truncated before overflow lists");
+		if (n_ovf == 0 || n_ovf > 0xFFFFFFFFull) return set_error(TM_ERR_INVALID, "%s: overflow lists out of range", path);
+	}
+	TM_CHECK(read_array(f, ovf, n_ovf, "overflow lists"));
+	TM_CHECK(read_array(f, dict, h.n_keys, "key dictionary"));
+	TM_CHECK(read_array(f, pinfo, P * 8, "partition info"));
+	TM_CHECK(read_array(f, pstat, P * 8, "partition statistics"));
+	TM_CHECK(read_array(f, psize, P, "partition sizes"));
+	TM_CHECK(read_array(f, pcap, P, "partition capacities"));
+	TM_CHECK(read_array(f, order, 192, "bit order"));
+
+	std::lock_guard<std::mutex> lk(fib->host_mu);
+	fib->release();
+	uint64_t bytes = 0;
+	TM_CHECK(upload(&fib->blk_words, words, bytes));
+	TM_CHECK(upload(&fib->blk_info, info, bytes));
+	TM_CHECK(upload(&fib->slot_keys, skeys, bytes));
+	TM_CHECK(upload(&fib->slot_fid, fid, bytes));
+	fib->ovf_cap = std::max<size_t>(ovf.size() * 2, 1024);
+	TM_CHECK(dalloc(&fib->ovf, fib->ovf_cap));
+	TM_CHECK(put(fib->ovf, ovf));
+	bytes += fib->ovf_cap * 4;
+	TM_CHECK(upload(&fib->key_dict, dict, bytes));
+	TM_CHECK(upload(&fib->part_info, pinfo, bytes));
+	TM_CHECK(upload(&fib->part_stat, pstat, bytes));
+	if (h.n_rich) {
+		TM_CHECK(upload(&fib->d_rich_words, rich_w, bytes));
+		TM_CHECK(upload(&fib->d_rich_bm, rich_bm, bytes));
+	}
+	fib->n_rich = (uint32_t)h.n_rich;
+	fib->n_rich_full = (uint32_t)h.n_rich_full;
+	fib->n_parts = (uint32_t)P;
+	fib->n_blocks = (uint32_t)nb;
+	fib->n_keys = (uint32_t)h.n_keys;
+	fib->n_filters = h.n_filters;
+	fib->n_key_instances = h.n_key_instances;
+	fib->max_part = h.max_part;
+	fib->reserve_pct = (uint32_t)h.reserve_pct;
+	fib->device_bytes = bytes;
+	fib->h_part_info = std::move(pinfo);
+	fib->h_part_stat = std::move(pstat);
+	fib->h_part_size = std::move(psize);
+	fib->h_part_cap = std::move(pcap);
+	fib->h_ovf = std::move(ovf);
+	fib->h_dict = std::move(dict);
+	fib->h_order = std::move(order);
+	fib->ovf_valid = (uint32_t)fib->h_ovf.size();
+	fib->prep_epoch = fib->epoch;
+	fib->version++;
+	return TM_OK;
+}
+
+} // extern "C"

@@ -163,3 +163,82 @@ def test_live_insert_declines_and_graph_follows_epochs(gpu):
         bw, bko, bk, _ = gen(300, 1, seed=440 + step)
         assert fib.insert_live(bw, bko, bk)
         w, ko, k = concat(w, ko, k, bw, bko, bk)
+
+
+def test_image_round_trip(gpu, tmp_path):
+    """tm_fib_save / tm_fib_load: a table built, then grown by inserts (new
+    keys and partitions), saved and loaded into a new FIB answers as the
+    oracle on the whole table; the loaded FIB takes further inserts; rich
+    filters travel in the image."""
+    cw, ck, cv, q, _ = tm.workload(30000, 400, seed=411)
+    fib, *_ = tm.build_fib(cw, ck, cv, 3000, reserve=0)
+    bw, _, _, _ = gen(3000, 1, seed=412)
+    rng = np.random.default_rng(413)
+    counts = rng.integers(1, 9, len(bw))
+    bko = np.zeros(len(bw) + 1, np.uint32)
+    np.cumsum(counts, out=bko[1:])
+    bk = np.concatenate([np.sort(rng.choice(np.arange(8_000_000, 8_000_050), c, replace=False))
+                         for c in counts]).astype(np.uint32)
+    fib.insert(bw, bko, bk, maxp=3000)
+    w, ko, k = concat(cw, ck, cv, bw, bko, bk)
+    path = tmp_path / "t.fib"
+    fib.save(str(path))
+    log(f"image: {path.stat().st_size} bytes")
+    g = tm.Fib.from_image(str(path))
+    a, b = fib.stats(), g.stats()
+    for f in ("n_filters", "n_parts", "n_blocks", "n_keys", "n_distinct_keys", "max_part_size"):
+        assert a[f] == b[f], f
+    for sched in (tm.SCHED_PERSISTENT, tm.SCHED_PLAIN):
+        check_all(g, w, ko, k, q, schedule=sched)
+    # a loaded FIB grows like a built one
+    cw3, ck3, cv3, _ = gen(2000, 1, seed=414)
+    g.insert(cw3, ck3, cv3, maxp=3000)
+    w2, ko2, k2 = concat(w, ko, k, cw3, ck3, cv3)
+    check_all(g, w2, ko2, k2, q)
+    # loading over a FIB replaces its table
+    fib.load(str(path))
+    check_all(fib, w, ko, k, q)
+
+    # rich filters (every key of the table) are part of the image
+    K = np.unique(cv)
+    extra = np.zeros((1, 6), np.uint32)
+    extra[0, 1] = 0b1011
+    rw = np.concatenate([cw, extra])
+    rko = np.concatenate([ck, [ck[-1] + len(K)]]).astype(np.uint64)
+    rk = np.concatenate([cv, K]).astype(np.uint32)
+    rw, rko, rk = tm.canonicalize(rw, rko, rk)
+    rf, *_ = tm.build_fib(rw, rko, rk, 3000)
+    rq = q.copy()
+    rq[::2, 1] |= 0b1011
+    rf.save(str(path))
+    rg = tm.Fib.from_image(str(path))
+    eo, ev = orc.match(rw, rko, rk, rq, mode=0)
+    for fe in (1, 2):
+        off, vals, _ = tm.match(rg, rq, full_exit=fe)
+        assert np.array_equal(off, eo) and np.array_equal(vals, ev), fe
+
+
+def test_image_errors(gpu, tmp_path):
+    cw, ck, cv, q, _ = tm.workload(5000, 10, seed=415)
+    fib, *_ = tm.build_fib(cw, ck, cv, 1000)
+    p = tmp_path / "t.fib"
+    fib.save(str(p))
+    data = p.read_bytes()
+    (tmp_path / "cut.fib").write_bytes(data[: len(data) // 2])
+    (tmp_path / "bad.fib").write_bytes(b"x" * 200)
+    v2 = bytearray(data)
+    v2[8] = 99                                   # image version
+    (tmp_path / "v99.fib").write_bytes(bytes(v2))
+    for name, what in (("cut.fib", "truncated"), ("bad.fib", "not a FIB image"), ("v99.fib", "version"),
+                       ("missing.fib", "cannot open")):
+        with pytest.raises(tm.TagMatchError) as e:
+            tm.Fib.from_image(str(tmp_path / name))
+        assert e.value.code == tm.TM_ERR_INVALID and what in str(e.value), name
+    # a failed load leaves the FIB as it was
+    with pytest.raises(tm.TagMatchError):
+        fib.load(str(tmp_path / "cut.fib"))
+    check_all(fib, cw, ck, cv, q)
+    empty = tm.Fib.empty()
+    with pytest.raises(tm.TagMatchError) as e:
+        empty.save(str(tmp_path / "e.fib"))
+    assert e.value.code == tm.TM_ERR_STATE
