@@ -97,25 +97,45 @@ def setup(args):
     return cw, cko, ck, q, fib, perm, off, masks, t3 - t0
 
 
-def cpu_baseline(cw, off_parts, masks, perm_words, q, budget_s):
-    """The oracle's CPU schedule (partition lookup + per-partition brute
-    force, restating the reference's CPU back end) on a bounded sample of
-    the same queries, on the host's cores."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
-    n = 16
+def _sized(fn, q, budget_s, n=16):
+    """Runs fn on a prefix of q sized to about budget_s seconds; returns
+    (queries run, seconds, fn's result)."""
     t = time.time()
-    orc.schedule_count(perm_words, off_parts, masks, q[:n], threads)
+    fn(q[:n])
     dt = max(time.time() - t, 1e-3)
     n2 = int(min(len(q), max(n, n * budget_s / dt)))
     t = time.time()
-    hits = orc.schedule_count(perm_words, off_parts, masks, q[:n2], threads)
-    dt = time.time() - t
-    return {"value": n2 / dt, "unit": "queries/s", "cores": threads, "kind": "port",
+    r = fn(q[:n2])
+    return n2, time.time() - t, r
+
+
+def cpu_baseline(cw, cko, ck, off_parts, masks, perm_words, q, budget_s):
+    """CPU baselines on bounded samples of the same queries, on the host's
+    cores: the reference's compact Patricia-trie matcher with match_unique
+    results (keys collected, sorted, deduplicated), built from the
+    reference's sources into oracle/_ref, when present; and the oracle's
+    schedule (partition lookup + per-partition brute force, hit counts only,
+    restating the reference's CPU back end)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    threads = min(16, os.cpu_count() or 1)
+    n2, dt, hits = _sized(lambda x: orc.schedule_count(perm_words, off_parts, masks, x, threads), q, budget_s / 2)
+    port = {"value": n2 / dt, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"first {n2} of the {len(q)} queries against all {cw.shape[0]} filters "
-                      f"(oracle lookup + per-partition brute force, {threads} threads, {dt:.1f}s)",
+                      f"(oracle lookup + per-partition brute force, hit counts only, {threads} threads, {dt:.1f}s)",
             "hits_per_query": hits / n2}
+    if not orc.ref_available("libref_patricia.so"):
+        return port
+    t = time.time()
+    trie = orc.RefPatricia(cw, cko, ck)
+    tb = time.time() - t
+    n3, dt3, (cnt, tot) = _sized(lambda x: trie.count(x, threads), q, budget_s / 2)
+    trie.close()
+    return {"value": n3 / dt3, "unit": "queries/s", "cores": threads, "kind": "reference",
+            "sample": f"first {n3} of the {len(q)} queries against all {cw.shape[0]} filters (reference compact "
+                      f"Patricia trie, built in {tb:.1f}s; each query's keys collected, sorted and deduplicated; "
+                      f"{threads} threads, {dt3:.1f}s)",
+            "keys_per_query": tot / n3, "port": port}
 
 
 def profile_figures():
@@ -486,7 +506,7 @@ def main():
     cpu = None
     if args.cpu_seconds > 0:
         # the same partitions, in partitioned order, for the CPU schedule
-        cpu = cpu_baseline(cw, part_off, masks, cw[perm], q, args.cpu_seconds)
+        cpu = cpu_baseline(cw, cko, ck, part_off, masks, cw[perm], q, args.cpu_seconds)
 
     out = {
         "metric": METRIC,

@@ -227,3 +227,59 @@ def ref_bitvector(pairs, workdir):
         "next_bit": raw[:, 8:12].copy().view(np.uint32).ravel(),
         "words": raw[:, 12:36].copy().view(np.uint32).reshape(-1, 6),
     }
+
+
+class RefPatricia:
+    """The reference compact Patricia-trie matcher (_ref/libref_patricia.so,
+    built from compact_patricia_predicate.hh and key_array.cc) over one
+    table: match_unique results on the host's threads."""
+
+    _lib = None
+
+    def __init__(self, words, key_off, keys):
+        if RefPatricia._lib is None:
+            lib = C.CDLL(os.path.join(REF_DIR, "libref_patricia.so"))
+            lib.rp_build.restype = P
+            lib.rp_build.argtypes = [P, C.c_uint64, P, P]
+            lib.rp_nodes.restype = C.c_uint64
+            lib.rp_nodes.argtypes = [P]
+            lib.rp_match_unique.restype = C.c_uint64
+            lib.rp_match_unique.argtypes = [P, P, C.c_uint64, C.c_int, P, P, P]
+            lib.rp_free.restype = None
+            lib.rp_free.argtypes = [P]
+            RefPatricia._lib = lib
+        w = _w(words).reshape(-1, 6)
+        ko = np.ascontiguousarray(key_off, np.uint64)
+        kv = _w(keys)
+        self.h = RefPatricia._lib.rp_build(_p(w), w.shape[0], _p(ko), _p(kv))
+        if not self.h:
+            raise ValueError("reference Patricia trie: empty table or a filter with more than 65535 keys")
+        self.n_nodes = int(RefPatricia._lib.rp_nodes(self.h))
+
+    def count(self, queries, threads=1):
+        """Distinct keys per query (and their total), keys not kept."""
+        q = _w(queries).reshape(-1, 6)
+        cnt = np.zeros(q.shape[0], np.uint32)
+        tot = RefPatricia._lib.rp_match_unique(self.h, _p(q), q.shape[0], threads, _p(cnt), None, None)
+        return cnt, int(tot)
+
+    def match_unique(self, queries, threads=1):
+        """(offsets, keys): each query's distinct keys, sorted."""
+        q = _w(queries).reshape(-1, 6)
+        cnt, tot = self.count(q, threads)
+        off = np.zeros(q.shape[0] + 1, np.uint64)
+        np.cumsum(cnt, out=off[1:])
+        out = np.zeros(max(tot, 1), np.uint32)
+        RefPatricia._lib.rp_match_unique(self.h, _p(q), q.shape[0], threads, _p(cnt), _p(off), _p(out))
+        return off, out[:tot]
+
+    def close(self):
+        if self.h:
+            RefPatricia._lib.rp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -328,3 +328,38 @@ def test_oracle_vs_live_reference():
     order, pid, masks, sizes = orc.ref_partition(cw, 300, d, threads=3)
     rc, perm, poff, pmasks = orc.partition(cw, 300)
     assert np.array_equal(order, perm)
+
+
+@pytest.mark.skipif(not orc.ref_available("libref_patricia.so"), reason="oracle/_ref not built")
+def test_oracle_vs_reference_patricia():
+    """The reference compact Patricia-trie matcher (the CPU baseline) and the
+    oracle give the same match_unique results; a table with repeated filters
+    merges their keys, and results do not depend on the thread count."""
+    import tagmatch_amd as tm
+    cw, cko, ck, q, s = tm.workload(20000, 300, seed=2025)
+    t = orc.RefPatricia(cw, cko, ck)
+    assert t.n_nodes == len(cw)
+    off, vals = orc.match(cw, cko, ck, q, mode=0)
+    po, pv = t.match_unique(q, threads=1)
+    assert np.array_equal(po, off.astype(np.uint64)) and np.array_equal(pv, vals)
+    po3, pv3 = t.match_unique(q, threads=3)
+    assert np.array_equal(po3, po) and np.array_equal(pv3, pv)
+    # raw (not canonical) table: every filter twice, keys split between copies
+    g = tm.Generator(seed=7)
+    rw, rko, rk = g.filters(3000)
+    n = len(rw)
+    half = rko[:-1] + (rko[1:] - rko[:-1]) // 2
+    w2 = np.concatenate([rw, rw])
+    k1 = [rk[int(rko[i]):int(half[i])] for i in range(n)]
+    k2 = [rk[int(half[i]):int(rko[i + 1])] for i in range(n)]
+    lists = k1 + k2
+    ko2 = np.zeros(2 * n + 1, np.uint64)
+    np.cumsum([len(x) for x in lists], out=ko2[1:])
+    kv2 = np.concatenate(lists).astype(np.uint32)
+    t2 = orc.RefPatricia(w2, ko2, kv2)
+    cw2, cko2, ck2 = tm.canonicalize(rw, rko, rk)
+    assert t2.n_nodes == len(cw2)
+    q2, _ = g.queries(200, 3000)
+    o2, v2 = orc.match(cw2, cko2, ck2, q2, mode=0)
+    po, pv = t2.match_unique(q2, threads=2)
+    assert np.array_equal(po, o2.astype(np.uint64)) and np.array_equal(pv, v2)
