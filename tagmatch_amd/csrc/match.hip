@@ -923,7 +923,11 @@ k_bm_count(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t
 // which is ascending key order (the dictionary is sorted).  Lanes take 64
 // consecutive key ids at a time: a set bit's output slot is the count of set
 // bits below it (mbcnt), so each store writes consecutive values (rows are
-// mostly dense) and the dictionary reads are coalesced.
+// mostly dense) and the dictionary reads are coalesced.  The dictionary
+// reads of BM_GROUP such 64-id groups are issued before any of their stores,
+// so a wave waits for one round trip per BM_GROUP groups, not per group.
+constexpr uint32_t BM_GROUP = 8;
+
 __global__ void __launch_bounds__(256)
 k_bm_write(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint64_t * __restrict__ off,
 		   const uint32_t * __restrict__ dict, uint32_t * __restrict__ vals, uint64_t cap) {
@@ -934,15 +938,25 @@ k_bm_write(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const ui
 	for (uint32_t wb = 0; wb < kw; wb += 64) {   // 64 words (2048 ids) per pass
 		const uint32_t v = wb + lane < kw ? row[wb + lane] : 0u;
 		const uint32_t nw = min(64u, kw - wb);
-		for (uint32_t c = 0; c < nw; c += 2) {
-			const uint64_t m = (uint64_t)(uint32_t)__shfl(v, (int)c, 64) |
-							   ((uint64_t)(uint32_t)__shfl(v, (int)c + 1, 64) << 32);
-			if (m == 0) continue;
-			if ((m >> lane) & 1) {
-				const uint64_t p = pos + mbcnt64(m);
-				if (p < cap) vals[p] = dict[(wb + c) * 32 + lane];
+		for (uint32_t c0 = 0; c0 < nw; c0 += 2 * BM_GROUP) {
+			uint64_t m[BM_GROUP];
+			uint32_t d[BM_GROUP];
+#pragma unroll
+			for (uint32_t k = 0; k < BM_GROUP; ++k) {
+				const uint32_t c = c0 + 2 * k;   // words c, c + 1 (c + 1 < 64: shuffles stay in the wave)
+				m[k] = (uint64_t)(uint32_t)__shfl(v, (int)c, 64) | ((uint64_t)(uint32_t)__shfl(v, (int)c + 1, 64) << 32);
+				// a set bit names a key id below the dictionary's size (bits past
+				// nw words are zero: v is zero past the row's kw words)
+				d[k] = ((m[k] >> lane) & 1) ? dict[(wb + c) * 32 + lane] : 0u;
 			}
-			pos += (uint64_t)__popcll(m);
+#pragma unroll
+			for (uint32_t k = 0; k < BM_GROUP; ++k) {
+				if ((m[k] >> lane) & 1) {
+					const uint64_t p = pos + mbcnt64(m[k]);
+					if (p < cap) vals[p] = d[k];
+				}
+				pos += (uint64_t)__popcll(m[k]);
+			}
 		}
 	}
 }
