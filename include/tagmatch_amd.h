@@ -398,7 +398,15 @@ typedef struct tm_server_config {
 	                            (0 = 16384)                                   */
 	uint64_t hbm_budget;     /* result bytes admitted at once (0 = a quarter
 	                            of the free HBM at creation)                  */
-	uint32_t reserved[8];
+	/* Latency flush (front_end.cc:249-267, :563-598: a batch is flushed when
+	 * full or when its oldest query passes the latency limit).  With
+	 * flush_queries > 0 a free slot waits until min(flush_queries,
+	 * slot_queries) queries are waiting, or until the oldest waiting request
+	 * has waited flush_us microseconds, or an insert is queued behind them,
+	 * or the server stops.  0 (the default) starts a step at once. */
+	uint32_t flush_queries;
+	uint32_t flush_us;
+	uint32_t reserved[6];
 } tm_server_config;
 
 typedef struct tm_server_stats {

@@ -225,6 +225,25 @@ void complete(tm_server * s, slot & sl) {
 	s->cv_done.notify_all();
 }
 
+// Latency flush: whether free slots should wait for more queries.  True
+// while fewer than min(flush_queries, slot_queries) queries wait at the head
+// of the queue, the oldest of them has waited less than flush_us, no insert
+// waits behind them and the server is not stopping.  Called under s->mu.
+bool holding(tm_server * s, clock_t_::time_point now) {
+	if (s->cfg.flush_queries == 0 || s->waiting.empty() || s->stop.load()) return false;
+	const request * first = s->waiting.front();
+	if (first->is_insert) return false;
+	if (now - first->t_submit >= std::chrono::microseconds(s->cfg.flush_us)) return false;
+	const uint64_t want = std::min(s->cfg.flush_queries, s->cfg.slot_queries);
+	uint64_t nq = 0;
+	for (const request * r : s->waiting) {
+		if (r->is_insert) return false;
+		nq += r->nq;
+		if (nq >= want) return false;
+	}
+	return true;
+}
+
 void engine(tm_server * s) {
 	for (;;) {
 		bool progressed = false;
@@ -237,8 +256,9 @@ void engine(tm_server * s) {
 		}
 		// 2. admission (and insert barriers), under the lock
 		std::unique_lock<std::mutex> lk(s->mu);
+		const bool hold = holding(s, clock_t_::now());
 		for (auto & sl : s->slots) {
-			if (sl.busy || s->waiting.empty()) continue;
+			if (hold || sl.busy || s->waiting.empty()) continue;
 			if (s->waiting.front()->is_insert) break;
 			uint32_t nq = 0;
 			uint64_t bytes = 0;
@@ -380,6 +400,12 @@ void engine(tm_server * s) {
 				}
 				lk.lock();
 				if (!ready && s->waiting.empty()) s->cv_engine.wait_for(lk, std::chrono::microseconds(20));
+			} else if (hold) {
+				// a latency flush is pending: sleep until its deadline or an arrival
+				// (requests are queued under the lock, so none is missed)
+				const auto due = s->waiting.front()->t_submit + std::chrono::microseconds(s->cfg.flush_us);
+				s->cv_engine.wait_until(lk, std::min(due, clock_t_::now() + std::chrono::milliseconds(5)),
+										[&] { return !holding(s, clock_t_::now()); });
 			} else {
 				s->cv_engine.wait_for(lk, std::chrono::milliseconds(5));   // idle: sleep on the queue
 			}

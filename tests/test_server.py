@@ -109,3 +109,52 @@ def test_inserts_ordered_with_requests(gpu):
     eo2, ev2 = orc.match(w, ko, kv, q, mode=0)
     assert np.array_equal(o2, eo2) and np.array_equal(v2, ev2)
     assert not np.array_equal(o1, o2)            # the insert changed some results
+
+
+def test_latency_flush(table):
+    """flush_queries / flush_us: a free slot waits until enough queries are
+    waiting (one step for ten requests), or until the oldest request has
+    waited the latency limit, or an insert is queued behind the requests."""
+    import time
+    cw, ck, cv, q, fib = table
+    eo, ev, _ = tm.match(fib, q[:640])
+    srv = tm.Server(fib, n_slots=2, slot_queries=4096, flush_queries=640, flush_us=5_000_000)
+    ids = [srv.submit(q[i * 64:(i + 1) * 64]) for i in range(10)]
+    res = []
+    for rid in ids:
+        assert srv.wait(rid, 60000.0)
+        res.append(srv.result(rid))
+    st = srv.stats()
+    log(f"flush at 640 queries: {st['n_batches']} steps for {st['n_requests']} requests")
+    assert st["n_batches"] == 1 and st["requests_per_batch_sum"] == 10
+    for i, (info, off, vals) in enumerate(res):
+        a, b = eo[i * 64], eo[(i + 1) * 64]
+        assert info.step_queries == 640
+        assert np.array_equal(off, eo[i * 64:(i + 1) * 64 + 1] - a) and np.array_equal(vals, ev[a:b])
+    srv.close()
+
+    # below the threshold: the step starts at the latency limit
+    srv = tm.Server(fib, n_slots=2, slot_queries=4096, flush_queries=4096, flush_us=50_000)
+    t = time.time()
+    rid = srv.submit(q[:64])
+    assert srv.wait(rid, 60000.0)
+    dt = time.time() - t
+    info, off, vals = srv.result(rid)
+    log(f"latency limit 50 ms: waited {info.queue_ms:.1f} ms in the queue, {dt * 1e3:.1f} ms in all")
+    assert info.queue_ms >= 49.0 and dt < 5.0
+    assert np.array_equal(off, eo[:65]) and np.array_equal(vals, ev[:eo[64]])
+
+    # an insert behind a held request flushes it
+    fib2, *_ = tm.build_fib(cw, ck, cv, 5000)      # the insert changes this table only
+    srv2 = tm.Server(fib2, n_slots=2, slot_queries=4096, flush_queries=4096, flush_us=30_000_000)
+    g = tm.Generator(seed=604)
+    bw, bko, bk = g.filters(10)
+    t = time.time()
+    rid = srv2.submit(q[:64])
+    iid = srv2.insert(bw, bko, bk)
+    assert srv2.wait(rid, 60000.0) and srv2.wait(iid, 60000.0)
+    assert time.time() - t < 10.0
+    info, off, vals = srv2.result(rid)
+    assert np.array_equal(off, eo[:65]) and np.array_equal(vals, ev[:eo[64]])
+    srv2.close()
+    srv.close()
