@@ -926,12 +926,16 @@ k_bm_count(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t
 // mostly dense) and the dictionary reads are coalesced.  The dictionary
 // reads of BM_GROUP such 64-id groups are issued before any of their stores,
 // so a wave waits for one round trip per BM_GROUP groups, not per group.
+// A workgroup takes BM_WAVES consecutive queries: their outputs are one
+// contiguous stretch written from one CU (16 per group measured 0.6 ms
+// faster per 1M-query step than 4, and 1 slower than 4).
 constexpr uint32_t BM_GROUP = 8;
+constexpr uint32_t BM_WAVES = 16;
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(BM_WAVES * 64)
 k_bm_write(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint64_t * __restrict__ off,
 		   const uint32_t * __restrict__ dict, uint32_t * __restrict__ vals, uint64_t cap) {
-	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+	const uint32_t qi = blockIdx.x * BM_WAVES + (threadIdx.x >> 6), lane = threadIdx.x & 63;
 	if (qi >= nq) return;
 	const uint32_t * row = gbm + (size_t)qi * kw;
 	uint64_t pos = off[qi];
@@ -1523,8 +1527,8 @@ void enqueue_finalize(tm_ring * r, uint32_t nq) {
 	if (mode == TM_MODE_KEYS_UNIQUE) {
 		hipLaunchKernelGGL(k_bm_count, dim3(gq), dim3(256), 0, s, r->d_gbm, r->kw, nq, r->d_qcount);
 		scan_u32_to_u64(r->d_qcount, r->d_off, nq, r->d_scan, s);
-		hipLaunchKernelGGL(k_bm_write, dim3(gq), dim3(256), 0, s, r->d_gbm, r->kw, nq, r->d_off,
-						   r->fib->key_dict, r->d_vals, r->val_cap);
+		hipLaunchKernelGGL(k_bm_write, dim3(std::max(1u, (nq + BM_WAVES - 1) / BM_WAVES)), dim3(BM_WAVES * 64), 0, s,
+						   r->d_gbm, r->kw, nq, r->d_off, r->fib->key_dict, r->d_vals, r->val_cap);
 	} else {
 		const bool keys = mode == TM_MODE_KEYS_ALL;
 		dev_fib_view v = r->fib->view();
