@@ -1,4 +1,4 @@
-// Query with matching state (reference tagmatch_query.hh:125-220).
+// Query with matching state (reference tagmatch_query.hh:25-120).
 //
 // The engine resolves a query in one device submission, so the per-partition
 // bookkeeping of the reference (partition_enqueue / partition_done) is kept

@@ -67,12 +67,12 @@ int tm_dev_arch(char * buf, size_t len);
 /* ------------------------------------------------------------------------ */
 /* Partitioner: the balanced partitioning of a filter table, run on the GPU.
  * Replaces partitioning::balanced_partitioning + partitioner_gpu
- * (reference partitioning.cc:382-432, partitioner_gpu.cu:15-158) with a
+ * (reference partitioning.cc:361-411, partitioner_gpu.cu:15-158) with a
  * level-synchronous restatement of the same split rules: bit histograms,
  * pivot choice and stable pivot splits all run as HIP kernels.
  *
  *   words      : n filters (n x 6 words), usually lexicographically sorted.
- *   maxp       : maximum partition size (reference partitioning.cc:366).
+ *   maxp       : maximum partition size (reference partitioning.cc:342-347).
  *   perm       : out, n entries; filter perm[i] of the input is at position i
  *                of the partitioned order.  Each partition is a contiguous
  *                run and keeps the input's relative order.
@@ -80,9 +80,9 @@ int tm_dev_arch(char * buf, size_t len);
  *                [part_offsets[p], part_offsets[p+1]).
  *   part_masks : out, capacity max_parts x 6; AND of the partition's filters
  *                (all-ones for an empty partition, reference
- *                partitioning.cc:412-423).  A partition of all-zero filters
+ *                partitioning.cc:389-402).  A partition of all-zero filters
  *                has an all-zero mask; the reference cannot terminate on it
- *                (partitioning.cc:251-271), this engine keeps it as a
+ *                (partitioning.cc:230-250), this engine keeps it as a
  *                partition that every query selects.
  *   n_parts    : out, number of partitions.
  * Returns TM_ERR_OVERFLOW if more than max_parts partitions result. */
@@ -93,8 +93,8 @@ int tm_partition(const uint32_t * words, uint32_t n, uint32_t maxp,
 /* ------------------------------------------------------------------------ */
 /* Device FIB: filters, partition masks and keys resident in HBM.
  * Replaces back_end::compile_fibs and its per-partition cudaMalloc/copy
- * (reference back_end.cc:404-494, gpu.cu:825-830); the keys, which the
- * reference keeps on the host (back_end.cc:109-129), are resident too. */
+ * (reference back_end.cc:363-453, gpu.cu:825-830); the keys, which the
+ * reference keeps on the host (back_end.cc:69-88), are resident too. */
 typedef struct tm_fib tm_fib;
 
 typedef struct tm_fib_stats {
@@ -131,7 +131,7 @@ int tm_fib_stats_get(const tm_fib * fib, tm_fib_stats * st);
  * dictionary, partitions and rich filters, as uploaded and inserted so far.
  * Loading replaces the FIB's content without partitioning or laying out
  * blocks again.  Replaces the reference's tagmatch.db round trip
- * (tagmatch.cc:79-142, written as partition_fib_entry, read back as
+ * (tagmatch.cc:79-87, :98-134, written as partition_fib_entry, read back as
  * fib_entry), which re-partitions on every start.  Little-endian hosts;
  * a file of another image version is refused. */
 int tm_fib_save(tm_fib * fib, const char * path);
@@ -148,7 +148,7 @@ int tm_fib_set_reserve(tm_fib * fib, uint32_t percent);
 /* Insert n filters into a resident table without rebuilding it.  The
  * reference adds filters on the host and rebuilds the whole FIB
  * (tagmatch::add + consolidate, tagmatch.cc:22-77, :98-165, then
- * back_end::compile_fibs, back_end.cc:404-494).  Here each filter joins the
+ * back_end::compile_fibs, back_end.cc:363-453).  Here each filter joins the
  * existing partition with the most mask bits among those whose mask it
  * contains and that have a spare lane; filters that fit none are
  * partitioned on their own with maxp and appended as new partitions.  Keys
@@ -179,7 +179,7 @@ int tm_fib_insert_live(tm_fib * fib, const uint32_t * words, uint32_t n, const u
  * A ring is one in-flight submission context on its own HIP stream: a query
  * buffer, the device-side lookup lists and work queue, and output buffers.
  * Rings replace the reference's stream_handle double buffers
- * (reference back_end.cc:172-343) and its 256-query __constant__ upload
+ * (reference back_end.cc:131-311) and its 256-query __constant__ upload
  * (gpu.cu:39-40, :809-811).  Several rings may be in flight at once. */
 typedef struct tm_ring tm_ring;
 
@@ -189,14 +189,14 @@ typedef struct tm_ring tm_ring;
                                   tagmatch_query.hh:99-119)                  */
 #define TM_MODE_KEYS_ALL    1  /* per query: keys of every matched filter,
                                   duplicates kept, order unspecified (match,
-                                  reference tagmatch.cc:239-243)             */
+                                  reference tagmatch.cc:200-204)             */
 #define TM_MODE_HITS        2  /* per query: ids of matched filters, order
                                   unspecified                                */
 
 /* Schedules. */
 #define TM_SCHED_PLAIN      0  /* one kernel launch per (partition, batch),
                                   the reference's dispatch shape
-                                  (back_end.cc:284-327, gpu.cu:878-912)      */
+                                  (back_end.cc:243-286, gpu.cu:878-912)      */
 #define TM_SCHED_PERSISTENT 1  /* one persistent, work-stealing launch over
                                   every (partition, query tile, block chunk) */
 
@@ -292,7 +292,7 @@ int tm_submit_resident(tm_ring * ring, uint32_t nq);
  * n_units + 1 entries from 0 to nq.  query_words: nq x 6 host words, or
  * NULL for the queries already in tm_ring_query_buffer().  This is the
  * reference's back-end entry, a batch of queries for one partition
- * (back_end::process_batch, back_end.cc:256-302), without the front end;
+ * (back_end::process_batch, back_end.cc:243-311), without the front end;
  * results per query as for tm_submit. */
 int tm_submit_units(tm_ring * ring, const uint32_t * query_words, uint32_t nq, const uint32_t * unit_part,
 					const uint32_t * unit_off, uint32_t n_units);
@@ -322,7 +322,7 @@ int tm_match(tm_fib * fib, const uint32_t * query_words, uint32_t nq,
 /* ------------------------------------------------------------------------ */
 /* Learned cost model.  No counterpart in the reference, whose scheduler has
  * no cost estimate (front_end.cc:350-376 flushes batches in arrival order and
- * back_end.cc:256-269 sizes every result buffer for the worst case).
+ * back_end.cc:243-286 sizes every result buffer for the worst case).
  *
  * Work units are (partition, query batch) pairs.  The plain schedule logs one
  * row per launch when cfg.log_units is set: TM_UNIT_FEATURES float features,
@@ -378,7 +378,7 @@ int tm_ring_set_cost_model(tm_ring * ring, tm_mlp * model, uint32_t flags);
 /* ------------------------------------------------------------------------ */
 /* Serving loop.  Replaces the reference front end's batching and latency
  * flush (front_end.cc:249-267, :350-376, :563-598) and its per-GPU double
- * buffering (back_end.cc:172-343) with continuous batching: an engine thread
+ * buffering (back_end.cc:131-311) with continuous batching: an engine thread
  * admits waiting requests (batches of queries) into free slots (rings, each
  * on its own HIP stream) as soon as a slot is free and the requests' result
  * bytes fit the HBM budget, so steps of several slots overlap on the
@@ -469,7 +469,7 @@ int tm_server_stats_get(tm_server * srv, tm_server_stats * st);
 /* ------------------------------------------------------------------------ */
 /* Host: canonical filter table from (filter, keys) entries, as
  * tagmatch::add + consolidate build it (reference tagmatch.cc:22-77,
- * :137-204): identical filters merged with the union of their keys, filters
+ * :98-165): identical filters merged with the union of their keys, filters
  * in the reference's lexicographic order (bitvector.hh:389-397), keys
  * ascending, entries without keys dropped.  Outputs are malloc'd; release
  * them with tm_free(). */

@@ -112,7 +112,7 @@ static int contains(const uint32_t * v, uint64_t n, uint32_t k) {
 }
 
 /* tagmatch.cc:22-77 (keyset_change: new = (old - to_remove) + (to_add -
- * old)), :137-204 (merge, drop empty key sets, map order = lexicographic).
+ * old)), :98-165 (merge, drop empty key sets, map order = lexicographic).
  * Without a previous table old = {} and the result is the set of added keys. */
 int orc_consolidate(const uint32_t * ops, uint64_t n_ops, uint32_t ** words, uint64_t * n_filters,
 					uint64_t ** key_off, uint32_t ** keys) {
@@ -257,7 +257,7 @@ typedef struct {
 
 typedef struct {
 	const uint32_t * words;
-	uint32_t * fib;     /* permutation being partitioned (partitioning.cc:79 fib_t) */
+	uint32_t * fib;     /* permutation being partitioned (partitioning.cc:58 fib_t) */
 	uint32_t * tmp;
 	uint32_t maxp;
 	cand_t * out;       /* finished partitions */
@@ -271,7 +271,7 @@ static int bit_of(const uint32_t * w, unsigned b) { return (w[b >> 5] >> (b & 31
 static void set_bit(uint32_t * w, unsigned b) { w[b >> 5] |= 1u << (b & 31); }
 static int is_empty(const uint32_t * w) { return !(w[0] | w[1] | w[2] | w[3] | w[4] | w[5]); }
 
-/* partition_candidate::compute_frequencies, partitioning.cc:113-134 */
+/* partition_candidate::compute_frequencies, partitioning.cc:92-113 */
 static void frequencies(const part_state * S, const cand_t * c, uint32_t * freq) {
 	memset(freq, 0, 192 * sizeof(uint32_t));
 	for (uint32_t i = c->begin; i < c->end; ++i) {
@@ -286,7 +286,7 @@ static void frequencies(const part_state * S, const cand_t * c, uint32_t * freq)
 	}
 }
 
-/* partition_candidate::split_zero, partitioning.cc:154-169: stable
+/* partition_candidate::split_zero, partitioning.cc:133-149: stable
  * partition (zeros first); c becomes the 1-side, the 0-side is returned. */
 static cand_t split_zero(part_state * S, cand_t * c, unsigned pivot) {
 	uint32_t m = c->begin, k = 0;
@@ -316,7 +316,7 @@ static void push(part_state * S, const cand_t * c) {
 	S->stack[S->n_stack++] = *c;
 }
 
-/* nonzero_mask_partitioning, partitioning.cc:251-271 */
+/* nonzero_mask_partitioning, partitioning.cc:230-250 */
 static void nonzero_mask(part_state * S, cand_t p) {
 	uint32_t freq[192];
 	for (;;) {
@@ -349,7 +349,7 @@ static void nonzero_mask(part_state * S, cand_t p) {
 
 static uint32_t udist(uint32_t a, uint32_t b) { return a > b ? a - b : b - a; }
 
-/* balancing_pivot, partitioning.cc:283-299 */
+/* balancing_pivot, partitioning.cc:262-278 */
 static unsigned balancing_pivot(const cand_t * p, const uint32_t * freq) {
 	unsigned pivot;
 	uint32_t half = (p->end - p->begin) / 2;
@@ -368,7 +368,7 @@ static unsigned balancing_pivot(const cand_t * p, const uint32_t * freq) {
 	return pivot;
 }
 
-/* do_balanced_partitioning, partitioning.cc:301-361, with an explicit stack
+/* do_balanced_partitioning, partitioning.cc:280-340, with an explicit stack
  * in place of the work queue and thread pool. */
 static void balanced(part_state * S) {
 	uint32_t freq[192];
@@ -411,7 +411,7 @@ static int cmp_cand(const void * a, const void * b) {
 	return x->end < y->end ? -1 : x->end > y->end;
 }
 
-/* balanced_partitioning, partitioning.cc:382-432; partitions are reported
+/* balanced_partitioning, partitioning.cc:361-411; partitions are reported
  * in position order, masks = AND of members (all-ones if empty). */
 int orc_partition(const uint32_t * words, uint32_t n, uint32_t maxp, uint32_t * perm,
 				  uint32_t * part_off, uint32_t * masks, uint32_t max_parts, uint32_t * n_parts) {
@@ -543,7 +543,7 @@ static void * brute_worker(void * arg) {
 }
 
 /* lookup then per-partition brute force: front_end.cc:465-477 and the
- * CPU back end of back_end.cc:535-557 */
+ * CPU back end of back_end.cc:490-516 */
 static void * sched_worker(void * arg) {
 	count_job * j = (count_job *)arg;
 	uint32_t * sel = (uint32_t *)malloc((j->lk->np ? j->lk->np : 1) * sizeof(uint32_t));

@@ -35,7 +35,7 @@ void orc_from_string(const char * s, uint32_t * out);
 /* Consolidate a list of (filter, key) additions and (filter, key) removals
  * into the canonical table: identical filters merged, key sets union of
  * additions minus removals, empty key sets dropped, filters in lexicographic
- * order, keys ascending (tagmatch.cc:22-77, :137-204).
+ * order, keys ascending (tagmatch.cc:22-77, :98-165).
  *   ops: n_ops records of {6 words, key, op} with op 0 = add, 1 = remove.
  * Outputs are malloc'd; free with orc_free. */
 int orc_consolidate(const uint32_t * ops, uint64_t n_ops,
@@ -51,7 +51,7 @@ int orc_match(const uint32_t * words, uint64_t n, const uint64_t * key_off, cons
 			  const uint32_t * queries, uint64_t nq, int mode, int threads,
 			  uint64_t * off, uint32_t ** vals);
 
-/* Recursive balanced partitioning (partitioning.cc:251-432), single thread,
+/* Recursive balanced partitioning (partitioning.cc:230-411), single thread,
  * on filters in the given order.  perm[i] = input index at partitioned
  * position i; part_off[np+1]; masks np x 6 (AND of members, all-ones when
  * empty).  Returns -1 on an all-zero filter (the reference does not
@@ -68,7 +68,7 @@ uint32_t orc_lookup_query(const orc_lookup * lk, const uint32_t * q, uint32_t * 
 void orc_lookup_free(orc_lookup * lk);
 
 /* CPU schedule of the reference back end (front_end.cc:465-477 lookup, then
- * back_end.cc:535-557 per-partition brute force); returns (query, filter)
+ * back_end.cc:490-516 per-partition brute force); returns (query, filter)
  * hit count.  Used as a CPU baseline. */
 uint64_t orc_schedule_count(const uint32_t * words, const uint32_t * part_off, const uint32_t * masks,
 							uint32_t np, const uint32_t * queries, uint64_t nq, int threads);

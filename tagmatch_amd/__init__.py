@@ -437,8 +437,10 @@ def build_fib(words, key_off, keys, maxp, reserve=None):
 
 def zipf_runs(n, smin, smax, s=1.2, seed=1):
     """Offsets cutting n consecutive filters into runs of Zipf(s) sizes in
-    [smin, smax] (a power law p(size) ~ size^-s); the last run takes the
-    remainder (at least 1)."""
+    [smin, smax] (a power law p(size) ~ size^-s).  The last run takes the
+    remainder; a remainder below smin joins the run before it, and when
+    that would pass smax the two are split in half (both then lie in
+    [smin, smax] as long as smax >= 2 * smin)."""
     rng = np.random.default_rng(seed)
     off = [0]
     a = 1.0 - s
@@ -447,6 +449,10 @@ def zipf_runs(n, smin, smax, s=1.2, seed=1):
         u = rng.random()
         size = int(round((lo + u * (hi - lo)) ** (1.0 / a)))
         off.append(min(n, off[-1] + max(size, 1)))
+    if len(off) > 2 and off[-1] - off[-2] < smin:
+        off.pop(-2)
+        if off[-1] - off[-2] > smax:
+            off.insert(-1, off[-2] + (off[-1] - off[-2]) // 2)
     return np.array(off, np.uint32)
 
 

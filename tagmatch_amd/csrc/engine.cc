@@ -3,7 +3,7 @@
 // (class tagmatch) over the C ABI.
 //
 // Reference counterparts:
-//   tagmatch::add/remove/consolidate   tagmatch.cc:22-77, :137-204
+//   tagmatch::add/remove/consolidate   tagmatch.cc:22-77, :98-165
 //   front_end::match / worker threads  front_end.cc:542-551, :563-691
 //   front_end::stop (drain)            front_end.cc:693-771
 //   tagmatch_query::finalize_matching  tagmatch_query.hh:99-119
@@ -108,7 +108,7 @@ struct edit {
 // New table = old table with the edits applied: per filter,
 // new keys = (old - removed) + (added - old), ascending; filters whose key
 // set becomes empty are dropped; filters in lexicographic order
-// (keyset_change::apply, reference tagmatch.cc:75-99, and :137-204).
+// (keyset_change::apply, reference tagmatch.cc:36-60, and :98-165).
 table apply_edits(const table & old, const std::vector<edit> & ed) {
 	const uint64_t n_old = old.size(), n_ed = ed.size();
 	std::vector<sort_rec> rec(n_old + n_ed);
@@ -159,8 +159,8 @@ std::mutex g_api_mtx;                      // serialises add/consolidate/start/s
 std::vector<edit> g_updates;
 table g_table;
 bool g_consolidated = false;
-unsigned g_psize = 1000, g_pthreads = 1;   // reference tagmatch.cc:128-129
-unsigned g_threads = 4, g_gpus = 1;        // reference tagmatch.cc:210-211
+unsigned g_psize = 1000, g_pthreads = 1;   // reference tagmatch.cc:89-90
+unsigned g_threads = 4, g_gpus = 1;        // reference tagmatch.cc:171-172
 std::string g_dbname;                      // empty: no database file
 std::atomic<int> g_status(TM_OK);
 std::atomic<unsigned> g_latency_ms(0);

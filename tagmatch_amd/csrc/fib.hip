@@ -1,5 +1,5 @@
 // FIB compilation and upload: the host-side layout pass that replaces the
-// reference's back_end::compile_fibs (reference back_end.cc:404-494).
+// reference's back_end::compile_fibs (reference back_end.cc:363-453).
 //
 // Differences from the reference, by design:
 //   * one structure-of-arrays table for all partitions, 64 filters per
@@ -7,7 +7,7 @@
 //   * per-block AND masks, used by the match kernel to discard whole blocks
 //     for a tile of queries before testing single filters;
 //   * keys resident in HBM as dense ids (reference keeps them on the host,
-//     back_end.cc:109-129, and expands hits there, back_end.cc:329-343);
+//     back_end.cc:69-88, and expands hits there, back_end.cc:288-301);
 //   * within a partition filters are re-ordered so that the rarest bits are
 //     the most significant sort positions, which makes block masks sparser
 //     in common bits and more selective (the partition's membership, and so
@@ -240,12 +240,15 @@ struct ins_rec {
 // Writes each inserted filter into its lane and folds it into the block's
 // AND mask and valid-lane count.
 __global__ void __launch_bounds__(256)
-k_insert_lanes(const ins_rec * __restrict__ recs, uint32_t n, uint32_t * __restrict__ blk_words,
+k_insert_lanes(const ins_rec * __restrict__ recs, uint32_t n, uint32_t n_blocks, uint32_t * __restrict__ blk_words,
 			   uint32_t * __restrict__ blk_info, uint64_t * __restrict__ slot_keys, uint32_t * __restrict__ slot_fid,
 			   uint32_t * __restrict__ slot_epoch, uint32_t epoch) {
 	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
 	if (i >= n) return;
 	const ins_rec r = recs[i];
+	// records are made against the current table; one naming a block it
+	// does not have is dropped rather than written out of bounds
+	if (r.block >= n_blocks || r.lane >= 64) return;
 	// a submission older than this insert sees the lane with its old epoch
 	// (none yet: invisible) or the new one (newer than it: invisible)
 	if (slot_epoch) slot_epoch[(size_t)r.block * 64 + r.lane] = epoch;
@@ -346,7 +349,7 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 
 	// Every partition mask must be a subset of each member filter, or the
 	// lookup would drop matches (reference front_end.cc:465-477 relies on the
-	// same property, partitioning.cc:412-423).
+	// same property, partitioning.cc:389-402).
 	for (uint32_t p = 0; p < n_parts; ++p)
 		for (uint32_t i = part_offsets[p]; i < part_offsets[p + 1]; ++i)
 			for (int j = 0; j < 6; ++j)
@@ -627,7 +630,7 @@ int live_commit_locked(tm_fib * fib, live_plan * pl) {
 	if (n) {
 		TM_HIP(hipMemcpyAsync(fib->d_recs, pl->recs.data(), sizeof(ins_rec) * n, hipMemcpyHostToDevice, s));
 		hipLaunchKernelGGL(k_insert_lanes, dim3((n + 255) / 256), dim3(256), 0, s, (const ins_rec *)fib->d_recs, n,
-						   fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid, fib->slot_epoch, pl->epoch);
+						   fib->n_blocks, fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid, fib->slot_epoch, pl->epoch);
 		TM_HIP(hipGetLastError());
 	}
 	TM_HIP(hipMemcpyAsync(fib->part_info, pl->part_info.data(), pl->part_info.size() * 4, hipMemcpyHostToDevice, s));
@@ -881,7 +884,7 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 		int rc = put(d_recs, recs);
 		if (rc == TM_OK) {
 			hipLaunchKernelGGL(k_insert_lanes, dim3((uint32_t)((recs.size() + 255) / 256)), dim3(256), 0, 0, d_recs,
-							   (uint32_t)recs.size(), fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid,
+							   (uint32_t)recs.size(), fib->n_blocks, fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid,
 							   fib->slot_epoch, e);
 			hipError_t e = hipGetLastError();
 			if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -964,6 +967,50 @@ int read_array(std::FILE * f, std::vector<T> & v, uint64_t want, const char * wh
 	return TM_OK;
 }
 
+// Consistency of a loaded image before it replaces the FIB: every index a
+// kernel or a later insert follows must stay inside the arrays it indexes.
+int check_image(uint64_t nb, uint64_t P, uint64_t K, const std::vector<uint32_t> & info,
+				const std::vector<uint64_t> & skeys, const std::vector<uint32_t> & ovf,
+				const std::vector<uint32_t> & dict, const std::vector<uint32_t> & pinfo,
+				const std::vector<uint32_t> & psize, const std::vector<uint32_t> & pcap, const std::vector<int> & order) {
+	std::vector<char> seen(192, 0);
+	for (int b : order) {
+		if (b < 0 || b >= 192 || seen[b]) return set_error(TM_ERR_INVALID, "image: bit order is not a permutation of 0..191");
+		seen[b] = 1;
+	}
+	for (uint64_t i = 1; i < dict.size(); ++i)
+		if (dict[i - 1] >= dict[i]) return set_error(TM_ERR_INVALID, "image: key dictionary is not strictly increasing");
+	for (uint64_t p = 0; p < P; ++p) {
+		const uint64_t first = pinfo[p * 8 + 6], used = pinfo[p * 8 + 7];
+		if (used > pcap[p] || first + pcap[p] > nb || psize[p] > used * 64)
+			return set_error(TM_ERR_INVALID, "image: partition %llu's blocks lie outside the table",
+							 (unsigned long long)p);
+	}
+	for (uint64_t b = 0; b < nb; ++b) {
+		const uint32_t valid = info[b * 8 + 6], part = info[b * 8 + 7];
+		if (valid > 64 || (valid && part >= P))
+			return set_error(TM_ERR_INVALID, "image: block %llu has %u lanes or partition %u", (unsigned long long)b,
+							 valid, part);
+		for (uint32_t l = 0; l < valid; ++l) {
+			const uint64_t kv = skeys[b * 64 + l];
+			if ((kv & 0xFFFF) == KEY_NONE) {
+				const uint64_t at = (kv >> 16) & 0xFFFFFFFFull;
+				if (at >= ovf.size() || at + 1 + ovf[at] > ovf.size())
+					return set_error(TM_ERR_INVALID, "image: block %llu lane %u: overflow list out of range",
+									 (unsigned long long)b, l);
+				for (uint64_t j = 0; j < ovf[at]; ++j)
+					if (ovf[at + 1 + j] >= K) return set_error(TM_ERR_INVALID, "image: key id out of range");
+			} else {
+				for (int j = 0; j < 4; ++j) {
+					const uint32_t id = (uint32_t)(kv >> (16 * j)) & 0xFFFF;
+					if (id != KEY_NONE && id >= K) return set_error(TM_ERR_INVALID, "image: key id out of range");
+				}
+			}
+		}
+	}
+	return TM_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -1016,7 +1063,9 @@ int tm_fib_save(tm_fib * fib, const char * path) {
 
 int tm_fib_load(tm_fib * fib, const char * path) {
 	if (!fib || !path) return set_error(TM_ERR_INVALID, "NULL argument");
+	std::lock_guard<std::mutex> lk(fib->host_mu);
 	if (fib->in_flight.load() > 0) return set_error(TM_ERR_BUSY, "a ring on this FIB has a submission in flight");
+	if (fib->prep_epoch != fib->epoch) return set_error(TM_ERR_BUSY, "a prepared live insert is not committed yet");
 	TM_CHECK(require_device());
 	std::FILE * f = std::fopen(path, "rb");
 	if (!f) return set_error(TM_ERR_INVALID, "cannot open %s", path);
@@ -1056,8 +1105,8 @@ int tm_fib_load(tm_fib * fib, const char * path) {
 	TM_CHECK(read_array(f, psize, P, "partition sizes"));
 	TM_CHECK(read_array(f, pcap, P, "partition capacities"));
 	TM_CHECK(read_array(f, order, 192, "bit order"));
+	TM_CHECK(check_image(nb, P, h.n_keys, info, skeys, ovf, dict, pinfo, psize, pcap, order));
 
-	std::lock_guard<std::mutex> lk(fib->host_mu);
 	fib->release();
 	uint64_t bytes = 0;
 	TM_CHECK(upload(&fib->blk_words, words, bytes));

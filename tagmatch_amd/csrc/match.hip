@@ -21,7 +21,7 @@
 //        (staged in LDS), or appended as (query, filter) pairs;
 //   k_bm_count / scan / k_bm_write   (KEYS_UNIQUE)
 //        bitmap -> sorted distinct keys (reference tagmatch_query.hh:99-119
-//        sort + unique, and back_end.cc:329-343 key expansion);
+//        sort + unique, and back_end.cc:288-301 key expansion);
 //   k_hits_count / scan / k_hits_write  (HITS, KEYS_ALL)
 //        group hits per query and expand them to ids or keys.
 #include "cost_model.h"
@@ -195,7 +195,7 @@ __device__ inline uint32_t key_count(uint64_t kv, const uint32_t * __restrict__ 
 // Because a group's results never leave its workgroup until the end, the
 // bitmaps are written out once with plain stores: no global atomics on
 // results.  The reference instead merges partial results on the host under
-// a per-query mutex (back_end.cc:329-343, tagmatch_query.hh:80-93).
+// a per-query mutex (back_end.cc:288-301, tagmatch_query.hh:80-93).
 
 struct mq_args {
 	dev_fib_view f;
@@ -1719,7 +1719,7 @@ void plain_segments(const tm_ring * r, punit & u) {
 int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit> & units);
 
 // The plain step: one launch per (partition, batch) unit, as the reference
-// launches one kernel per (partition, batch) (back_end.cc:284-327).  Each
+// launches one kernel per (partition, batch) (back_end.cc:243-286).  Each
 // launch runs the persistent kernel's inner loop over the unit only: the
 // batch's queries (through the lookup list) against the partition's blocks,
 // cut into block segments so that the launch spreads over the GPU; key

@@ -1,12 +1,12 @@
 // GPU balanced partitioning of a filter table.
 //
 // Restates the reference's recursive balanced partitioning
-// (reference partitioning.cc:251-432) level-synchronously: every pending
+// (reference partitioning.cc:230-411) level-synchronously: every pending
 // candidate ("segment") of the current level is processed at once.
 //
 //   * A BALANCE segment (size > maxp) is split on the bit whose frequency is
 //     nearest half its size, searching from its first unused pivot bit
-//     (balancing_pivot, partitioning.cc:283-299).  The 0-side keeps the
+//     (balancing_pivot, partitioning.cc:262-278).  The 0-side keeps the
 //     segment's pivot mask, the 1-side adds the pivot (:307-312).  A child
 //     larger than maxp stays BALANCE; a smaller child is finished if its
 //     pivot mask is non-empty, else it goes to NONZERO (:314-357).
@@ -195,7 +195,7 @@ inline bool used_bit(const seg_t & s, int b) { return (s.used[b >> 5] >> (b & 31
 
 uint32_t dist(uint32_t a, uint32_t b) { return a > b ? a - b : b - a; }
 
-// balancing_pivot (reference partitioning.cc:283-299)
+// balancing_pivot (reference partitioning.cc:262-278)
 int balancing_pivot(const seg_t & s, const uint32_t * f) {
 	const uint32_t size = s.end - s.begin, half = size / 2;
 	int pivot = 0;

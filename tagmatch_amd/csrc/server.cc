@@ -5,7 +5,7 @@
 // The reference's serving path is the front end's per-partition queues
 // flushed when full or when the oldest query passes the latency limit
 // (front_end.cc:249-267, :350-376, :563-598), feeding two double-buffered
-// streams per GPU (back_end.cc:172-343).  Here a request is a batch of
+// streams per GPU (back_end.cc:131-311).  Here a request is a batch of
 // queries; an engine thread admits as many waiting requests as fit one free
 // ring (slot) and the budget, submits them as one device step, and returns
 // each request's own results when the step completes.  Steps of different
@@ -256,9 +256,10 @@ void engine(tm_server * s) {
 		}
 		// 2. admission (and insert barriers), under the lock
 		std::unique_lock<std::mutex> lk(s->mu);
-		const bool hold = holding(s, clock_t_::now());
 		for (auto & sl : s->slots) {
-			if (hold || sl.busy || s->waiting.empty()) continue;
+			// the full-or-latency rule holds for every slot: what a slot
+			// left behind waits for its own flush
+			if (sl.busy || s->waiting.empty() || holding(s, clock_t_::now())) continue;
 			if (s->waiting.front()->is_insert) break;
 			uint32_t nq = 0;
 			uint64_t bytes = 0;
@@ -400,7 +401,7 @@ void engine(tm_server * s) {
 				}
 				lk.lock();
 				if (!ready && s->waiting.empty()) s->cv_engine.wait_for(lk, std::chrono::microseconds(20));
-			} else if (hold) {
+			} else if (holding(s, clock_t_::now())) {
 				// a latency flush is pending: sleep until its deadline or an arrival
 				// (requests are queued under the lock, so none is missed)
 				const auto due = s->waiting.front()->t_submit + std::chrono::microseconds(s->cfg.flush_us);

@@ -234,9 +234,39 @@ def test_image_errors(gpu, tmp_path):
         with pytest.raises(tm.TagMatchError) as e:
             tm.Fib.from_image(str(tmp_path / name))
         assert e.value.code == tm.TM_ERR_INVALID and what in str(e.value), name
+    # images whose sizes are right but whose indexes point outside the
+    # table: refused before anything is uploaded
+    arrays = {}
+    pos = 88                                     # header: magic, version, 9 sizes
+    for name, item in (("words", 4), ("info", 4), ("slots", 8), ("fid", 4), ("rich", 4), ("rich_bm", 4),
+                       ("ovf", 4), ("dict", 4), ("pinfo", 4), ("pstat", 4), ("psize", 4), ("pcap", 4),
+                       ("order", 4)):
+        n = int.from_bytes(data[pos:pos + 8], "little")
+        arrays[name] = (pos + 8, n)
+        pos += 8 + n * item
+    assert pos == len(data)
+    n_blocks = arrays["info"][1] // 8
+
+    def corrupt(name, index, value, item=4):
+        b = bytearray(data)
+        at = arrays[name][0] + index * item
+        b[at:at + item] = int(value).to_bytes(item, "little")
+        return bytes(b)
+
+    order1 = int.from_bytes(data[arrays["order"][0] + 4:arrays["order"][0] + 8], "little")
+    cases = {"order.fib": (corrupt("order", 0, order1), "bit order"),
+             "part.fib": (corrupt("pinfo", 6, n_blocks + 5), "outside the table"),
+             "slot.fib": (corrupt("slots", 0, 0xFFFFFFFFFFFFFFFE, 8), "key id out of range"),
+             "ovf.fib": (corrupt("slots", 0, 0x0000FFFFFFFFFFFF, 8), "overflow list out of range")}
+    for name, (blob, what) in cases.items():
+        (tmp_path / name).write_bytes(blob)
+        with pytest.raises(tm.TagMatchError) as e:
+            tm.Fib.from_image(str(tmp_path / name))
+        assert e.value.code == tm.TM_ERR_INVALID and what in str(e.value), name
     # a failed load leaves the FIB as it was
-    with pytest.raises(tm.TagMatchError):
-        fib.load(str(tmp_path / "cut.fib"))
+    for name in ("cut.fib", "part.fib"):
+        with pytest.raises(tm.TagMatchError):
+            fib.load(str(tmp_path / name))
     check_all(fib, cw, ck, cv, q)
     empty = tm.Fib.empty()
     with pytest.raises(tm.TagMatchError) as e:

@@ -133,6 +133,23 @@ def test_latency_flush(table):
         assert np.array_equal(off, eo[i * 64:(i + 1) * 64 + 1] - a) and np.array_equal(vals, ev[a:b])
     srv.close()
 
+    # 1.5 x flush_queries over two slots: the first slot takes a full step,
+    # what it leaves behind waits for the latency limit in the second
+    srv = tm.Server(fib, n_slots=2, slot_queries=640, flush_queries=640, flush_us=200_000)
+    ids = [srv.submit(q[i * 64:(i + 1) * 64]) for i in range(15)]
+    eo15, ev15, _ = tm.match(fib, q[:960])
+    for i, rid in enumerate(ids):
+        assert srv.wait(rid, 60000.0)
+        info, off, vals = srv.result(rid)
+        a, b = eo15[i * 64], eo15[(i + 1) * 64]
+        assert np.array_equal(off, eo15[i * 64:(i + 1) * 64 + 1] - a) and np.array_equal(vals, ev15[a:b])
+        if i >= 10:
+            assert info.step_queries == 320 and info.queue_ms >= 180.0, (i, info.step_queries, info.queue_ms)
+    st = srv.stats()
+    log(f"1.5 x flush_queries: {st['n_batches']} steps")
+    assert st["n_batches"] == 2
+    srv.close()
+
     # below the threshold: the step starts at the latency limit
     srv = tm.Server(fib, n_slots=2, slot_queries=4096, flush_queries=4096, flush_us=50_000)
     t = time.time()
