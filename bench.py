@@ -5,18 +5,29 @@ Workload (the metric's configuration): 10M generated 192-bit filters (1-10
 Zipf tags x 7 bits, 1-4 keys each, identical filters merged), partitioned on
 the GPU into partitions of at most 256k filters; 1M generated queries (10-30
 Zipf tags, 30% supersets of a filter), matched in batches of 256 queries per
-partition.  A step is one pass of the whole engine over the 1M queries
-already resident in HBM: on-device partition lookup, batched subset match,
-key expansion and per-query dedup into sorted key lists.
+partition.  A step is one pass of the whole engine over the 1M queries,
+which are resident in HBM when timing starts: on-device partition lookup,
+batched subset match, key expansion and per-query dedup, and the delivery of
+every query's sorted unique key set to host memory (as a key bitmap row over
+the FIB's sorted key dictionary, 256 bytes per query at 2048 keys).
 
 Beside the headline step the run measures, on the same resident table:
+  * the device-resident step with sorted key lists left in HBM, the same
+    lists copied to host memory, and the rows expanded into lists on the host;
+  * parity outside the timed region: the delivered rows against the key-list
+    step's rows and the rows of a step that enumerates every match, and a
+    sample of queries against the oracle;
   * the plain single-launch form (one launch per (partition, 256-query
     batch), the reference's dispatch shape) on a prefix of the queries;
   * the learned cost model: trained on the plain form's own unit logs,
     held-out MAPE, and the persistent step priced by it;
+  * uneven batching: Zipf(1.2) partitions of 1k-1M filters, batches of
+    1-4096 queries, persistent launch vs the plain form;
   * the serving loop: Poisson arrivals of 64-query requests at 30/70/95% of
-    its measured peak, with 10k-filter inserts every 100 ms; p50/p99/p999;
-  * the CPU baseline: the oracle's CPU schedule on a bounded sample.
+    its sustained peak, with 10k-filter inserts every 100 ms; p50/p99/p999;
+  * the CPU baselines: the oracle's restatement of the reference's CPU
+    schedule and brute force, and the reference's Patricia-trie matcher, on
+    bounded samples.
 
 Prints one JSON line (rank 0).  --gpus above 1 is refused: this engine runs
 on one GPU.
@@ -38,6 +49,9 @@ METRIC = "queries/s and matches/s at 10M 192-bit filters; p99 latency; % HBM roo
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # int32 lane-ops/s, non-packed
 OPS_PER_PAIR = 7                            # 6 and-not/or + 1 compare per pair-test
+PCIE_PEAK_GBS = 63.0                        # PCIe 5.0 x16, one direction
+# (queries per work item, waves per workgroup, blocks per segment; 0 = automatic)
+AUTOTUNE_CANDIDATES = [(64, 8, 0), (64, 8, 2048), (64, 8, 1024), (64, 4, 0), (128, 8, 0), (128, 8, 2048)]
 
 
 def log(*a):
@@ -73,6 +87,10 @@ def parse():
     ap.add_argument("--serve-insert", type=int, default=10000, help="filters per insert (0: none)")
     ap.add_argument("--serve-insert-ms", type=float, default=100.0, help="interval between inserts")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0: skip)")
+    ap.add_argument("--parity-queries", type=int, default=1000,
+                    help="queries checked against the oracle after the timed steps (0: skip)")
+    ap.add_argument("--host-key-steps", type=int, default=2,
+                    help="steps timed with full sorted key lists copied to host memory (0: skip)")
     ap.add_argument("--seed", type=lambda x: int(x, 0), default=0x7461676D61746302)
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -109,33 +127,90 @@ def _sized(fn, q, budget_s, n=16):
     return n2, time.time() - t, r
 
 
-def cpu_baseline(cw, cko, ck, off_parts, masks, perm_words, q, budget_s):
+def cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"hardware_concurrency": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline(cw, cko, ck, perm, part_off, masks, q, budget_s):
     """CPU baselines on bounded samples of the same queries, on the host's
-    cores: the reference's compact Patricia-trie matcher with match_unique
-    results (keys collected, sorted, deduplicated), built from the
-    reference's sources into oracle/_ref, when present; and the oracle's
-    schedule (partition lookup + per-partition brute force, hit counts only,
-    restating the reference's CPU back end)."""
+    cores, all with the reference's match_unique output (each query's keys
+    collected, sorted and deduplicated):
+      * primary: the reference's CPU schedule, restated by the oracle --
+        partition lookup (front_end.cc:465-477) and per-partition brute force
+        (back_end.cc:490-516), all threads;
+      * the brute force of test_matcher.cc:74-81 over the whole table, all
+        threads and one thread;
+      * the reference's compact Patricia-trie matcher built from its sources
+        into oracle/_ref, when present (kind "reference")."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     threads = min(16, os.cpu_count() or 1)
-    n2, dt, hits = _sized(lambda x: orc.schedule_count(perm_words, off_parts, masks, x, threads), q, budget_s / 2)
-    port = {"value": n2 / dt, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"first {n2} of the {len(q)} queries against all {cw.shape[0]} filters "
-                      f"(oracle lookup + per-partition brute force, hit counts only, {threads} threads, {dt:.1f}s)",
-            "hits_per_query": hits / n2}
-    if not orc.ref_available("libref_patricia.so"):
-        return port
-    t = time.time()
-    trie = orc.RefPatricia(cw, cko, ck)
-    tb = time.time() - t
-    n3, dt3, (cnt, tot) = _sized(lambda x: trie.count(x, threads), q, budget_s / 2)
-    trie.close()
-    return {"value": n3 / dt3, "unit": "queries/s", "cores": threads, "kind": "reference",
-            "sample": f"first {n3} of the {len(q)} queries against all {cw.shape[0]} filters (reference compact "
-                      f"Patricia trie, built in {tb:.1f}s; each query's keys collected, sorted and deduplicated; "
-                      f"{threads} threads, {dt3:.1f}s)",
-            "keys_per_query": tot / n3, "port": port}
+    cnt = (cko[1:] - cko[:-1]).astype(np.int64)[perm]
+    ko = np.zeros(len(perm) + 1, np.uint64)
+    np.cumsum(cnt, out=ko[1:])
+    idx = np.repeat(cko[:-1][perm].astype(np.int64) - ko[:-1].astype(np.int64), cnt) + np.arange(int(ko[-1]))
+    pw, pk = cw[perm], ck[idx]
+    del idx
+    share = budget_s / 4
+    n1, dt1, (o1, _) = _sized(lambda x: orc.schedule_match(pw, ko, pk, part_off, masks, x, mode=0, threads=threads),
+                              q, share)
+    n2, dt2, (o2, _) = _sized(lambda x: orc.match(cw, cko, ck, x, mode=0, threads=threads), q, share)
+    n3, dt3, _ = _sized(lambda x: orc.match(cw, cko, ck, x, mode=0, threads=1), q, share / 2, n=2)
+    out = {"value": n1 / dt1, "unit": "queries/s", "cores": threads, "kind": "port",
+           "sample": f"first {n1} of the {len(q)} queries against all {cw.shape[0]} filters: oracle restatement of "
+                     f"the reference's CPU schedule (partition lookup + per-partition brute force), keys sorted and "
+                     f"deduplicated per query, {threads} threads, {dt1:.1f}s",
+           "keys_per_query": float(o1[-1]) / n1, "compiler": "gcc " + orc.compiler(), **cpu_info(),
+           "bruteforce": {"value": n2 / dt2, "unit": "queries/s", "cores": threads, "queries": n2, "seconds": dt2},
+           "bruteforce_single_thread": {"value": n3 / dt3, "unit": "queries/s", "cores": 1, "queries": n3,
+                                        "seconds": dt3}}
+    if orc.ref_available("libref_patricia.so"):
+        t = time.time()
+        trie = orc.RefPatricia(cw, cko, ck)
+        tb = time.time() - t
+        n4, dt4, (c4, tot4) = _sized(lambda x: trie.count(x, threads), q, share)
+        trie.close()
+        out["reference_patricia"] = {
+            "value": n4 / dt4, "unit": "queries/s", "cores": threads, "kind": "reference",
+            "sample": f"first {n4} queries; the reference's compact Patricia trie built from its sources "
+                      f"(oracle/_ref), built in {tb:.1f}s, keys collected, sorted and deduplicated, {dt4:.1f}s"}
+    log(f"cpu baseline: {out}")
+    return out
+
+
+def host_key_lists(ring, nq, steps):
+    """KEYS_UNIQUE steps whose sorted key lists are copied to pinned host
+    memory (tm_read_results) before the step counts as done."""
+    if steps <= 0:
+        return None
+    ring.submit_resident(nq)
+    st = ring.wait()
+    n = int(st.n_results)
+    buf = tm.PinnedRows(n)
+    off = np.zeros(nq + 1, np.uint64)
+    lib = tm.load()
+    tt = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        ring.submit_resident(nq)
+        ring.wait()
+        tm.check(lib.tm_read_results(ring.h, tm._ptr(off), tm.C.c_void_p(buf.ptr), n), "tm_read_results")
+        tt.append(time.perf_counter() - t)
+    buf.close()
+    ms = 1e3 * float(np.mean(tt))
+    res = {"ms_per_step": ms, "queries_per_s": nq / (ms / 1e3), "keys": n, "bytes": 4 * n + 8 * (nq + 1),
+           "gb_per_s": (4 * n + 8 * (nq + 1)) / (ms / 1e3) / 1e9,
+           "note": "KEYS_UNIQUE step, then offsets and every key copied to pinned host memory"}
+    log(f"full key lists to the host: {res}")
+    return res
 
 
 def profile_figures():
@@ -324,7 +399,7 @@ def serving_leg(fib, args, n_filters):
 
     # Sustained peak: the highest rate of a ladder (fractions of the burst
     # peak) that open-loop arrivals sustain: achieved >= 98% of offered and
-    # a p99 within 10x that of the lowest rung.  The burst drain forms full
+    # a p99 within 2x that of the lowest rung.  The burst drain forms full
     # steps that Poisson arrivals do not, so it overstates this rate.
     ins_first = n_filters
     ladder = []
@@ -334,7 +409,7 @@ def serving_leg(fib, args, n_filters):
         lat, _, ach, _ = poisson(frac * burst_peak, 0.6, False)
         p99 = float(np.percentile(lat, 99))
         base_p99 = base_p99 or p99
-        ok = ach >= 0.98 * frac * burst_peak and p99 <= 10 * base_p99
+        ok = ach >= 0.98 * frac * burst_peak and p99 <= 2 * base_p99
         ladder.append({"offered_qps": frac * burst_peak, "achieved_qps": ach, "p99_ms": p99, "sustained": bool(ok)})
         if not ok:
             break
@@ -356,7 +431,7 @@ def serving_leg(fib, args, n_filters):
     st = srv.stats()
     srv.close()
     return {"peak_qps": peak, "burst_peak_qps": burst_peak, "peak_ladder": ladder,
-            "peak_note": "peak = highest Poisson rate sustained (achieved >= 98% of offered, p99 within 10x of "
+            "peak_note": "peak = highest Poisson rate sustained (achieved >= 98% of offered, p99 within 2x of "
                          "the lowest rung); burst_peak = a drained backlog of requests",
             "request_queries": req, "slots": args.serve_slots,
             "slot_queries": args.serve_slot_queries,
@@ -379,6 +454,52 @@ def copy_calibration(mib):
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
                       "launches": 3}), flush=True)
+
+
+def lists_equal_sample(off, vals, idx, eo, ev):
+    """Whether the lists of queries idx (in a whole-run off/vals) equal the
+    oracle's lists eo/ev for those queries."""
+    cnt = (off[idx + 1] - off[idx]).astype(np.int64)
+    ecnt = np.diff(eo).astype(np.int64)
+    if not np.array_equal(cnt, ecnt):
+        return False
+    pos = np.repeat(off[idx].astype(np.int64) - eo[:-1].astype(np.int64), ecnt) + np.arange(int(eo[-1]))
+    return bool(np.array_equal(vals[pos], ev))
+
+
+def delivered_steps(fib, q, cfg_kw, steps, warmup, kw):
+    """The headline step: KEY_BITMAP rings whose submissions end with every
+    query's key bitmap row copied into pinned host memory (tm_ring_set_host_rows).
+    Two rings alternate so that one ring's copy overlaps the other's match.
+    Returns (ms per step, match kernel ms per step, last ring's rows, rings)."""
+    nq = q.shape[0]
+    rings, bufs = [], []
+    for _ in range(2):
+        r = tm.Ring(fib, nq, tm.config(mode=tm.MODE_KEY_BITMAP, **cfg_kw))
+        b = tm.PinnedRows(nq * kw)
+        r.set_host_rows(b)
+        r.submit(q)
+        r.wait()
+        rings.append(r)
+        bufs.append(b)
+
+    def run(k):
+        match_ms = []
+        rings[0].submit_resident(nq)
+        for i in range(k):
+            if i + 1 < k:
+                rings[(i + 1) % 2].submit_resident(nq)
+            st = rings[i % 2].wait()
+            match_ms.append(st.match_ms)
+        return match_ms, bufs[(k - 1) % 2]
+
+    run(max(warmup, 1))
+    tm.check(tm.load().tm_dev_synchronize(), "sync")
+    t0 = time.perf_counter()
+    match_ms, last = run(steps)
+    tm.check(tm.load().tm_dev_synchronize(), "sync")
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, match_ms, last, rings, bufs
 
 
 def main():
@@ -407,10 +528,10 @@ def main():
     log(f"first step (with H2D and capacity growth): {time.time() - t:.2f}s, overflow={st.overflow}")
     tuned = None
     if args.autotune and sched == tm.SCHED_PERSISTENT and not args.tile:
-        # the autotuner (tm_ring_autotune) picks the tile size and waves per
-        # workgroup on this workload, before the timed steps
+        # the autotuner (tm_ring_autotune) picks the tile size, waves per
+        # workgroup and segment size on this workload, before the timed steps
         cands = [dict(batch_size=b, waves_per_group=w, chunk_blocks=c, use_graph=int(args.graph))
-                 for b, w in ((64, 4), (64, 8), (128, 4), (128, 8)) for c in (0, 4096)]
+                 for b, w, c in AUTOTUNE_CANDIDATES]
         best, times = ring.autotune(nq, cands, reps=1)
         cfg = ring.cfg
         kw.update(batch_size=cands[best]["batch_size"], waves_per_group=cands[best]["waves_per_group"],
@@ -421,28 +542,37 @@ def main():
         ring.submit_resident(nq)
         ring.wait()
 
+    # ---- device-resident step: sorted key lists left in HBM
     counts = ring.lookup_counts()[:len(part_off) - 1]   # the FIB's spare last partition aside
     tm.check(tm.load().tm_dev_synchronize(), "sync")
-    step_ms, match_ms, stats = [], [], []
+    dev_ms, stats = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
         ring.submit_resident(nq)
         st = ring.wait()
-        step_ms.append((time.perf_counter() - ts) * 1e3)
-        match_ms.append(st.match_ms)
+        dev_ms.append((time.perf_counter() - ts) * 1e3)
         stats.append(st.as_dict())
-        log(f"step {i}: {step_ms[-1]:.1f} ms (match kernel {st.match_ms:.1f} ms, device {st.device_ms:.1f} ms, "
-            f"{st.n_items} items, {st.n_hits} hits)")
     tm.check(tm.load().tm_dev_synchronize(), "sync")
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    qps = nq / (ms_per_step / 1e3)
+    device_step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     st = stats[-1]
+    log(f"device-resident KEYS_UNIQUE step: {device_step_ms:.2f} ms (match kernel {st['match_ms']:.2f} ms, "
+        f"{st['n_results']} keys)")
 
-    # The same step with every (query, filter) match enumerated: the default
-    # step stops testing a query once its key set holds all keys of the
-    # table, so it does not count every match.
+    # ---- headline: every query's key set delivered to host memory
+    kwords = (int(fib.stats()["n_distinct_keys"]) + 31) // 32
+    ms_per_step, hl_match_ms, host_rows, hrings, hbufs = delivered_steps(fib, q, kw, args.steps, args.warmup, kwords)
+    qps = nq / (ms_per_step / 1e3)
+    kernel_ms = float(np.mean(hl_match_ms))
+    log(f"host-delivered step: {ms_per_step:.2f} ms -> {qps / 1e6:.1f}M queries/s (match kernel {kernel_ms:.2f} ms)")
+
+    # ---- parity outside the timed region: the delivered rows equal the
+    # KEYS_UNIQUE step's rows and the rows of the step that enumerates every
+    # match; a sample of queries against the oracle
+    parity = {}
+    rows_ku = ring.key_bitmaps()
+    delivered = host_rows.rows(nq, kwords)
+    parity["rows_equal_keys_unique_step"] = bool(np.array_equal(delivered, rows_ku))
     ring.configure(tm.config(mode=tm.MODE_KEYS_UNIQUE, schedule=sched, full_exit=2, **kw))
     ring.submit_resident(nq)
     ring.wait()
@@ -450,32 +580,102 @@ def main():
     ring.submit_resident(nq)
     st_all = ring.wait()
     all_ms = (time.perf_counter() - ts) * 1e3
+    parity["rows_equal_no_exit_step"] = bool(np.array_equal(ring.key_bitmaps(), delivered))
     n_hits = st_all.n_hits
     no_exit = {"ms_per_step": all_ms, "queries_per_s": nq / (all_ms / 1e3), "matches_per_s": n_hits / (all_ms / 1e3),
                "match_ms": st_all.match_ms, "hits_enumerated": st_all.n_hits, "default_hits_enumerated": st["n_hits"]}
     log(f"without the full-key-set exit: {all_ms:.1f} ms per step, {n_hits} matches")
     ring.configure(cfg)
+    ring.submit_resident(nq)
+    ring.wait()
+    dictionary = fib.dictionary()
+    if args.parity_queries > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        idx = np.unique(np.linspace(0, nq - 1, min(args.parity_queries, nq)).astype(np.int64))
+        t = time.time()
+        eo, ev = orc.match(cw, cko, ck, q[idx], mode=0)
+        so, sv = tm.expand_key_bitmaps(delivered[idx], dictionary)
+        parity["oracle_sample_queries"] = int(len(idx))
+        parity["oracle_equal_delivered"] = bool(np.array_equal(so, eo) and np.array_equal(sv, ev))
+        off_ku, vals_ku = ring.results()
+        parity["oracle_equal_keys_unique"] = lists_equal_sample(off_ku, vals_ku, idx, eo, ev)
+        del off_ku, vals_ku
+        log(f"parity: {parity} (oracle {time.time() - t:.1f}s)")
+    parity["ok"] = all(v for k, v in parity.items() if isinstance(v, bool))
+    if not parity["ok"]:
+        log("PARITY FAILURE: the delivered results differ from the reference check")
 
-    # algorithmic work of the (partition, batch) units of this workload
+    # ---- the same results as full sorted key lists in host memory, and the
+    # rows expanded into key lists on the host
+    hk = host_key_lists(ring, nq, args.host_key_steps)
+    t = time.time()
+    exp_off = np.zeros(nq + 1, np.uint64)
+    lib = tm.load()
+    tm.check(lib.tm_expand_key_bitmaps(tm._ptr(delivered), nq, kwords, tm._ptr(dictionary), dictionary.size,
+                                       tm._ptr(exp_off), None, 0, 0), "expand")
+    exp_vals = np.empty(int(exp_off[-1]), np.uint32)
+    exp_vals[::1024] = 0       # touch the pages before timing
+    threads = os.cpu_count() or 1
+    tt = []
+    for _ in range(2):
+        t = time.perf_counter()
+        tm.check(lib.tm_expand_key_bitmaps(tm._ptr(delivered), nq, kwords, tm._ptr(dictionary), dictionary.size,
+                                           tm._ptr(exp_off), tm._ptr(exp_vals), exp_vals.size, threads), "expand")
+        tt.append(time.perf_counter() - t)
+    host_expand = {"queries_per_s": nq / min(tt), "seconds": min(tt), "threads": threads,
+                   "keys": int(exp_off[-1]), "keys_per_s": int(exp_off[-1]) / min(tt)}
+    log(f"host expansion of the rows: {host_expand}")
+    del exp_vals
+    for r in hrings:
+        r.close()
+    for b in hbufs:
+        b.close()
+
+    # ---- roofline of the dominant kernel (k_match_q) and of the step
     sizes = np.diff(part_off).astype(np.float64)
     batches = np.ceil(counts / args.batch)
-    bytes_alg = float(np.sum(batches * 24.0 * sizes) + 24.0 * counts.sum())
+    bytes_bruteforce = float(np.sum(batches * 24.0 * sizes) + 24.0 * counts.sum())
     pair_tests = float(np.sum(sizes * counts))
-    kernel_s = (np.mean(match_ms) / 1e3) if match_ms[-1] > 0 else ms_per_step / 1e3
-    hbm_gbs = bytes_alg / kernel_s / 1e9
-    valu_tops = OPS_PER_PAIR * pair_tests / kernel_s / 1e12
+    fs = fib.stats()
+    n_blocks = int(fs["n_blocks"])
+    # what the algorithm must move per launch: every block of the table once
+    # (6 word planes, block info, key slots), the active queries (words and
+    # bitmap rows in and out) and the partition table
+    alg_bytes = float(n_blocks * (1536 + 32 + 512) + len(part_off) * 64)
     prof = profile_figures()
     traffic = prof.get("bytes_per_launch")
+    kernel_s = kernel_ms / 1e3
+    achieved = (traffic / kernel_s / 1e9) if traffic else alg_bytes / kernel_s / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_match_q", "kernel_ms": kernel_ms,
+            "alg_bytes": alg_bytes, "alg_frac": alg_bytes / kernel_s / 1e9 / HBM_PEAK_GBS,
+            "measured_valu_frac": (prof["valu_insts"] * 64 / kernel_s / 1e12 / VALU_PEAK_TOPS)
+            if prof.get("valu_insts") else None,
+            "bruteforce_bytes": bytes_bruteforce,
+            "note": "achieved = HBM traffic of one launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated against a "
+                    "copy kernel; profiles/match_traffic.json) / the launch's mean duration, measured here with HIP "
+                    "events on the ring's stream over the timed steps.  alg_bytes = what the algorithm must move: "
+                    "every block of the table once (words, info, key slots).  bruteforce_bytes = what the reference's "
+                    "(partition, 256-query batch) units would read; block masks, the rich-filter pass and the "
+                    "full-key-set exit skip most of it exactly"}
+    step_bytes = float(nq * kwords * 4)
+    step_roof = {"bound": "pcie", "achieved": step_bytes / (ms_per_step / 1e3) / 1e9, "peak": PCIE_PEAK_GBS,
+                 "unit": "GB/s", "frac": step_bytes / (ms_per_step / 1e3) / 1e9 / PCIE_PEAK_GBS,
+                 "bytes_per_step": step_bytes,
+                 "note": "the host-delivered step moves every query's key bitmap row (4 x ceil(K/32) bytes) to "
+                         "host memory; PCIe 5.0 x16 is 63 GB/s per direction"}
+    for name, rr in (("roofline", roof), ("step_roofline", step_roof)):
+        assert rr["frac"] <= 1.0, f"{name}: fraction {rr['frac']:.3f} above 1"
     if no_exit["match_ms"] > 0:
         ne_s = no_exit["match_ms"] / 1e3
-        no_exit["roofline_hbm_frac"] = bytes_alg / ne_s / 1e9 / HBM_PEAK_GBS
         no_exit["roofline_valu_frac"] = OPS_PER_PAIR * pair_tests / ne_s / 1e12 / VALU_PEAK_TOPS
         if prof.get("no_full_exit_bytes_per_launch"):
             no_exit["measured_hbm_frac"] = prof["no_full_exit_bytes_per_launch"] / ne_s / 1e9 / HBM_PEAK_GBS
         if prof.get("no_full_exit_valu_insts"):
             no_exit["measured_valu_frac"] = prof["no_full_exit_valu_insts"] * 64 / ne_s / 1e12 / VALU_PEAK_TOPS
 
-    # the plain single-launch form on the same queries (reference dispatch shape)
+    # ---- the plain single-launch form on a prefix of the same queries
     plain = None
     if args.plain_steps > 0 and sched == tm.SCHED_PERSISTENT:
         ring.configure(tm.config(mode=tm.MODE_KEYS_UNIQUE, schedule=tm.SCHED_PLAIN, batch_size=args.batch))
@@ -488,11 +688,11 @@ def main():
             tt.append(time.perf_counter() - ts)
             log(f"plain step {i}: {npq} queries in {tt[-1] * 1e3:.1f} ms, {pst.n_items} launches")
         plain = {"queries_per_s": npq / np.mean(tt), "ms_per_step": 1e3 * np.mean(tt), "launches": pst.n_items,
-                 "queries": npq, "match_ms": pst.match_ms}
+                 "queries": npq, "match_ms": pst.match_ms, "matches": pst.n_hits}
     model = None
     if args.model and sched == tm.SCHED_PERSISTENT:
         ring.configure(cfg)
-        model = cost_model_leg(fib, q, ring, nq, max(1, args.steps // 2), ms_per_step, 4096)
+        model = cost_model_leg(fib, q, ring, nq, max(1, args.steps // 2), device_step_ms, 4096)
     ring.close()
 
     uneven = None
@@ -505,8 +705,7 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        # the same partitions, in partitioned order, for the CPU schedule
-        cpu = cpu_baseline(cw, cko, ck, part_off, masks, cw[perm], q, args.cpu_seconds)
+        cpu = cpu_baseline(cw, cko, ck, perm, part_off, masks, q, args.cpu_seconds)
 
     out = {
         "metric": METRIC,
@@ -520,45 +719,50 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic: seeded generator (Zipf tags, Bloom bits, keys), inputs resident in HBM",
+        "data": "synthetic: seeded generator (Zipf tags, Bloom bits, keys); queries resident in HBM when timing "
+                "starts, results delivered to host memory",
         "config": {
             "workload": f"{args.filters // 1_000_000}M filters (x 192-bit), {nq} queries, partitions <= {args.maxp}, "
-                        f"batches of {args.batch} queries per partition, KEYS_UNIQUE output",
+                        f"batches of {args.batch} queries per partition; a step ends with every query's sorted "
+                        f"unique key set in host memory (key bitmap row over the sorted dictionary)",
             "filters": args.filters, "distinct_filters": int(cw.shape[0]), "queries": nq,
             "partitions": int(len(part_off) - 1), "maxp": args.maxp, "batch": args.batch,
-            "schedule": args.schedule, "graph": bool(args.graph),
+            "schedule": args.schedule, "graph": bool(args.graph), "result_form": "key bitmap rows in host memory",
         },
+        "value_note": "queries/s with every query's match_unique result (sorted unique keys) in host memory as a "
+                      f"{kwords * 4}-byte key bitmap row over the FIB's sorted dictionary; two rings alternate so "
+                      "that one step's copy overlaps the next step's match",
+        "device_step_ms": device_step_ms,
+        "device_step_queries_per_s": nq / (device_step_ms / 1e3),
+        "device_step_note": "KEYS_UNIQUE step whose sorted key lists stay in HBM",
+        "host_key_lists": hk,
+        "host_expand": host_expand,
+        "parity_checked": parity,
         "matches_per_s": no_exit["matches_per_s"],
         "matches_note": "every (query, filter) match enumerated, in a step without the full-key-set exit",
         "keys_per_s": st["n_results"] / (ms_per_step / 1e3),
         "hits_per_query": n_hits / nq,
         "no_full_exit": no_exit,
         "p99_latency_ms": serving["levels"]["70%"]["p99_ms"] if serving else None,
-        "latency_note": "p99 of 64-query requests through the serving loop at 70% of its peak, "
+        "latency_note": "p99 of 64-query requests through the serving loop at 70% of its sustained peak, "
                         "10k-filter inserts every 100 ms",
-        "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)), "max": float(np.max(step_ms))},
+        "step_latency_ms": {"p50": float(np.percentile(dev_ms, 50)), "max": float(np.max(dev_ms))},
         "serving": serving,
         "cost_model": model,
         "uneven_batching": uneven,
         "autotune": tuned,
-        "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_match_q", "kernel_ms": kernel_s * 1e3,
-                     "bytes_per_launch": bytes_alg,
-                     "measured_frac": (traffic / kernel_s / 1e9 / HBM_PEAK_GBS) if traffic else None,
-                     "measured_valu_frac": (prof["valu_insts"] * 64 / kernel_s / 1e12 / VALU_PEAK_TOPS)
-                     if prof.get("valu_insts") else None,
-                     "note": "achieved = algorithmic bytes of the (partition, 256-query batch) units the launch "
-                             "covers, as brute force reads them; block masks, the full-key-set exit and complete "
-                             "filters skip most of that work exactly, so frac above 1 is work avoided. "
-                             "measured_frac = HBM traffic (rocprofv3) / kernel time / peak; measured_valu_frac "
-                             "= SQ_INSTS_VALU x 64 / kernel time / 39.3 T"},
-        "roofline_valu": {"bound": "valu", "achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                          "frac": valu_tops / VALU_PEAK_TOPS, "pair_tests_per_launch": pair_tests,
-                          "note": "algorithmic int ops (7 per pair-test over selected partitions); "
-                                  "block masks let the kernel skip most of them"},
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "roofline_valu": {"bound": "valu", "pair_tests_per_launch": pair_tests,
+                          "bruteforce_ops_per_launch": OPS_PER_PAIR * pair_tests,
+                          "measured_frac": roof["measured_valu_frac"],
+                          "note": "measured_frac = SQ_INSTS_VALU x 64 / kernel time / 39.3 T lane-ops/s"},
         "plain": plain,
         "speedup_vs_plain": (qps / plain["queries_per_s"]) if plain else None,
+        "speedup_vs_plain_same_algorithm": (no_exit["queries_per_s"] / plain["queries_per_s"]) if plain else None,
+        "plain_note": "the plain form is the reference's dispatch shape (one launch per partition and 256-query "
+                      "batch, no rich-filter pass or full-key-set exit) on a prefix of the queries; the same-"
+                      "algorithm ratio compares it with the persistent step without the exit",
         "cpu_baseline": cpu,
         "setup_s": setup_s,
         "step_stats": stats[-1],

@@ -192,6 +192,17 @@ typedef struct tm_ring tm_ring;
                                   reference tagmatch.cc:200-204)             */
 #define TM_MODE_HITS        2  /* per query: ids of matched filters, order
                                   unspecified                                */
+#define TM_MODE_KEY_BITMAP  3  /* per query: a row of ceil(K/32) uint32 words
+                                  over the FIB's sorted key dictionary
+                                  (tm_fib_key_dictionary, K keys): bit i
+                                  (word i/32, bit i%32) is set iff dictionary
+                                  key i is one of the query's keys.  The set
+                                  bits in increasing order are exactly the
+                                  KEYS_UNIQUE list (match_unique, reference
+                                  tagmatch_query.hh:99-119) in a fixed-size
+                                  form; read with tm_read_key_bitmaps or
+                                  delivered by tm_ring_set_host_rows, expanded
+                                  with tm_expand_key_bitmaps                 */
 
 /* Schedules. */
 #define TM_SCHED_PLAIN      0  /* one kernel launch per (partition, batch),
@@ -308,6 +319,41 @@ int tm_read_results(tm_ring * ring, uint64_t * offsets, uint32_t * values, uint6
 /* Device pointers of the results of the finished submission. */
 int tm_ring_results_device(tm_ring * ring, const uint64_t ** dev_offsets,
 						   const uint32_t ** dev_values);
+
+/* Key bitmap rows of the finished submission (KEYS_UNIQUE or KEY_BITMAP
+ * rings): nq rows of *kw words each (kw = ceil(K/32)) into rows, which
+ * holds capacity_words.  TM_ERR_OVERFLOW if nq * kw > capacity_words (*kw
+ * is still set); TM_ERR_STATE for the other modes.  With the reference's
+ * result delivery (back_end.cc:288-301, tagmatch_query.hh:99-119) the keys
+ * reach the host one hit at a time; a row is a fixed 4 * kw bytes. */
+int tm_read_key_bitmaps(tm_ring * ring, uint32_t * rows, uint64_t capacity_words, uint32_t * kw);
+
+/* Host delivery for KEY_BITMAP rings: every later submission ends, on the
+ * ring's stream, with its rows copied into host_rows (nq * kw words), so
+ * that tm_poll / tm_wait report them already in host memory.  host_rows
+ * should be pinned (tm_host_alloc) for the copy to overlap other rings'
+ * work; it must stay valid until tm_wait of each submission returns.  A
+ * submission with nq * kw > capacity_words fails with TM_ERR_OVERFLOW.
+ * host_rows NULL stops the delivery. */
+int tm_ring_set_host_rows(tm_ring * ring, uint32_t * host_rows, uint64_t capacity_words);
+
+/* Pinned (page-locked) host memory for result delivery, and its release. */
+int tm_host_alloc(void ** ptr, uint64_t bytes);
+int tm_host_free(void * ptr);
+
+/* The FIB's sorted distinct keys, the dictionary KEY_BITMAP rows index:
+ * *n = K; keys (capacity entries, may be NULL to read K only). */
+int tm_fib_key_dictionary(const tm_fib * fib, uint32_t * keys, uint64_t capacity, uint64_t * n);
+
+/* Expand nq KEY_BITMAP rows (kw words each) over dict (n_keys sorted keys)
+ * into KEYS_UNIQUE lists on the host: offsets (nq + 1 entries) and values
+ * (capacity entries; NULL to compute offsets only).  Runs on `threads` host
+ * threads (0: one per core).  TM_ERR_OVERFLOW if the keys exceed capacity
+ * (offsets are still written).  Bits at or above n_keys are ignored.  This
+ * is the host half of the reference's add_output + finalize_matching
+ * (tagmatch_query.hh:80-119) for a whole submission at once. */
+int tm_expand_key_bitmaps(const uint32_t * rows, uint64_t nq, uint32_t kw, const uint32_t * dict, uint32_t n_keys,
+						  uint64_t * offsets, uint32_t * values, uint64_t capacity, uint32_t threads);
 
 /* Per-partition query counts of the finished submission (the on-device
  * lookup's result; the reference's per-partition queue fill,

@@ -173,24 +173,38 @@ typedef struct {
 	uint64_t * counts;  /* per query */
 	vec32 out;
 	uint64_t hits;
+	/* schedule form (CPU-B): only the filters of the partitions the lookup
+	 * selects; NULL for brute force over the whole table */
+	const orc_lookup * lk;
+	const uint32_t * part_off;
 } match_job;
 
 /* test_matcher.cc:74-81: every filter, in table order, that is a subset of
  * the query contributes its keys; tagmatch_query.hh:105-109 sorts and
  * removes duplicates for match_unique. */
+static void collect(const match_job * j, const uint32_t * qw, uint64_t f0, uint64_t f1, vec32 * tmp) {
+	for (uint64_t f = f0; f < f1; ++f) {
+		if (!orc_subset(j->words + f * 6, qw)) continue;
+		if (j->mode == 2) {
+			vpush(tmp, (uint32_t)f);
+		} else {
+			for (uint64_t k = j->key_off[f]; k < j->key_off[f + 1]; ++k) vpush(tmp, j->keys[k]);
+		}
+	}
+}
+
 static void * match_worker(void * arg) {
 	match_job * j = (match_job *)arg;
 	vec32 tmp = {0};
+	uint32_t * sel = j->lk ? (uint32_t *)malloc(((uint64_t)orc_lookup_size(j->lk) + 1) * sizeof(uint32_t)) : NULL;
 	for (uint64_t q = j->qb; q < j->qe; ++q) {
 		const uint32_t * qw = j->queries + q * 6;
 		tmp.n = 0;
-		for (uint64_t f = 0; f < j->n; ++f) {
-			if (!orc_subset(j->words + f * 6, qw)) continue;
-			if (j->mode == 2) {
-				vpush(&tmp, (uint32_t)f);
-			} else {
-				for (uint64_t k = j->key_off[f]; k < j->key_off[f + 1]; ++k) vpush(&tmp, j->keys[k]);
-			}
+		if (j->lk) {
+			const uint32_t ns = orc_lookup_query(j->lk, qw, sel);
+			for (uint32_t i = 0; i < ns; ++i) collect(j, qw, j->part_off[sel[i]], j->part_off[sel[i] + 1], &tmp);
+		} else {
+			collect(j, qw, 0, j->n, &tmp);
 		}
 		if (j->mode != 2) qsort(tmp.v, tmp.n, sizeof(uint32_t), cmp_u32);
 		uint64_t m = tmp.n;
@@ -203,6 +217,7 @@ static void * match_worker(void * arg) {
 		for (uint64_t k = 0; k < m; ++k) vpush(&j->out, tmp.v[k]);
 	}
 	free(tmp.v);
+	free(sel);
 	return NULL;
 }
 
@@ -213,9 +228,36 @@ static int clamp_threads(int t, uint64_t work) {
 	return t;
 }
 
+static int run_match(const uint32_t * words, uint64_t n, const uint64_t * key_off, const uint32_t * keys,
+					 const uint32_t * queries, uint64_t nq, int mode, int threads, uint64_t * off, uint32_t ** vals,
+					 const orc_lookup * lk, const uint32_t * part_off);
+
 int orc_match(const uint32_t * words, uint64_t n, const uint64_t * key_off, const uint32_t * keys,
 			  const uint32_t * queries, uint64_t nq, int mode, int threads, uint64_t * off,
 			  uint32_t ** vals) {
+	return run_match(words, n, key_off, keys, queries, nq, mode, threads, off, vals, NULL, NULL);
+}
+
+/* The reference schedule on the CPU with the reference's results: each
+ * query goes to the partitions whose mask it contains (front_end.cc:465-477)
+ * and is brute-forced there (the CPU back end, back_end.cc:490-516); its
+ * keys are sorted and deduplicated as match_unique does
+ * (tagmatch_query.hh:105-109).  words / key_off / keys are in partitioned
+ * order; HITS ids are positions in that order. */
+int orc_schedule_match(const uint32_t * words, const uint64_t * key_off, const uint32_t * keys,
+					   const uint32_t * part_off, const uint32_t * masks, uint32_t np, const uint32_t * queries,
+					   uint64_t nq, int mode, int threads, uint64_t * off, uint32_t ** vals) {
+	orc_lookup * lk = orc_lookup_build(masks, np);
+	int rc = run_match(words, part_off[np], key_off, keys, queries, nq, mode, threads, off, vals, lk, part_off);
+	orc_lookup_free(lk);
+	return rc;
+}
+
+const char * orc_compiler(void) { return __VERSION__; }
+
+static int run_match(const uint32_t * words, uint64_t n, const uint64_t * key_off, const uint32_t * keys,
+					 const uint32_t * queries, uint64_t nq, int mode, int threads, uint64_t * off, uint32_t ** vals,
+					 const orc_lookup * lk, const uint32_t * part_off) {
 	if (mode < 0 || mode > 2) return -1;
 	int T = clamp_threads(threads, nq);
 	match_job * jobs = (match_job *)calloc(T, sizeof(match_job));
@@ -230,6 +272,8 @@ int orc_match(const uint32_t * words, uint64_t n, const uint64_t * key_off, cons
 		if (j->qb > nq) j->qb = nq;
 		j->mode = mode;
 		j->counts = counts;
+		j->lk = lk;
+		j->part_off = part_off;
 		pthread_create(&th[t], NULL, match_worker, j);
 	}
 	for (int t = 0; t < T; ++t) pthread_join(th[t], NULL);
@@ -511,6 +555,8 @@ uint32_t orc_lookup_query(const orc_lookup * lk, const uint32_t * q, uint32_t * 
 	qsort(sel, n, sizeof(uint32_t), cmp_u32);
 	return n;
 }
+
+uint32_t orc_lookup_size(const orc_lookup * lk) { return lk->np; }
 
 void orc_lookup_free(orc_lookup * lk) {
 	if (!lk) return;

@@ -66,6 +66,18 @@ typedef struct orc_lookup orc_lookup;
 orc_lookup * orc_lookup_build(const uint32_t * masks, uint32_t np);
 uint32_t orc_lookup_query(const orc_lookup * lk, const uint32_t * q, uint32_t * sel);
 void orc_lookup_free(orc_lookup * lk);
+uint32_t orc_lookup_size(const orc_lookup * lk);   /* partitions */
+
+/* The reference schedule with key output: lookup (front_end.cc:465-477),
+ * per-partition brute force (back_end.cc:490-516), then match_unique's sort
+ * and dedup (tagmatch_query.hh:105-109) for mode 0; modes as orc_match.
+ * words / key_off / keys in partitioned order (part_off[np] filters). */
+int orc_schedule_match(const uint32_t * words, const uint64_t * key_off, const uint32_t * keys,
+					   const uint32_t * part_off, const uint32_t * masks, uint32_t np, const uint32_t * queries,
+					   uint64_t nq, int mode, int threads, uint64_t * off, uint32_t ** vals);
+
+/* Version of the compiler that built this library. */
+const char * orc_compiler(void);
 
 /* CPU schedule of the reference back end (front_end.cc:465-477 lookup, then
  * back_end.cc:490-516 per-partition brute force); returns (query, filter)

@@ -46,6 +46,9 @@ def load():
         "orc_schedule_count": (C.c_uint64, [P, P, P, C.c_uint32, P, C.c_uint64, C.c_int]),
         "orc_bruteforce_count": (C.c_uint64, [P, C.c_uint64, P, C.c_uint64, C.c_int]),
         "orc_free": (None, [P]),
+        "orc_schedule_match": (C.c_int, [P, P, P, P, P, C.c_uint32, P, C.c_uint64, C.c_int, C.c_int, P,
+                                         C.POINTER(P)]),
+        "orc_compiler": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -113,6 +116,30 @@ def match(words, key_off, keys, queries, mode=0, threads=None):
     v = np.ctypeslib.as_array(C.cast(vals, C.POINTER(C.c_uint32)), shape=(max(n, 1),))[:n].copy()
     load().orc_free(vals)
     return off, v
+
+
+def schedule_match(words, key_off, keys, part_off, masks, queries, mode=0, threads=None):
+    """The reference schedule (lookup + per-partition brute force) with key
+    output; words/key_off/keys in partitioned order."""
+    words = _w(words).reshape(-1, 6)
+    queries = _w(queries).reshape(-1, 6)
+    m = _w(masks).reshape(-1, 6)
+    po = _w(part_off)
+    ko = np.ascontiguousarray(key_off, np.uint64)
+    nq = queries.shape[0]
+    off = np.zeros(nq + 1, np.uint64)
+    vals = C.c_void_p()
+    t = threads or min(16, os.cpu_count() or 1)
+    load().orc_schedule_match(_p(words), _p(ko), _p(_w(keys)), _p(po), _p(m), m.shape[0], _p(queries), nq, mode, t,
+                              _p(off), C.byref(vals))
+    n = int(off[-1])
+    v = np.ctypeslib.as_array(C.cast(vals, C.POINTER(C.c_uint32)), shape=(max(n, 1),))[:n].copy()
+    load().orc_free(vals)
+    return off, v
+
+
+def compiler():
+    return load().orc_compiler().decode()
 
 
 def partition(words, maxp):

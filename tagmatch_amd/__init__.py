@@ -26,6 +26,7 @@ TM_ERR_BUSY = -7
 MODE_KEYS_UNIQUE = 0
 MODE_KEYS_ALL = 1
 MODE_HITS = 2
+MODE_KEY_BITMAP = 3
 SCHED_PLAIN = 0
 SCHED_PERSISTENT = 1
 
@@ -132,6 +133,12 @@ _SIGS = {
     "tm_wait": (C.c_int, [P, C.POINTER(Status)]),
     "tm_read_results": (C.c_int, [P, P, P, C.c_uint64]),
     "tm_ring_results_device": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
+    "tm_read_key_bitmaps": (C.c_int, [P, P, C.c_uint64, U32P]),
+    "tm_ring_set_host_rows": (C.c_int, [P, P, C.c_uint64]),
+    "tm_host_alloc": (C.c_int, [C.POINTER(P), C.c_uint64]),
+    "tm_host_free": (C.c_int, [P]),
+    "tm_fib_key_dictionary": (C.c_int, [P, P, C.c_uint64, U64P]),
+    "tm_expand_key_bitmaps": (C.c_int, [P, C.c_uint64, C.c_uint32, P, C.c_uint32, P, P, C.c_uint64, C.c_uint32]),
     "tm_ring_lookup_counts": (C.c_int, [P, P, C.c_uint32]),
     "tm_match": (C.c_int, [P, P, C.c_uint32, C.POINTER(MatchConfig), P, P, C.c_uint64, C.POINTER(Status)]),
     "tm_mlp_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(P)]),
@@ -401,6 +408,14 @@ class Fib:
         """Replace the table with an image from save() (tm_fib_load)."""
         check(load().tm_fib_load(self.h, os.fsencode(path)), "tm_fib_load")
 
+    def dictionary(self):
+        """The sorted distinct keys that KEY_BITMAP rows index."""
+        n = C.c_uint64()
+        check(load().tm_fib_key_dictionary(self.h, None, 0, C.byref(n)), "tm_fib_key_dictionary")
+        d = np.zeros(n.value, np.uint32)
+        check(load().tm_fib_key_dictionary(self.h, _ptr(d), d.size, C.byref(n)), "tm_fib_key_dictionary")
+        return d
+
     def stats(self):
         s = FibStats()
         check(load().tm_fib_stats_get(self.h, C.byref(s)), "tm_fib_stats_get")
@@ -543,6 +558,25 @@ class Ring:
         check(load().tm_read_results(self.h, _ptr(off), _ptr(vals), vals.size), "tm_read_results")
         return off, vals
 
+    def key_bitmaps(self):
+        """Key bitmap rows of the finished submission: (nq, kw) uint32."""
+        kw = C.c_uint32()
+        rc = load().tm_read_key_bitmaps(self.h, None, 0, C.byref(kw))
+        if rc not in (TM_OK, TM_ERR_OVERFLOW):
+            check(rc, "tm_read_key_bitmaps")
+        rows = np.zeros((self.nq, kw.value), np.uint32)
+        check(load().tm_read_key_bitmaps(self.h, _ptr(rows), rows.size, C.byref(kw)), "tm_read_key_bitmaps")
+        return rows
+
+    def set_host_rows(self, buf):
+        """Deliver each KEY_BITMAP submission's rows into `buf` (a PinnedRows
+        or None to stop)."""
+        self.host_rows = buf
+        if buf is None:
+            check(load().tm_ring_set_host_rows(self.h, None, 0), "tm_ring_set_host_rows")
+        else:
+            check(load().tm_ring_set_host_rows(self.h, C.c_void_p(buf.ptr), buf.words), "tm_ring_set_host_rows")
+
     def lookup_counts(self):
         n = int(self.fib.stats()["n_parts"])
         c = np.zeros(max(n, 1), np.uint32)
@@ -600,6 +634,48 @@ class Ring:
             self.close()
         except Exception:
             pass
+
+
+class PinnedRows:
+    """Pinned host memory (tm_host_alloc) viewed as uint32 words."""
+
+    def __init__(self, words):
+        p = C.c_void_p()
+        check(load().tm_host_alloc(C.byref(p), int(words) * 4), "tm_host_alloc")
+        self.ptr = p.value
+        self.words = int(words)
+        self.array = np.ctypeslib.as_array(C.cast(p, U32P), shape=(max(self.words, 1),))[:self.words]
+
+    def rows(self, nq, kw):
+        return self.array[:nq * kw].reshape(nq, kw)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            load().tm_host_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def expand_key_bitmaps(rows, dictionary, threads=0):
+    """KEY_BITMAP rows (nq, kw) over `dictionary` -> KEYS_UNIQUE (offsets,
+    values), expanded on the host (tm_expand_key_bitmaps)."""
+    rows = np.ascontiguousarray(rows, np.uint32)
+    nq, kw = rows.shape
+    d = np.ascontiguousarray(dictionary, np.uint32)
+    off = np.zeros(nq + 1, np.uint64)
+    lib = load()
+    check(lib.tm_expand_key_bitmaps(_ptr(rows), nq, kw, _ptr(d), d.size, _ptr(off), None, 0, threads),
+          "tm_expand_key_bitmaps")
+    vals = np.zeros(int(off[-1]), np.uint32)
+    check(lib.tm_expand_key_bitmaps(_ptr(rows), nq, kw, _ptr(d), d.size, _ptr(off), _ptr(vals), vals.size, threads),
+          "tm_expand_key_bitmaps")
+    return off, vals
 
 
 def match(fib, queries, mode=MODE_KEYS_UNIQUE, schedule=SCHED_PERSISTENT, **kw):

@@ -29,7 +29,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE, "-I", os.path.join(INCLUDE, "tagmatch")]
 
 HIP_SOURCES = ["hip_util.hip", "fib.hip", "match.hip", "partition.hip", "cost_model.hip"]
-CXX_SOURCES = ["workload.cc", "engine.cc", "server.cc", "io.cc"]
+CXX_SOURCES = ["workload.cc", "engine.cc", "server.cc", "io.cc", "expand.cc"]
 
 
 def _run(cmd):

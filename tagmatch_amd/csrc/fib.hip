@@ -309,6 +309,18 @@ int tm_fib_destroy(tm_fib * fib) {
 	return TM_OK;
 }
 
+int tm_fib_key_dictionary(const tm_fib * fib, uint32_t * keys, uint64_t capacity, uint64_t * n) {
+	if (!fib || !n) return set_error(TM_ERR_INVALID, "NULL argument");
+	std::lock_guard<std::mutex> lk(const_cast<tm_fib *>(fib)->host_mu);   // inserts grow the dictionary
+	*n = fib->h_dict.size();
+	if (!keys) return TM_OK;
+	if (capacity < fib->h_dict.size())
+		return set_error(TM_ERR_OVERFLOW, "%llu keys, capacity %llu", (unsigned long long)fib->h_dict.size(),
+						 (unsigned long long)capacity);
+	if (!fib->h_dict.empty()) std::memcpy(keys, fib->h_dict.data(), fib->h_dict.size() * 4);
+	return TM_OK;
+}
+
 int tm_fib_stats_get(const tm_fib * fib, tm_fib_stats * st) {
 	if (!fib || !st) return set_error(TM_ERR_INVALID, "NULL argument");
 	st->n_filters = fib->n_filters;

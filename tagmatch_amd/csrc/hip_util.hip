@@ -259,4 +259,19 @@ int tm_dev_arch(char * buf, size_t len) {
 	return TM_OK;
 }
 
+int tm_host_alloc(void ** ptr, uint64_t bytes) {
+	if (!ptr) return set_error(TM_ERR_INVALID, "ptr is NULL");
+	*ptr = nullptr;
+	TM_CHECK(require_device());
+	void * p = pinned_alloc(bytes);
+	if (!p) return set_error(TM_ERR_NOMEM, "pinned host allocation of %llu bytes failed", (unsigned long long)bytes);
+	*ptr = p;
+	return TM_OK;
+}
+
+int tm_host_free(void * ptr) {
+	pinned_free(ptr);
+	return TM_OK;
+}
+
 } // extern "C"

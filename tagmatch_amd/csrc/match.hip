@@ -1030,7 +1030,7 @@ __global__ void k_gather_stats(const uint64_t * __restrict__ pbase, uint32_t P,
 		out[0] = pbase[P];        // (query, partition) pairs
 		out[1] = counters[0];     // hit slots taken
 		out[2] = counters[1];     // hits
-		out[3] = off[nq];         // result values
+		out[3] = off ? off[nq] : 0;   // result values (none: KEY_BITMAP rows)
 		out[4] = item_prefix[P];  // work items
 		out[5] = counters[2];     // block visits
 		out[6] = counters[3];     // subset tests
@@ -1393,6 +1393,8 @@ struct tm_ring {
 	uint32_t epoch = 0;        // FIB insert epoch the submission sees
 	uint32_t graph_epoch = 0;
 	const uint32_t * graph_slot_epoch = nullptr;
+	uint32_t * host_rows = nullptr;   // KEY_BITMAP: rows delivered here at the end of each step
+	uint64_t host_rows_cap = 0;       // words
 	bool pending = false;
 	bool match_events = false;
 	int retries = 0;
@@ -1431,16 +1433,19 @@ void tm_ring::release() {
 
 namespace {
 
-int mode_of(const tm_match_config & c) { return c.mode == TM_MODE_KEYS_UNIQUE ? 0 : 1; }
+// KEYS_UNIQUE and KEY_BITMAP both build per-query key bitmap rows; only
+// KEYS_UNIQUE expands them into sorted key lists.
+bool keyrows(const tm_match_config & c) { return c.mode == TM_MODE_KEYS_UNIQUE || c.mode == TM_MODE_KEY_BITMAP; }
+int mode_of(const tm_match_config & c) { return keyrows(c) ? 0 : 1; }
 
 // Effective launch parameters from the config and the FIB.
 int resolve_params(tm_ring * r) {
 	const tm_match_config & c = r->cfg;
-	if (c.mode > TM_MODE_HITS) return set_error(TM_ERR_INVALID, "unknown mode %u", c.mode);
+	if (c.mode > TM_MODE_KEY_BITMAP) return set_error(TM_ERR_INVALID, "unknown mode %u", c.mode);
 	if (c.schedule > TM_SCHED_PERSISTENT) return set_error(TM_ERR_INVALID, "unknown schedule %u", c.schedule);
-	if (c.mode == TM_MODE_KEYS_UNIQUE && r->fib->n_keys > 65536)
+	if (keyrows(c) && r->fib->n_keys > 65536)
 		return set_error(TM_ERR_INVALID,
-						 "KEYS_UNIQUE supports up to 65536 distinct keys (FIB has %u); use KEYS_ALL",
+						 "KEYS_UNIQUE and KEY_BITMAP support up to 65536 distinct keys (FIB has %u); use KEYS_ALL",
 						 r->fib->n_keys);
 	// default batch: the reference's 256 for the plain form (parameters.hh:22),
 	// 128-query tiles for the persistent kernel (measured best at 10M filters)
@@ -1524,7 +1529,9 @@ void enqueue_finalize(tm_ring * r, uint32_t nq) {
 	const int mode = r->cfg.mode;
 	uint32_t gq = (nq + 3) / 4;
 	if (gq == 0) gq = 1;
-	if (mode == TM_MODE_KEYS_UNIQUE) {
+	if (mode == TM_MODE_KEY_BITMAP) {
+		// the rows are the results: no expansion
+	} else if (mode == TM_MODE_KEYS_UNIQUE) {
 		hipLaunchKernelGGL(k_bm_count, dim3(gq), dim3(256), 0, s, r->d_gbm, r->kw, nq, r->d_qcount);
 		scan_u32_to_u64(r->d_qcount, r->d_off, nq, r->d_scan, s);
 		hipLaunchKernelGGL(k_bm_write, dim3(std::max(1u, (nq + BM_WAVES - 1) / BM_WAVES)), dim3(BM_WAVES * 64), 0, s,
@@ -1547,8 +1554,8 @@ void enqueue_finalize(tm_ring * r, uint32_t nq) {
 			hipLaunchKernelGGL(k_hits_write<false>, dim3(gh), dim3(256), 0, s, r->d_hits, r->d_cnt, r->hit_cap, v,
 							   r->d_qcur, r->d_vals, r->val_cap);
 	}
-	hipLaunchKernelGGL(k_gather_stats, dim3(1), dim3(64), 0, s, r->d_pbase, r->P, r->d_cnt, r->d_off, nq,
-					   r->d_iprefix, r->d_stat);
+	hipLaunchKernelGGL(k_gather_stats, dim3(1), dim3(64), 0, s, r->d_pbase, r->P, r->d_cnt,
+					   mode == TM_MODE_KEY_BITMAP ? (const uint64_t *)nullptr : r->d_off, nq, r->d_iprefix, r->d_stat);
 }
 
 // The persistent step: everything between the query upload and the stats.
@@ -1559,7 +1566,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	fill_u64(reinterpret_cast<uint64_t *>(r->d_cnt), 0, 8, s);
 	const uint64_t nqg = (uint64_t)r->tiles * 64;
 	const uint64_t tiles = (nq + nqg - 1) / nqg;
-	const bool exit_on = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
+	const bool exit_on = keyrows(r->cfg) && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
 	const bool rich_pass = exit_on && r->fib->n_rich > 0 && nq > 0;
 	// Compacted step: only the queries whose rows the pass leaves incomplete
 	// are matched (large steps only: the extra kernels cost about 0.1 ms,
@@ -1577,14 +1584,14 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	// table from HBM.
 	uint32_t nseg = 1;
 	const uint32_t nblk = r->fib->n_blocks;
-	const bool can_cut = (r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE) && !r->use_iorder && !r->cfg.log_units;
+	const bool can_cut = (r->lds_bm || !keyrows(r->cfg)) && !r->use_iorder && !r->cfg.log_units;
 	if (can_cut && tiles > 0) {
 		nseg = (uint32_t)std::min<uint64_t>((2 * resident + tiles - 1) / tiles, std::max<uint32_t>(nblk / 16, 1));
-		const uint32_t seg_max = r->cfg.mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->cfg.full_exit != 2
+		const uint32_t seg_max = keyrows(r->cfg) && r->lds_bm && r->cfg.full_exit != 2
 									 ? SEG_MAX_BLOCKS_EXIT : SEG_MAX_BLOCKS;
 		nseg = std::max<uint32_t>(nseg, (nblk + seg_max - 1) / seg_max);
 	}
-	if (r->cfg.chunk_blocks && (r->lds_bm || r->cfg.mode != TM_MODE_KEYS_UNIQUE))   // fixed segment size
+	if (r->cfg.chunk_blocks && (r->lds_bm || !keyrows(r->cfg)))   // fixed segment size
 		nseg = std::max<uint32_t>(1, (nblk + r->cfg.chunk_blocks - 1) / r->cfg.chunk_blocks);
 	const uint32_t seg_blocks = nseg > 1 ? (nblk + nseg - 1) / nseg : std::max<uint32_t>(nblk, 1);
 	if (nseg > 1) nseg = (nblk + seg_blocks - 1) / seg_blocks;
@@ -1615,7 +1622,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 		hipLaunchKernelGGL(k_rich_rows, dim3((nq + 3) / 4), dim3(256), 0, s, r->d_q, nq, r->fib->d_rich_words,
 						   r->fib->d_rich_bm, r->fib->n_rich, r->fib->n_rich_full, r->kw, last_mask, r->d_gbm,
 						   r->d_qfull);
-	} else if (nseg > 1 && r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
+	} else if (nseg > 1 && keyrows(r->cfg)) {
 		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 		fill_u32(r->d_qfull, 0, nq, s);
 	}
@@ -1711,7 +1718,7 @@ void plain_segments(const tm_ring * r, punit & u) {
 	const uint32_t min_seg = r->cfg.chunk_blocks ? r->cfg.chunk_blocks : 64;
 	const uint64_t want = (2 * resident + std::max<uint32_t>(u.ntiles, 1) - 1) / std::max<uint32_t>(u.ntiles, 1);
 	uint32_t nseg = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (nb + min_seg - 1) / min_seg));
-	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->lds_bm) nseg = 1;   // rows in HBM: one workgroup per tile
+	if (keyrows(r->cfg) && !r->lds_bm) nseg = 1;   // rows in HBM: one workgroup per tile
 	u.seg_blocks = std::max<uint32_t>(1, (nb + nseg - 1) / nseg);
 	u.nseg = nb ? (nb + u.seg_blocks - 1) / u.seg_blocks : 1;
 }
@@ -1726,7 +1733,7 @@ int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit
 // bitmaps go to staging rows that k_or_rows merges into the queries' rows.
 int enqueue_plain(tm_ring * r, uint32_t nq) {
 	hipStream_t s = r->s;
-	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
+	if (keyrows(r->cfg)) fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 	enqueue_front(r, nq);
 	std::vector<uint4> sched(r->P);
 	std::vector<uint64_t> pbase(r->P + 1, 0);
@@ -1775,7 +1782,7 @@ int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit
 		TM_CHECK(dalloc(&r->d_unext, (size_t)std::max<uint32_t>(nu, 1) * NQUEUE * QPAD));
 		r->uq_cap = nu;
 	}
-	if (mode == TM_MODE_KEYS_UNIQUE && r->stage_cap < (uint64_t)bq * r->kw) {
+	if (keyrows(r->cfg) && r->stage_cap < (uint64_t)bq * r->kw) {
 		TM_CHECK(grow_u32(&r->d_stage, r->stage_cap, (uint64_t)bq * r->kw));
 		fill_u32(r->d_stage, 0, r->stage_cap, s);
 	}
@@ -1841,7 +1848,7 @@ int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit
 		const uint64_t nit = (uint64_t)u.ntiles * u.nseg;
 		const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(resident, nit));
 		hipLaunchKernelGGL(fn, dim3(grid), dim3(r->waves * 64), r->stage_lds, s, a);
-		if (mode == TM_MODE_KEYS_UNIQUE)
+		if (keyrows(r->cfg))
 			hipLaunchKernelGGL(k_or_rows, dim3((u.nqv * r->kw + 255) / 256), dim3(256), 0, s, r->d_stage, r->kw,
 							   a.qmap, u.nqv, r->d_gbm);
 	}
@@ -1882,17 +1889,17 @@ int ensure_capacity(tm_ring * r) {
 		r->scan_words = std::max(scan_tmp_words(r->max_q), scan_tmp_words(P));
 		TM_CHECK(dalloc(&r->d_scan, r->scan_words));
 	}
-	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->d_gbm) {
+	if (keyrows(r->cfg) && !r->d_gbm) {
 		size_t n = (size_t)r->max_q * r->kw;
 		TM_CHECK(dalloc(&r->d_gbm, n));
 		TM_HIP(hipMemset(r->d_gbm, 0, std::max<size_t>(n, 1) * 4));
 	}
-	if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && !r->d_qfull) {
+	if (keyrows(r->cfg) && !r->d_qfull) {
 		TM_CHECK(dalloc(&r->d_qfull, std::max<uint32_t>(r->max_q, 1)));
 		TM_CHECK(dalloc(&r->d_active, std::max<uint32_t>(r->max_q, 1)));
 		TM_CHECK(dalloc(&r->d_nact, 4));
 	}
-	if (r->cfg.mode != TM_MODE_KEYS_UNIQUE && !r->d_hits) {
+	if (!keyrows(r->cfg) && !r->d_hits) {
 		uint64_t cap = r->cfg.hit_capacity ? r->cfg.hit_capacity : std::max<uint64_t>(1 << 20, (uint64_t)r->max_q * 64);
 		TM_CHECK(dalloc(&r->d_hits, cap));
 		r->hit_cap = cap;
@@ -1907,6 +1914,7 @@ int ensure_capacity(tm_ring * r) {
 		uint64_t per = r->cfg.mode == TM_MODE_KEYS_UNIQUE ? std::min<uint32_t>(r->fib->n_keys, 256) : 64;
 		if (r->cfg.mode == TM_MODE_KEYS_UNIQUE && (uint64_t)r->max_q * r->fib->n_keys <= (1ull << 26))
 			per = std::max<uint32_t>(r->fib->n_keys, 1);
+		if (r->cfg.mode == TM_MODE_KEY_BITMAP) per = 0;   // the rows are the results
 		TM_CHECK(grow_u32(&r->d_vals, r->val_cap, (uint64_t)r->max_q * std::max<uint64_t>(per, 1)));
 	}
 	if (r->cfg.log_units && !r->d_log) {
@@ -1930,7 +1938,7 @@ int enqueue_units(tm_ring * r, uint32_t nq) {
 	const int mode = r->cfg.mode;
 	const tm_fib * f = r->fib;
 	const uint32_t n = (uint32_t)r->u_part.size();
-	if (mode == TM_MODE_KEYS_UNIQUE) {
+	if (keyrows(r->cfg)) {
 		fill_u32(r->d_gbm, 0, (uint64_t)nq * r->kw, s);
 		fill_u32(r->d_qfull, 0, nq, s);
 	}
@@ -1972,7 +1980,7 @@ int enqueue_units(tm_ring * r, uint32_t nq) {
 	}
 	uint32_t seg = r->cfg.chunk_blocks ? r->cfg.chunk_blocks
 									   : (uint32_t)std::max<uint64_t>(4, (work + 4 * resident - 1) / (4 * resident));
-	if (mode == TM_MODE_KEYS_UNIQUE && !r->lds_bm) seg = 0xFFFFFFFFu;   // rows in HBM: one workgroup per tile
+	if (keyrows(r->cfg) && !r->lds_bm) seg = 0xFFFFFFFFu;   // rows in HBM: one workgroup per tile
 	std::vector<uint32_t> tab((size_t)std::max<uint32_t>(n, 1) * 8, 0), pre(n + 1, 0);
 	uint32_t max_nseg = 1;
 	uint64_t items = 0;
@@ -2020,7 +2028,7 @@ int enqueue_units(tm_ring * r, uint32_t nq) {
 	a.hits = r->d_hits;
 	a.hit_cap = r->hit_cap;
 	a.counters = r->d_cnt;
-	a.full_exit = mode == TM_MODE_KEYS_UNIQUE && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
+	a.full_exit = keyrows(r->cfg) && r->lds_bm && r->kw > 0 && r->cfg.full_exit != 2;
 	a.last_mask = (f->n_keys & 31) ? (1u << (f->n_keys & 31)) - 1u : 0xFFFFFFFFu;
 	mq_fn fn = pick_match_q(mode_of(r->cfg), r->lds_bm, r->tiles, r->waves, r->variant);
 	const uint32_t grid = (uint32_t)std::min<uint64_t>(resident, std::max<uint64_t>(items, 1));
@@ -2034,11 +2042,20 @@ int enqueue_units(tm_ring * r, uint32_t nq) {
 	return TM_OK;
 }
 
+// KEY_BITMAP with host delivery: the step's rows into the caller's host
+// buffer, on the ring's stream after the step (outside any captured graph).
+int enqueue_delivery(tm_ring * r, uint32_t nq) {
+	if (r->cfg.mode != TM_MODE_KEY_BITMAP || !r->host_rows || nq == 0 || r->kw == 0) return TM_OK;
+	TM_HIP(hipMemcpyAsync(r->host_rows, r->d_gbm, (size_t)nq * r->kw * 4, hipMemcpyDeviceToHost, r->s));
+	return TM_OK;
+}
+
 int enqueue_step(tm_ring * r, uint32_t nq, bool allow_graph) {
 	TM_HIP(hipEventRecord(r->e0, r->s));
 	if (r->units_mode) {
 		TM_CHECK(enqueue_units(r, nq));
 		r->match_events = true;
+		TM_CHECK(enqueue_delivery(r, nq));
 		TM_HIP(hipMemcpyAsync(r->h_stat, r->d_stat, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, r->s));
 		TM_HIP(hipEventRecord(r->e1, r->s));
 		return TM_OK;
@@ -2076,6 +2093,7 @@ int enqueue_step(tm_ring * r, uint32_t nq, bool allow_graph) {
 		TM_CHECK(enqueue_plain(r, nq));
 		r->match_events = true;
 	}
+	TM_CHECK(enqueue_delivery(r, nq));
 	TM_HIP(hipMemcpyAsync(r->h_stat, r->d_stat, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, r->s));
 	TM_HIP(hipEventRecord(r->e1, r->s));
 	return TM_OK;
@@ -2186,7 +2204,9 @@ int price_submission(tm_ring * r, uint32_t nq) {
 		const tm_fib * f = r->fib;
 		const double margin = 1.25;   // predictions are fitted in log space: allow for spread
 		uint64_t slots = 0, results = 0;
-		if (r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
+		if (r->cfg.mode == TM_MODE_KEY_BITMAP) {
+			results = 0;   // rows, sized for every query already
+		} else if (r->cfg.mode == TM_MODE_KEYS_UNIQUE) {
 			// expected distinct keys among k key instances drawn from K keys,
 			// from the tile's mean instances per query (concave in k, so the
 			// tile mean over-estimates: a safe direction)
@@ -2255,6 +2275,9 @@ int submit_common(tm_ring * r, uint32_t nq) {
 	r->epoch = r->fib->epoch;   // the inserts this submission sees
 	TM_CHECK(sync_fib(r));
 	TM_CHECK(ensure_capacity(r));
+	if (r->cfg.mode == TM_MODE_KEY_BITMAP && r->host_rows && (uint64_t)nq * r->kw > r->host_rows_cap)
+		return set_error(TM_ERR_OVERFLOW, "host rows hold %llu words, the submission needs %llu",
+						 (unsigned long long)r->host_rows_cap, (unsigned long long)nq * r->kw);
 	if (!r->units_mode) TM_CHECK(price_submission(r, nq));
 	int rc = enqueue_step(r, nq, true);
 	if (rc != TM_OK) return rc;
@@ -2425,7 +2448,7 @@ int tm_wait(tm_ring * r, tm_status * st) {
 				TM_CHECK(grow_u32(&r->d_list, r->list_cap, pairs + pairs / 4 + 1024));
 				again = true;
 			}
-			if (r->cfg.mode != TM_MODE_KEYS_UNIQUE && slots > r->hit_cap) {
+			if (!keyrows(r->cfg) && slots > r->hit_cap) {
 				dfree(r->d_hits);
 				uint64_t cap = slots + slots / 4 + 1024;
 				TM_CHECK(dalloc(&r->d_hits, cap));
@@ -2472,6 +2495,8 @@ int tm_wait(tm_ring * r, tm_status * st) {
 int tm_read_results(tm_ring * r, uint64_t * offsets, uint32_t * values, uint64_t capacity) {
 	if (!r || !offsets) return set_error(TM_ERR_INVALID, "NULL argument");
 	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
+	if (r->cfg.mode == TM_MODE_KEY_BITMAP)
+		return set_error(TM_ERR_STATE, "KEY_BITMAP results are rows: tm_read_key_bitmaps");
 	TM_CHECK(require_device());
 	const uint32_t nq = r->last_nq;
 	if (nq == 0) {
@@ -2485,6 +2510,32 @@ int tm_read_results(tm_ring * r, uint64_t * offsets, uint32_t * values, uint64_t
 		return set_error(TM_ERR_OVERFLOW, "%llu result values, capacity %llu", (unsigned long long)total,
 						 (unsigned long long)capacity);
 	if (total) TM_HIP(hipMemcpy(values, r->d_vals, total * sizeof(uint32_t), hipMemcpyDeviceToHost));
+	return TM_OK;
+}
+
+int tm_read_key_bitmaps(tm_ring * r, uint32_t * rows, uint64_t capacity_words, uint32_t * kw) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
+	if (!keyrows(r->cfg)) return set_error(TM_ERR_STATE, "the ring's mode has no key bitmap rows");
+	if (kw) *kw = r->kw;
+	const uint64_t n = (uint64_t)r->last_nq * r->kw;
+	if (n > capacity_words)
+		return set_error(TM_ERR_OVERFLOW, "%llu row words, capacity %llu", (unsigned long long)n,
+						 (unsigned long long)capacity_words);
+	if (n == 0) return TM_OK;
+	if (!rows) return set_error(TM_ERR_INVALID, "rows is NULL");
+	TM_CHECK(require_device());
+	TM_HIP(hipMemcpy(rows, r->d_gbm, n * 4, hipMemcpyDeviceToHost));
+	return TM_OK;
+}
+
+int tm_ring_set_host_rows(tm_ring * r, uint32_t * host_rows, uint64_t capacity_words) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
+	if (host_rows && r->cfg.mode != TM_MODE_KEY_BITMAP)
+		return set_error(TM_ERR_STATE, "host rows are delivered in KEY_BITMAP mode only");
+	r->host_rows = host_rows;
+	r->host_rows_cap = host_rows ? capacity_words : 0;
 	return TM_OK;
 }
 

@@ -437,6 +437,10 @@ int tm_server_create(tm_fib * fib, const tm_server_config * cfg, const tm_match_
 		delete s;
 		return tmx::set_error(TM_ERR_INVALID, "n_slots above 64");
 	}
+	if (s->mcfg.mode == TM_MODE_KEY_BITMAP) {
+		delete s;
+		return tmx::set_error(TM_ERR_INVALID, "the server returns key lists: KEY_BITMAP rings are driven directly");
+	}
 	if (s->cfg.hbm_budget == 0) {
 		size_t fr = 0, tot = 0;
 		int rc = tm_mem_info(&fr, &tot);

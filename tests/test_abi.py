@@ -112,3 +112,31 @@ def test_canonicalize_merges_and_orders():
 
 def test_cpp_api_driver_built():
     assert os.path.exists(os.path.join(ROOT, "tagmatch_amd", "lib", "tm_api_tests"))
+
+
+@pytest.mark.parametrize("n_keys", [1, 31, 32, 33, 100, 2048])
+def test_expand_key_bitmaps_host(n_keys):
+    """tm_expand_key_bitmaps (host code, no GPU): the keys of the set bits in
+    increasing order, bits at or above n_keys ignored, full words and empty
+    rows, against numpy."""
+    rng = np.random.default_rng(n_keys)
+    kw = (n_keys + 31) // 32 + 1                        # one spare word of garbage
+    nq = 3000
+    rows = rng.integers(0, 2**32, size=(nq, kw), dtype=np.uint64).astype(np.uint32)
+    rows[::7] = 0xFFFFFFFF                              # full rows
+    rows[1::11] = 0                                     # empty rows
+    rows[2::5, 0] = 0xFFFFFFFF                          # a full first word
+    dictionary = np.sort(rng.choice(2**31, n_keys, replace=False)).astype(np.uint32)
+    for threads in (1, 0):
+        off, vals = tm.expand_key_bitmaps(rows, dictionary, threads=threads)
+        bits = np.unpackbits(rows.view(np.uint8), axis=1, bitorder="little")[:, :n_keys].astype(bool)
+        assert np.array_equal(np.diff(off), bits.sum(axis=1))
+        exp = np.concatenate([dictionary[b] for b in bits])
+        assert np.array_equal(vals, exp)
+    # capacity too small: TM_ERR_OVERFLOW, offsets still written
+    lib = tm.load()
+    o2 = np.zeros(nq + 1, np.uint64)
+    small = np.zeros(4, np.uint32)
+    rc = lib.tm_expand_key_bitmaps(tm._ptr(rows), nq, kw, tm._ptr(dictionary), n_keys, tm._ptr(o2), tm._ptr(small),
+                                   small.size, 2)
+    assert rc == tm.TM_ERR_OVERFLOW and np.array_equal(o2, off)

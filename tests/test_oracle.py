@@ -277,6 +277,41 @@ def test_schedule_count_equals_bruteforce():
     assert orc.schedule_count(pw, off, masks, q) == orc.bruteforce_count(cw, q)
 
 
+def test_schedule_match_equals_brute_force_keys():
+    """The CPU schedule with key output (lookup + per-partition brute force +
+    sort/dedup) gives the brute force's per-query keys, in all three modes
+    (HITS ids mapped back through the permutation); a planted error in the
+    partition table (one partition's mask widened by a bit its filters lack)
+    loses matches, so the comparison can fail."""
+    import tagmatch_amd as tm
+    cw, cko, ck, q, s = tm.workload(8000, 200, seed=6)
+    rc, perm, off, masks = orc.partition(cw, 500)
+    cnt = (cko[1:] - cko[:-1]).astype(np.int64)[perm]
+    ko = np.zeros(len(perm) + 1, np.uint64)
+    np.cumsum(cnt, out=ko[1:])
+    idx = np.repeat(cko[:-1][perm].astype(np.int64) - ko[:-1].astype(np.int64), cnt) + np.arange(int(ko[-1]))
+    pk = ck[idx]
+    for mode in (0, 1, 2):
+        eo, ev = orc.match(cw, cko, ck, q, mode=mode)
+        go, gv = orc.schedule_match(cw[perm], ko, pk, off, masks, q, mode=mode, threads=3)
+        assert np.array_equal(go, eo), mode
+        for i in range(len(q)):
+            a = gv[int(go[i]):int(go[i + 1])]
+            b = ev[int(eo[i]):int(eo[i + 1])]
+            if mode == 2:
+                a = np.sort(perm[a])
+            elif mode == 1:
+                a = np.sort(a)
+            assert np.array_equal(a, b), (mode, i)
+    bad = masks.copy()
+    big = int(np.argmax(np.diff(off)))
+    free_bit = next(b for b in range(192) if not (bad[big, b // 32] >> (b % 32)) & 1)
+    bad[big, free_bit // 32] |= np.uint32(1 << (free_bit % 32))
+    go, gv = orc.schedule_match(cw[perm], ko, pk, off, bad, q, mode=2)
+    eo, ev = orc.match(cw, cko, ck, q, mode=2)
+    assert int(go[-1]) < int(eo[-1])
+
+
 # ------------------------------------------------------------- consolidate
 
 def _ops(entries):
