@@ -63,6 +63,13 @@ int tm_mem_info(size_t * free_bytes, size_t * total_bytes);
 int tm_dev_synchronize(void);
 /* Name of the active device's architecture (e.g. "gfx950"). */
 int tm_dev_arch(char * buf, size_t len);
+/* Device memory for callers without the HIP headers (bindings, tests):
+ * allocate, free, and synchronous copies to and from it.  The reference
+ * allocates through its gpu class (gpu.cu:825-830). */
+int tm_dev_alloc(void ** ptr, uint64_t bytes);
+int tm_dev_free(void * ptr);
+int tm_dev_upload(void * dev_dst, const void * host_src, uint64_t bytes);
+int tm_dev_download(void * host_dst, const void * dev_src, uint64_t bytes);
 
 /* ------------------------------------------------------------------------ */
 /* Partitioner: the balanced partitioning of a filter table, run on the GPU.
@@ -567,6 +574,8 @@ int tm_gen_queries(const tm_gen * g, uint64_t n_filters, uint64_t first, uint64_
  *                     bitvector.hh:543-580); other lines are skipped.
  *   TM_IO_BINARY_PARTITIONED  FIB entries preceded by a u32 partition id
  *                     (fib.hh:41-56, partition_fib_entry).
+ *   TM_IO_ASCII_PARTITIONED   FIB entries "f <partition> <192 bits> key ..."
+ *                     (fib.cc:100-131, partition_fib_entry::write_ascii).
  * Reading is two calls: scan for the counts, then read into buffers of that
  * size (key_offsets holds n_filters + 1 entries).  Replaces the reference's
  * fib_entry / basic_query read_binary, write_binary, read_ascii and
@@ -574,6 +583,7 @@ int tm_gen_queries(const tm_gen * g, uint64_t n_filters, uint64_t first, uint64_
 #define TM_IO_BINARY              0u
 #define TM_IO_ASCII               1u
 #define TM_IO_BINARY_PARTITIONED  2u
+#define TM_IO_ASCII_PARTITIONED   3u
 
 int tm_io_scan_fib(const char * path, uint32_t format, uint64_t * n_filters, uint64_t * n_keys);
 int tm_io_read_fib(const char * path, uint32_t format, uint32_t * words, uint32_t * key_offsets,
@@ -584,6 +594,63 @@ int tm_io_write_fib(const char * path, uint32_t format, const uint32_t * words, 
 int tm_io_scan_queries(const char * path, uint32_t format, uint64_t * n);
 int tm_io_read_queries(const char * path, uint32_t format, uint32_t * words, uint64_t n);
 int tm_io_write_queries(const char * path, uint32_t format, const uint32_t * words, uint64_t n);
+
+/* Partition prefixes (reference partition_prefix, fib.hh:58-81): the first
+ * `length` bits of `words` are the prefix of partition `partition`, which
+ * holds `size` filters.  TM_IO_BINARY: the 24-byte filter, u8 length, u32
+ * partition, u32 size (33 bytes); TM_IO_ASCII: "p <partition> <the first
+ * length bits> <size>" (fib.cc:72-98), whose bit string's length is read
+ * back as the prefix length. */
+typedef struct tm_prefix {
+	uint32_t words[6];
+	uint32_t length;         /* 0..255, as the reference's uint8_t */
+	uint32_t partition;
+	uint32_t size;
+} tm_prefix;
+int tm_io_scan_prefixes(const char * path, uint32_t format, uint64_t * n);
+int tm_io_read_prefixes(const char * path, uint32_t format, tm_prefix * prefixes, uint64_t n);
+int tm_io_write_prefixes(const char * path, uint32_t format, const tm_prefix * prefixes, uint64_t n);
+
+/* Workload tools: the reference's convert_workload, sample_workload and
+ * split_workload programs (convert_workload.cc:12-59, sample_workload.cc:24-88,
+ * split_workload.cc:19-119) as library calls, on the same files.  Record kinds: */
+#define TM_IO_KIND_FIB      0u   /* FIB entries (any TM_IO_* FIB format)     */
+#define TM_IO_KIND_QUERIES  1u   /* queries (TM_IO_BINARY or TM_IO_ASCII)    */
+#define TM_IO_KIND_PREFIXES 2u   /* partition prefixes (BINARY or ASCII)     */
+
+/* Rewrite every record of `in` (in_format) to `out` (out_format); *n =
+ * records written.  FIB entries keep their partition ids across the
+ * partitioned formats (0 when the input has none).  Unlike the reference
+ * tool, ASCII lines of another record type are skipped rather than
+ * repeating the previous record. */
+int tm_io_convert(const char * in, uint32_t in_format, const char * out, uint32_t out_format, uint32_t kind,
+				  uint64_t * n);
+
+/* Keep k records of `in`, in input order and in the same format, by the
+ * reference tool's selection sampling over n_total records: while fewer
+ * are wanted than remain, a record is kept with probability (records still
+ * wanted) / (records remaining), else it is kept outright.  As in the
+ * reference (sample_workload.cc:24-56), a record kept outright is not
+ * counted as seen, so n_total = k does not keep the first k records and a
+ * choice can fall past the first n_total.  Choices follow the reference
+ * tool's random stream for the same seed (std::default_random_engine, i.e.
+ * minstd_rand0, with libstdc++'s uniform_int_distribution), so its output
+ * files are reproduced byte for byte.  kind: FIB or QUERIES.  *n_out =
+ * records written (k, unless the input runs out first). */
+int tm_io_sample(const char * in, uint32_t format, const char * out, uint32_t kind, uint64_t n_total, uint64_t k,
+				 uint32_t seed, uint64_t * n_out);
+
+/* Cut a FIB file into prefix partitions as the reference's split_workload
+ * does: entries kept in input order when it is strictly ascending, else
+ * sorted in descending filter order; then each partition starts at the next
+ * entry and takes the following entries while the longest common prefix
+ * stays longer than that entry's min_k-th set bit, at most max_size entries
+ * (split_workload.cc:37-119).  format TM_IO_BINARY or TM_IO_ASCII for the
+ * input and both outputs: prefixes (NULL: not written) and partitioned FIB
+ * entries (NULL: not written; TM_IO_BINARY_PARTITIONED or
+ * TM_IO_ASCII_PARTITIONED records).  *n_parts = partitions. */
+int tm_io_split(const char * in, uint32_t format, const char * prefixes_out, const char * filters_out,
+				uint32_t max_size, uint32_t min_k, uint64_t * n_parts);
 
 #ifdef __cplusplus
 }

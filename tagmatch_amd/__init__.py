@@ -31,7 +31,8 @@ SCHED_PLAIN = 0
 SCHED_PERSISTENT = 1
 
 
-IO_BINARY, IO_ASCII, IO_BINARY_PARTITIONED = 0, 1, 2
+IO_BINARY, IO_ASCII, IO_BINARY_PARTITIONED, IO_ASCII_PARTITIONED = 0, 1, 2, 3
+KIND_FIB, KIND_QUERIES, KIND_PREFIXES = 0, 1, 2
 
 
 class TagMatchError(RuntimeError):
@@ -79,6 +80,10 @@ class FibStats(C.Structure):
         ("n_keys", C.c_uint64), ("n_distinct_keys", C.c_uint64), ("device_bytes", C.c_uint64),
         ("max_part_size", C.c_uint64),
     ]
+
+
+class Prefix(C.Structure):
+    _fields_ = [("words", C.c_uint32 * 6), ("length", C.c_uint32), ("partition", C.c_uint32), ("size", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -136,6 +141,10 @@ _SIGS = {
     "tm_read_key_bitmaps": (C.c_int, [P, P, C.c_uint64, U32P]),
     "tm_ring_set_host_rows": (C.c_int, [P, P, C.c_uint64]),
     "tm_host_alloc": (C.c_int, [C.POINTER(P), C.c_uint64]),
+    "tm_dev_alloc": (C.c_int, [C.POINTER(P), C.c_uint64]),
+    "tm_dev_free": (C.c_int, [P]),
+    "tm_dev_upload": (C.c_int, [P, P, C.c_uint64]),
+    "tm_dev_download": (C.c_int, [P, P, C.c_uint64]),
     "tm_host_free": (C.c_int, [P]),
     "tm_fib_key_dictionary": (C.c_int, [P, P, C.c_uint64, U64P]),
     "tm_expand_key_bitmaps": (C.c_int, [P, C.c_uint64, C.c_uint32, P, C.c_uint32, P, P, C.c_uint64, C.c_uint32]),
@@ -180,6 +189,13 @@ _SIGS = {
     "tm_io_scan_queries": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "tm_io_read_queries": (C.c_int, [C.c_char_p, C.c_uint32, P, C.c_uint64]),
     "tm_io_write_queries": (C.c_int, [C.c_char_p, C.c_uint32, P, C.c_uint64]),
+    "tm_io_scan_prefixes": (C.c_int, [C.c_char_p, C.c_uint32, U64P]),
+    "tm_io_read_prefixes": (C.c_int, [C.c_char_p, C.c_uint32, P, C.c_uint64]),
+    "tm_io_write_prefixes": (C.c_int, [C.c_char_p, C.c_uint32, P, C.c_uint64]),
+    "tm_io_convert": (C.c_int, [C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32, C.c_uint32, U64P]),
+    "tm_io_sample": (C.c_int, [C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32,
+                               U64P]),
+    "tm_io_split": (C.c_int, [C.c_char_p, C.c_uint32, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, U64P]),
 }
 
 # Declared in include/tagmatch_amd.h; tests check every one is exported.
@@ -636,6 +652,37 @@ class Ring:
             pass
 
 
+class DeviceBuffer:
+    """Device memory from tm_dev_alloc, with synchronous copies."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        check(load().tm_dev_alloc(C.byref(p), int(nbytes)), "tm_dev_alloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(load().tm_dev_upload(C.c_void_p(self.ptr), _ptr(a), a.nbytes), "tm_dev_upload")
+
+    def download(self, a):
+        assert a.nbytes <= self.nbytes and a.flags["C_CONTIGUOUS"]
+        check(load().tm_dev_download(_ptr(a), C.c_void_p(self.ptr), a.nbytes), "tm_dev_download")
+        return a
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            load().tm_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class PinnedRows:
     """Pinned host memory (tm_host_alloc) viewed as uint32 words."""
 
@@ -922,3 +969,49 @@ def read_query_file(path, fmt=IO_BINARY):
 def write_query_file(path, words, fmt=IO_BINARY):
     q = np.ascontiguousarray(words, np.uint32).reshape(-1, 6)
     check(load().tm_io_write_queries(os.fsencode(path), fmt, _ptr(q), q.shape[0]), "tm_io_write_queries")
+
+
+def read_prefix_file(path, fmt=IO_BINARY):
+    """Partition prefixes (tm_io_read_prefixes): dict of words (n x 6),
+    length, partition, size."""
+    n = C.c_uint64()
+    bp = os.fsencode(path)
+    check(load().tm_io_scan_prefixes(bp, fmt, C.byref(n)), "tm_io_scan_prefixes")
+    arr = (Prefix * max(n.value, 1))()
+    check(load().tm_io_read_prefixes(bp, fmt, arr, n.value), "tm_io_read_prefixes")
+    a = np.ctypeslib.as_array(C.cast(arr, U32P), shape=(max(n.value, 1) * 9,)).reshape(-1, 9)[:n.value]
+    return {"words": a[:, :6].copy(), "length": a[:, 6].copy(), "partition": a[:, 7].copy(), "size": a[:, 8].copy()}
+
+
+def write_prefix_file(path, words, length, partition, size, fmt=IO_BINARY):
+    a = np.zeros((len(length), 9), np.uint32)
+    a[:, :6] = np.asarray(words, np.uint32).reshape(-1, 6)
+    a[:, 6] = length
+    a[:, 7] = partition
+    a[:, 8] = size
+    check(load().tm_io_write_prefixes(os.fsencode(path), fmt, _ptr(a), a.shape[0]), "tm_io_write_prefixes")
+
+
+def convert_file(src, src_fmt, dst, dst_fmt, kind=KIND_FIB):
+    """The reference's convert_workload (tm_io_convert); returns records written."""
+    n = C.c_uint64()
+    check(load().tm_io_convert(os.fsencode(src), src_fmt, os.fsencode(dst), dst_fmt, kind, C.byref(n)),
+          "tm_io_convert")
+    return n.value
+
+
+def sample_file(src, fmt, dst, kind, n_total, k, seed):
+    """The reference's sample_workload (tm_io_sample); returns records written."""
+    n = C.c_uint64()
+    check(load().tm_io_sample(os.fsencode(src), fmt, os.fsencode(dst), kind, n_total, k, seed, C.byref(n)),
+          "tm_io_sample")
+    return n.value
+
+
+def split_file(src, fmt, prefixes_path, filters_path, max_size=100, min_k=2):
+    """The reference's split_workload (tm_io_split); returns partitions."""
+    n = C.c_uint64()
+    pp = os.fsencode(prefixes_path) if prefixes_path else None
+    fp = os.fsencode(filters_path) if filters_path else None
+    check(load().tm_io_split(os.fsencode(src), fmt, pp, fp, max_size, min_k, C.byref(n)), "tm_io_split")
+    return n.value

@@ -259,6 +259,38 @@ int tm_dev_arch(char * buf, size_t len) {
 	return TM_OK;
 }
 
+int tm_dev_alloc(void ** ptr, uint64_t bytes) {
+	if (!ptr) return set_error(TM_ERR_INVALID, "ptr is NULL");
+	*ptr = nullptr;
+	TM_CHECK(require_device());
+	void * p = nullptr;
+	if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
+		return set_error(TM_ERR_NOMEM, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
+	*ptr = p;
+	return TM_OK;
+}
+
+int tm_dev_free(void * ptr) {
+	if (ptr) TM_HIP(hipFree(ptr));
+	return TM_OK;
+}
+
+int tm_dev_upload(void * dev_dst, const void * host_src, uint64_t bytes) {
+	if (bytes == 0) return TM_OK;
+	if (!dev_dst || !host_src) return set_error(TM_ERR_INVALID, "NULL argument");
+	TM_CHECK(require_device());
+	TM_HIP(hipMemcpy(dev_dst, host_src, bytes, hipMemcpyHostToDevice));
+	return TM_OK;
+}
+
+int tm_dev_download(void * host_dst, const void * dev_src, uint64_t bytes) {
+	if (bytes == 0) return TM_OK;
+	if (!host_dst || !dev_src) return set_error(TM_ERR_INVALID, "NULL argument");
+	TM_CHECK(require_device());
+	TM_HIP(hipMemcpy(host_dst, dev_src, bytes, hipMemcpyDeviceToHost));
+	return TM_OK;
+}
+
 int tm_host_alloc(void ** ptr, uint64_t bytes) {
 	if (!ptr) return set_error(TM_ERR_INVALID, "ptr is NULL");
 	*ptr = nullptr;
