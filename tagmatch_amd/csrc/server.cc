@@ -55,10 +55,16 @@ struct request {
 	uint32_t ins_n = 0, ins_maxp = 0;
 	bool ins_tried_live = false;      // the live insert declined: apply at a barrier
 	tmx::live_plan * plan = nullptr;  // prepared when submitted: the engine only commits it
-	// results: offsets of its own, values a slice of its step's buffer
+	// results: offsets of its own, values a slice of its step's buffer; or,
+	// for KEYS_UNIQUE, rows [row0, row0 + nq) of its step's key bitmap rows
+	// over the dictionary the step ran with, expanded when they are read
 	std::vector<uint64_t> off;
 	std::shared_ptr<struct result_buf> rbuf;
 	uint64_t vbase = 0, vn = 0;
+	bool rows = false, counted = false;
+	uint64_t row0 = 0;
+	uint32_t kw = 0;
+	std::shared_ptr<const std::vector<uint32_t>> dict;
 	int status = TM_OK;
 	std::string error;
 	uint64_t reserved = 0;            // bytes of the HBM budget held
@@ -94,6 +100,11 @@ struct slot {
 	bool busy = false;
 	std::vector<uint32_t> words;
 	std::vector<uint64_t> off;
+	// KEYS_UNIQUE: the step's rows, delivered by its ring into this pinned
+	// buffer, and the dictionary they index
+	std::shared_ptr<struct result_buf> rows;
+	std::shared_ptr<const std::vector<uint32_t>> dict;
+	uint32_t kw = 0;
 };
 
 } // namespace
@@ -121,6 +132,11 @@ struct tm_server {
 	std::mutex ins_mu;                // orders the preparation of inserts
 	std::atomic<uint64_t> arrivals{0};   // requests and inserts submitted so far
 	uint32_t unprepared = 0;          // queued inserts without a plan (they change the layout)
+	// KEYS_UNIQUE requests are served as key bitmap rows (the slots' rings run
+	// TM_MODE_KEY_BITMAP and copy each step's rows to the host as part of the
+	// step); the dictionary is replaced when an insert changes it
+	bool rows_mode = false;
+	std::shared_ptr<const std::vector<uint32_t>> dict;
 };
 
 namespace {
@@ -128,6 +144,19 @@ namespace {
 // Result bytes a request of nq queries is charged against the budget.
 uint64_t estimate(const tm_server * s, uint32_t nq) {
 	return (uint64_t)((double)nq * (s->bytes_per_query * 1.5 + (double)s->fixed_per_query)) + 4096;
+}
+
+// The FIB's current key dictionary (rows mode).
+int refresh_dict(tm_server * s) {
+	uint64_t n = 0;
+	int rc = tm_fib_key_dictionary(s->fib, nullptr, 0, &n);
+	if (rc != TM_OK) return rc;
+	auto d = std::make_shared<std::vector<uint32_t>>(n);
+	rc = tm_fib_key_dictionary(s->fib, d->data(), n, &n);
+	if (rc != TM_OK) return rc;
+	d->resize(n);
+	s->dict = std::move(d);
+	return TM_OK;
 }
 
 void finish(tm_server * s, request * r) {
@@ -181,6 +210,36 @@ void complete(tm_server * s, slot & sl) {
 	tm_status st;
 	int rc = tm_wait(sl.ring, &st);
 	const uint32_t nq = (uint32_t)(sl.words.size() / 6);
+	if (s->rows_mode) {
+		// the rows are in host memory already: hand each request its slice
+		std::string err = rc == TM_OK ? std::string() : std::string(tm_last_error());
+		std::lock_guard<std::mutex> lk(s->mu);
+		uint32_t q0 = 0;
+		for (request * r : sl.reqs) {
+			r->status = rc;
+			r->step_queries = nq;
+			r->rows = true;
+			r->rbuf = sl.rows;
+			r->row0 = q0;
+			r->kw = sl.kw;
+			r->dict = sl.dict;
+			if (rc != TM_OK) r->error = err;
+			q0 += r->nq;
+			finish(s, r);
+		}
+		if (rc == TM_OK) {
+			s->st.n_steps++;
+			s->st.device_ms_sum += st.device_ms;
+		}
+		s->reserved -= sl.reserved;
+		sl.reserved = 0;
+		sl.reqs.clear();
+		sl.rows.reset();
+		sl.dict.reset();
+		sl.busy = false;
+		s->cv_done.notify_all();
+		return;
+	}
 	std::shared_ptr<result_buf> buf;
 	if (rc == TM_OK) {
 		sl.off.resize((size_t)nq + 1);
@@ -292,7 +351,17 @@ void engine(tm_server * s) {
 				if (r->nq) std::memcpy(&sl.words[(size_t)q0 * 6], r->q.data(), (size_t)r->nq * 24);
 				q0 += r->nq;
 			}
-			int rc = tm_submit(sl.ring, sl.words.data(), nq);
+			int rc = TM_OK;
+			if (s->rows_mode) {
+				// this step's rows go straight into a pinned buffer its
+				// requests will share
+				sl.dict = s->dict;
+				sl.kw = (uint32_t)((sl.dict->size() + 31) / 32);
+				sl.rows = take_buf(s, std::max<uint64_t>((uint64_t)nq * sl.kw, 1));
+				rc = sl.rows ? tm_ring_set_host_rows(sl.ring, sl.rows->p, sl.rows->cap)
+							 : tmx::set_error(TM_ERR_NOMEM, "pinned rows of %u queries", nq);
+			}
+			if (rc == TM_OK) rc = tm_submit(sl.ring, sl.words.data(), nq);
 			lk.lock();
 			if (rc != TM_OK) {
 				const std::string err = tm_last_error();
@@ -304,6 +373,8 @@ void engine(tm_server * s) {
 				s->reserved -= sl.reserved;
 				sl.reserved = 0;
 				sl.reqs.clear();
+				sl.rows.reset();
+				sl.dict.reset();
 				s->cv_done.notify_all();
 				continue;
 			}
@@ -364,6 +435,7 @@ void engine(tm_server * s) {
 				lk.unlock();
 				int rc = tm_fib_insert(s->fib, r->ins_words.data(), r->ins_n, r->ins_koff.data(), r->ins_keys.data(),
 									   r->ins_fid.empty() ? nullptr : r->ins_fid.data(), r->ins_maxp);
+				if (rc == TM_OK && s->rows_mode) rc = refresh_dict(s);   // new keys extend the dictionary
 				const std::string err = rc == TM_OK ? std::string() : std::string(tm_last_error());
 				lk.lock();
 				s->unprepared--;
@@ -454,10 +526,20 @@ int tm_server_create(tm_fib * fib, const tm_server_config * cfg, const tm_match_
 	tm_fib_stats_get(fib, &fs);
 	// key bitmap rows are per query and independent of hits
 	s->fixed_per_query = s->mcfg.mode == TM_MODE_KEYS_UNIQUE ? 4 * ((fs.n_distinct_keys + 31) / 32) : 8;
+	s->rows_mode = s->mcfg.mode == TM_MODE_KEYS_UNIQUE;
+	if (s->rows_mode) {
+		int rc = refresh_dict(s);
+		if (rc != TM_OK) {
+			delete s;
+			return rc;
+		}
+	}
+	tm_match_config ring_cfg = s->mcfg;
+	if (s->rows_mode) ring_cfg.mode = TM_MODE_KEY_BITMAP;
 	s->slots.resize(s->cfg.n_slots);
 	s->pool->keep = 2 * (size_t)s->cfg.n_slots + 4;
 	for (auto & sl : s->slots) {
-		int rc = tm_ring_create(fib, s->cfg.slot_queries, &s->mcfg, &sl.ring);
+		int rc = tm_ring_create(fib, s->cfg.slot_queries, &ring_cfg, &sl.ring);
 		if (rc != TM_OK) {
 			for (auto & x : s->slots)
 				if (x.ring) tm_ring_destroy(x.ring);
@@ -597,11 +679,48 @@ int tm_server_collect(tm_server * s, uint64_t * ids, uint32_t cap, uint32_t * n)
 int tm_server_result(tm_server * s, uint64_t id, tm_request_info * info, uint64_t * offsets, uint32_t * values,
 					 uint64_t capacity) {
 	if (!s) return tmx::set_error(TM_ERR_INVALID, "server is NULL");
-	std::lock_guard<std::mutex> lk(s->mu);
+	std::unique_lock<std::mutex> lk(s->mu);
 	auto it = s->reqs.find(id);
 	if (it == s->reqs.end()) return tmx::set_error(TM_ERR_INVALID, "unknown request %llu", (unsigned long long)id);
 	request * r = it->second;
 	if (!r->done) return tmx::set_error(TM_ERR_BUSY, "request %llu still running", (unsigned long long)id);
+	if (r->rows && r->status == TM_OK) {
+		// key lists from the step's rows, expanded outside the server lock
+		// (the caller does not release this request concurrently)
+		const std::shared_ptr<result_buf> rb = r->rbuf;
+		const std::shared_ptr<const std::vector<uint32_t>> dict = r->dict;
+		const uint32_t * rows = rb->p + r->row0 * r->kw;
+		const uint32_t nq = r->nq, kw = r->kw, K = (uint32_t)dict->size();
+		if (!r->counted) {
+			lk.unlock();
+			std::vector<uint64_t> off((size_t)nq + 1);
+			int rc = tm_expand_key_bitmaps(rows, nq, kw, dict->data(), K, off.data(), nullptr, 0, 1);
+			if (rc != TM_OK) return rc;
+			lk.lock();
+			r->off = std::move(off);
+			r->vn = r->off[nq];
+			r->counted = true;
+		}
+		if (info) {
+			info->status = r->status;
+			info->n_queries = r->nq;
+			info->n_values = r->vn;
+			info->step_queries = r->step_queries;
+			info->latency_ms = ms_since(r->t_submit, r->t_done);
+			info->queue_ms = ms_since(r->t_submit, r->t_admit);
+		}
+		if (offsets) std::memcpy(offsets, r->off.data(), r->off.size() * sizeof(uint64_t));
+		const uint64_t vn = r->vn;
+		lk.unlock();
+		if (values) {
+			if (vn > capacity)
+				return tmx::set_error(TM_ERR_OVERFLOW, "%llu values, capacity %llu", (unsigned long long)vn,
+									  (unsigned long long)capacity);
+			std::vector<uint64_t> off((size_t)nq + 1);
+			return tm_expand_key_bitmaps(rows, nq, kw, dict->data(), K, off.data(), values, capacity, 1);
+		}
+		return TM_OK;
+	}
 	if (info) {
 		info->status = r->status;
 		info->n_queries = r->nq;
