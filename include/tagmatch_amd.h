@@ -367,6 +367,26 @@ int tm_expand_key_bitmaps(const uint32_t * rows, uint64_t nq, uint32_t kw, const
  * front_end.cc:350-376).  counts: n_parts entries. */
 int tm_ring_lookup_counts(tm_ring * ring, uint32_t * counts, uint32_t n_parts);
 
+/* No-allocation match.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It runs the persistent
+ * KEY_BITMAP step of a ring -- lookup, rich-filter pass, compacted match --
+ * over nq queries already in device memory (dev_queries: nq x 8 words, the
+ * 6 filter words and 2 pad words, the layout of tm_ring_query_buffer) and
+ * writes their key bitmap rows to device memory (dev_rows: nq x
+ * ceil(K/32) words), on `stream` (a hipStream_t; NULL: the default stream),
+ * and returns when they are written.  It allocates nothing: its working
+ * buffers live in a fixed static device array of 0.82 MiB per GPU,
+ * which bounds nq by TM_STATIC_MAX_QUERIES and the FIB's partitions by
+ * TM_STATIC_MAX_PARTS (larger sizes: TM_ERR_INVALID).  Calls on one GPU share
+ * that array: calls that overlap on one GPU (from several threads) race.
+ * The FIB must not be changed by tm_fib_insert / upload / load during a call
+ * (live inserts are safe: the call sees the inserts made before it). */
+#define TM_STATIC_MAX_QUERIES 65536
+#define TM_STATIC_MAX_PARTS   16384
+int tm_match_rows_static(tm_fib * fib, const uint32_t * dev_queries, uint32_t nq, uint32_t * dev_rows,
+						 void * stream);
+
 /* One-shot synchronous match through a temporary ring. */
 int tm_match(tm_fib * fib, const uint32_t * query_words, uint32_t nq,
 			 const tm_match_config * cfg, uint64_t * offsets, uint32_t * values,

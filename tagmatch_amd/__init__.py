@@ -27,6 +27,8 @@ MODE_KEYS_UNIQUE = 0
 MODE_KEYS_ALL = 1
 MODE_HITS = 2
 MODE_KEY_BITMAP = 3
+STATIC_MAX_QUERIES = 65536
+STATIC_MAX_PARTS = 16384
 SCHED_PLAIN = 0
 SCHED_PERSISTENT = 1
 
@@ -140,6 +142,7 @@ _SIGS = {
     "tm_ring_results_device": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
     "tm_read_key_bitmaps": (C.c_int, [P, P, C.c_uint64, U32P]),
     "tm_ring_set_host_rows": (C.c_int, [P, P, C.c_uint64]),
+    "tm_match_rows_static": (C.c_int, [P, P, C.c_uint32, P, P]),
     "tm_host_alloc": (C.c_int, [C.POINTER(P), C.c_uint64]),
     "tm_dev_alloc": (C.c_int, [C.POINTER(P), C.c_uint64]),
     "tm_dev_free": (C.c_int, [P]),

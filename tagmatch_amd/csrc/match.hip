@@ -31,6 +31,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -1393,6 +1394,7 @@ struct tm_ring {
 	uint32_t epoch = 0;        // FIB insert epoch the submission sees
 	uint32_t graph_epoch = 0;
 	const uint32_t * graph_slot_epoch = nullptr;
+	bool no_weights = false;          // segments by block count only (no per-block weight arrays)
 	uint32_t * host_rows = nullptr;   // KEY_BITMAP: rows delivered here at the end of each step
 	uint64_t host_rows_cap = 0;       // words
 	bool pending = false;
@@ -1600,7 +1602,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	// counts.  Blocks of low-popcount filters are matched by most dense
 	// queries; uniform segments put them in a few long work items, which
 	// then set the step's time.
-	const bool weighted = nseg > 1 && tiles <= WSEG_MAX_TILES && !r->cfg.chunk_blocks;
+	const bool weighted = nseg > 1 && tiles <= WSEG_MAX_TILES && !r->cfg.chunk_blocks && !r->no_weights;
 	if (weighted) {
 		const float l2 = std::round(r->rho_l2 * 16.f) / 16.f;
 		if (r->bw_version != r->fib->version || r->bw_nb != nblk || r->bw_l2 != l2) {
@@ -2636,6 +2638,118 @@ int tm_match(tm_fib * fib, const uint32_t * words, uint32_t nq, const tm_match_c
 	if (rc == TM_OK) rc = tm_read_results(r, offsets, values, capacity);
 	tm_ring_destroy(r);
 	return rc;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------------
+// No-allocation entry point (tm_match_rows_static).
+//
+// The persistent KEY_BITMAP step of a ring, for callers that cannot
+// allocate device memory: every buffer the step's kernels need beyond the
+// caller's queries and rows (lookup counts and their prefix sums, the work
+// queues, counters, full-key-set flags and the active-query list) is carved
+// out of `scratch`, a file-scope device array of fixed size.  Each GPU the
+// library runs on holds its own copy of it until the process ends.  Host
+// code must not use the array's name as a device address: its address on
+// each device comes from hipGetSymbolAddress, taken once per device on the
+// entry point's first call there.  Calls on one GPU share that GPU's array;
+// the call clears what must start at zero, and calls that overlap on one GPU
+// race (the call is synchronous, so calls from one thread never overlap).
+
+namespace {
+
+constexpr size_t SC_ALIGN = 256;
+constexpr size_t sc_round(size_t b) { return (b + SC_ALIGN - 1) / SC_ALIGN * SC_ALIGN; }
+constexpr size_t SC_P = (size_t)TM_STATIC_MAX_PARTS + 1;
+constexpr size_t SC_Q = (size_t)TM_STATIC_MAX_QUERIES;
+// byte offsets of the sub-arrays
+constexpr size_t SC_PCOUNT = 0;                                          // u32 [P + 1]
+constexpr size_t SC_PBASE = SC_PCOUNT + sc_round(SC_P * 4);             // u64 [P + 1]
+constexpr size_t SC_IPREFIX = SC_PBASE + sc_round(SC_P * 8);            // u64 [P + 1]
+constexpr size_t SC_QBEG = SC_IPREFIX + sc_round(SC_P * 8);             // u64 [NQUEUE + 1]
+constexpr size_t SC_QNEXT = SC_QBEG + sc_round((NQUEUE + 1) * 8);       // u64 [NQUEUE * QPAD]
+constexpr size_t SC_CNT = SC_QNEXT + sc_round((size_t)NQUEUE * QPAD * 8);   // u64 [8]
+constexpr size_t SC_STAT = SC_CNT + sc_round(8 * 8);                    // u64 [8]
+constexpr size_t SC_NACT = SC_STAT + sc_round(8 * 8);                   // u32 [4]
+constexpr size_t SC_SCAN = SC_NACT + sc_round(4 * 4);                   // u64 [scan words of P]
+constexpr size_t SC_SCAN_WORDS = (SC_P + 255) / 256 + 2;                // >= scan_tmp_words(P)
+constexpr size_t SC_QFULL = SC_SCAN + sc_round(SC_SCAN_WORDS * 8);      // u32 [Q]
+constexpr size_t SC_ACTIVE = SC_QFULL + sc_round(SC_Q * 4);             // u32 [Q]
+constexpr size_t SC_BYTES = SC_ACTIVE + sc_round(SC_Q * 4);
+
+static __device__ __align__(256) unsigned char scratch[SC_BYTES];
+
+constexpr int SC_MAX_DEVICES = 64;
+std::mutex sc_mu;
+unsigned char * sc_addr[SC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device.
+int scratch_on_device(unsigned char ** out) {
+	const int d = dev().device;
+	if (d < 0 || d >= SC_MAX_DEVICES) return set_error(TM_ERR_STATE, "no device");
+	std::lock_guard<std::mutex> lk(sc_mu);
+	if (!sc_addr[d]) {
+		void * p = nullptr;
+		TM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		if (!p) return set_error(TM_ERR_HIP, "no device address for the scratch array");
+		sc_addr[d] = static_cast<unsigned char *>(p);
+	}
+	*out = sc_addr[d];
+	return TM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int tm_match_rows_static(tm_fib * fib, const uint32_t * dev_queries, uint32_t nq, uint32_t * dev_rows,
+						 void * stream) {
+	if (!fib) return set_error(TM_ERR_INVALID, "fib is NULL");
+	if (nq > TM_STATIC_MAX_QUERIES)
+		return set_error(TM_ERR_INVALID, "%u queries: this entry point takes at most %u", nq,
+						 (unsigned)TM_STATIC_MAX_QUERIES);
+	if (nq && (!dev_queries || !dev_rows)) return set_error(TM_ERR_INVALID, "queries or rows are NULL");
+	if (fib->h_order.size() != 192 || !fib->blk_words) return set_error(TM_ERR_STATE, "no table uploaded");
+	if (fib->n_parts > TM_STATIC_MAX_PARTS)
+		return set_error(TM_ERR_INVALID, "the FIB has %u partitions: this entry point takes at most %u",
+						 fib->n_parts, (unsigned)TM_STATIC_MAX_PARTS);
+	TM_CHECK(require_device());
+	if (nq == 0) return TM_OK;
+	unsigned char * sc = nullptr;
+	TM_CHECK(scratch_on_device(&sc));
+	tm_ring r;   // host-side state only; no buffer of its own
+	r.fib = fib;
+	tm_match_config_default(&r.cfg);
+	r.cfg.mode = TM_MODE_KEY_BITMAP;
+	r.max_q = nq;
+	r.P = fib->n_parts;
+	r.kw = (fib->n_keys + 31) / 32;
+	TM_CHECK(resolve_params(&r));
+	if (scan_tmp_words(r.P) > SC_SCAN_WORDS) return set_error(TM_ERR_STATE, "scan scratch too small");
+	r.no_weights = true;
+	r.s = static_cast<hipStream_t>(stream);
+	r.d_q = const_cast<uint32_t *>(dev_queries);
+	r.d_gbm = dev_rows;
+	r.d_pcount = reinterpret_cast<uint32_t *>(sc + SC_PCOUNT);
+	r.d_pbase = reinterpret_cast<uint64_t *>(sc + SC_PBASE);
+	r.d_iprefix = reinterpret_cast<uint64_t *>(sc + SC_IPREFIX);
+	r.d_qbeg = reinterpret_cast<uint64_t *>(sc + SC_QBEG);
+	r.d_qnext = reinterpret_cast<unsigned long long *>(sc + SC_QNEXT);
+	r.d_cnt = reinterpret_cast<unsigned long long *>(sc + SC_CNT);
+	r.d_stat = reinterpret_cast<uint64_t *>(sc + SC_STAT);
+	r.d_nact = reinterpret_cast<uint32_t *>(sc + SC_NACT);
+	r.d_scan = reinterpret_cast<uint64_t *>(sc + SC_SCAN);
+	r.d_qfull = reinterpret_cast<uint32_t *>(sc + SC_QFULL);
+	r.d_active = reinterpret_cast<uint32_t *>(sc + SC_ACTIVE);
+	r.epoch = fib->epoch;   // the inserts this call sees
+	fib->in_flight.fetch_add(1);
+	int rc = enqueue_persistent(&r, nq, false);
+	const hipError_t e = hipStreamSynchronize(r.s);
+	fib->in_flight.fetch_sub(1);
+	if (rc != TM_OK) return rc;
+	TM_HIP(e);
+	return TM_OK;
 }
 
 } // extern "C"
