@@ -479,7 +479,11 @@ typedef struct tm_server_config {
 	 * or the server stops.  0 (the default) starts a step at once. */
 	uint32_t flush_queries;
 	uint32_t flush_us;
-	uint32_t reserved[6];
+	/* Steps replayed from captured HIP graphs, one per power-of-two step
+	 * size (persistent schedule; requests padded with zero queries): 0 =
+	 * default (on), 1 = on, 2 = off. */
+	uint32_t step_graphs;
+	uint32_t reserved[5];
 } tm_server_config;
 
 typedef struct tm_server_stats {

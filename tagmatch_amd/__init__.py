@@ -90,7 +90,8 @@ class Prefix(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("n_slots", C.c_uint32), ("slot_queries", C.c_uint32), ("hbm_budget", C.c_uint64),
-                ("flush_queries", C.c_uint32), ("flush_us", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+                ("flush_queries", C.c_uint32), ("flush_us", C.c_uint32), ("step_graphs", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
 
 
 class ServerStats(C.Structure):
@@ -747,7 +748,7 @@ class Server:
     """Continuous-batching serving loop (tm_server)."""
 
     def __init__(self, fib, n_slots=4, slot_queries=16384, hbm_budget=0, cfg=None, flush_queries=0, flush_us=0,
-                 **kw):
+                 step_graphs=0, **kw):
         sc = ServerConfig()
         load().tm_server_config_default(C.byref(sc))
         sc.n_slots = n_slots
@@ -755,6 +756,7 @@ class Server:
         sc.hbm_budget = hbm_budget
         sc.flush_queries = flush_queries
         sc.flush_us = flush_us
+        sc.step_graphs = step_graphs
         self.cfg = cfg if cfg is not None else config(**kw)
         self.fib = fib
         self.h = C.c_void_p()

@@ -536,6 +536,9 @@ int tm_server_create(tm_fib * fib, const tm_server_config * cfg, const tm_match_
 	}
 	tm_match_config ring_cfg = s->mcfg;
 	if (s->rows_mode) ring_cfg.mode = TM_MODE_KEY_BITMAP;
+	// small steps spend much of their time launching kernels: replay them
+	// from graphs (a step of nq queries runs as the next power of two)
+	if (s->cfg.step_graphs != 2 && ring_cfg.schedule == TM_SCHED_PERSISTENT) ring_cfg.use_graph = 1;
 	s->slots.resize(s->cfg.n_slots);
 	s->pool->keep = 2 * (size_t)s->cfg.n_slots + 4;
 	for (auto & sl : s->slots) {
