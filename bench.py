@@ -320,8 +320,15 @@ def uneven_leg(cw, cko, ck, args):
     ring.submit_units(q, up, uo)
     ring.wait()
     ref = ring.results()
-    for name, sched, steps in (("persistent", tm.SCHED_PERSISTENT, 3), ("plain", tm.SCHED_PLAIN, 1)):
+    for name, sched, steps in (("persistent", tm.SCHED_PERSISTENT, 3), ("persistent_tuned", tm.SCHED_PERSISTENT, 3),
+                               ("plain", tm.SCHED_PLAIN, 1)):
         ring.configure(schedule=sched)
+        if name == "persistent_tuned":
+            # blocks per work item by partition-size bucket, tuned on these units
+            b0, b1, table = ring.autotune_buckets(up, uo)
+            res["bucket_table"] = [int(x) for x in table]
+            res["bucket_tune_device_ms"] = [b0, b1]
+            log(f"uneven bucket table {res['bucket_table']}: device {b0:.2f} -> {b1:.2f} ms")
         ring.submit_units(None, up, uo)
         ring.wait()
         tt = []
@@ -336,6 +343,8 @@ def uneven_leg(cw, cko, ck, args):
                      "match_ms": st.match_ms, "launches": st.n_items, "same_results": same}
         log(f"uneven {name}: {res[name]}")
     res["speedup_persistent_vs_plain"] = res["persistent"]["queries_per_s"] / res["plain"]["queries_per_s"]
+    res["speedup_tuned_vs_persistent"] = (res["persistent_tuned"]["queries_per_s"] /
+                                          res["persistent"]["queries_per_s"])
     ring.close()
     fib.close()
     return res

@@ -295,6 +295,27 @@ int tm_ring_config(tm_ring * ring, const tm_match_config * cfg);
 int tm_ring_autotune(tm_ring * ring, uint32_t nq, const tm_match_config * cands, uint32_t n_cands, uint32_t reps,
 					 uint32_t * best, double * times_ms);
 
+/* Work-item size by partition-size bucket, for explicit units
+ * (tm_submit_units, persistent schedule): a unit whose partition has nb
+ * blocks (of 64 filters) is cut into work items of seg[bucket(nb)] blocks,
+ * bucket(nb) = 0 below 16 blocks, 1 below 64, 2 below 256, 3 below 1024,
+ * 4 below 4096, 5 from 4096; 0 = the automatic size (the submission's
+ * tile-blocks spread over about four items per resident workgroup).  The
+ * reference runs one fixed launch shape per (partition, batch)
+ * (gpu.cu:878-912). */
+#define TM_SIZE_BUCKETS 6
+int tm_ring_set_bucket_segments(tm_ring * ring, const uint32_t * seg, uint32_t n);
+int tm_ring_get_bucket_segments(const tm_ring * ring, uint32_t * seg, uint32_t n);
+/* Tune that table on the caller's units (queries already in the ring):
+ * bucket by bucket, each of the n_cands candidate sizes is run (reps timed
+ * runs each after one warm-up) with the other buckets at their best so far,
+ * and the fastest by device time is kept.  *base_ms: the step with the
+ * table as it was; *best_ms: with the tuned table, which stays set.
+ * Results do not depend on the table. */
+int tm_ring_autotune_buckets(tm_ring * ring, const uint32_t * unit_part, const uint32_t * unit_off, uint32_t n_units,
+							 const uint32_t * cands, uint32_t n_cands, uint32_t reps, double * base_ms,
+							 double * best_ms);
+
 /* Device pointer of the ring's query buffer (max_queries x 8 words: 6 filter
  * words + 2 pad), for callers that keep queries resident in HBM. */
 int tm_ring_query_buffer(tm_ring * ring, uint32_t ** dev_words);
@@ -376,7 +397,7 @@ int tm_ring_lookup_counts(tm_ring * ring, uint32_t * counts, uint32_t n_parts);
  * writes their key bitmap rows to device memory (dev_rows: nq x
  * ceil(K/32) words), on `stream` (a hipStream_t; NULL: the default stream),
  * and returns when they are written.  It allocates nothing: its working
- * buffers live in a fixed static device array of 0.82 MiB per GPU,
+ * buffers live in a fixed static device array of 0.88 MiB per GPU,
  * which bounds nq by TM_STATIC_MAX_QUERIES and the FIB's partitions by
  * TM_STATIC_MAX_PARTS (larger sizes: TM_ERR_INVALID).  Calls on one GPU share
  * that array: calls that overlap on one GPU (from several threads) race.

@@ -29,6 +29,7 @@ MODE_HITS = 2
 MODE_KEY_BITMAP = 3
 STATIC_MAX_QUERIES = 65536
 STATIC_MAX_PARTS = 16384
+SIZE_BUCKETS = 6
 SCHED_PLAIN = 0
 SCHED_PERSISTENT = 1
 
@@ -165,6 +166,10 @@ _SIGS = {
                                C.POINTER(C.c_float)]),
     "tm_ring_autotune": (C.c_int, [P, C.c_uint32, C.POINTER(MatchConfig), C.c_uint32, C.c_uint32, U32P, P]),
     "tm_ring_unit_log": (C.c_int, [P, P, P, P, P, P, C.c_uint32, U32P]),
+    "tm_ring_set_bucket_segments": (C.c_int, [P, P, C.c_uint32]),
+    "tm_ring_get_bucket_segments": (C.c_int, [P, P, C.c_uint32]),
+    "tm_ring_autotune_buckets": (C.c_int, [P, P, P, C.c_uint32, P, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double)]),
     "tm_ring_set_cost_model": (C.c_int, [P, P, C.c_uint32]),
     "tm_server_config_default": (None, [C.POINTER(ServerConfig)]),
     "tm_server_create": (C.c_int, [P, C.POINTER(ServerConfig), C.POINTER(MatchConfig), C.POINTER(P)]),
@@ -614,6 +619,28 @@ class Ring:
               "tm_ring_autotune")
         self.cfg = cands[best.value]
         return best.value, times
+
+    def set_bucket_segments(self, seg):
+        a = np.ascontiguousarray(seg, np.uint32)
+        check(load().tm_ring_set_bucket_segments(self.h, _ptr(a), a.size), "tm_ring_set_bucket_segments")
+
+    def bucket_segments(self):
+        a = np.zeros(SIZE_BUCKETS, np.uint32)
+        check(load().tm_ring_get_bucket_segments(self.h, _ptr(a), a.size), "tm_ring_get_bucket_segments")
+        return a
+
+    def autotune_buckets(self, unit_part, unit_off, candidates=(4, 16, 64, 256, 1024, 1 << 20), reps=1):
+        """Tune blocks per work item by partition-size bucket on the resident
+        queries' units (tm_ring_autotune_buckets); returns (base_ms, best_ms,
+        table)."""
+        up = np.ascontiguousarray(unit_part, np.uint32)
+        uo = np.ascontiguousarray(unit_off, np.uint32)
+        c = np.ascontiguousarray(candidates, np.uint32)
+        b0, b1 = C.c_double(), C.c_double()
+        check(load().tm_ring_autotune_buckets(self.h, _ptr(up), _ptr(uo), len(up), _ptr(c), c.size, reps,
+                                              C.byref(b0), C.byref(b1)), "tm_ring_autotune_buckets")
+        self.nq = int(uo[-1])
+        return b0.value, b1.value, self.bucket_segments()
 
     def unit_log(self):
         """(features [n,16] f32, hits [n], keys [n], ticks [n], units [n,3]

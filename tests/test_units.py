@@ -80,3 +80,39 @@ def test_units_argument_errors(gpu):
         ring.submit_units(q, [0, 1], [0, 6, 5])         # not monotone
     ring.submit_units(q, [0, 1], [0, 4, 10])
     ring.wait()
+
+
+def test_bucket_segments_do_not_change_results(gpu):
+    """Work items cut by partition-size bucket (tm_ring_set_bucket_segments)
+    and the tuner over them: every table gives the same results."""
+    cw, ck, cv, _, _ = tm.workload(80000, 1, seed=811)
+    off = tm.zipf_runs(len(cw), 100, 40000, seed=7)        # 2..625 blocks: buckets 0-3
+    fib, _ = tm.fib_from_runs(cw, ck, cv, off)
+    up, uo = tm.uneven_units(len(off) - 1, 1, 900, seed=8)
+    nq = int(uo[-1])
+    q, _ = tm.Generator(seed=812).queries(nq, len(cw))
+    eo, ev = expected(cw, ck, cv, off, q, up, uo, 0)
+    ring = tm.Ring(fib, nq, mode=tm.MODE_KEYS_UNIQUE)
+    assert not ring.bucket_segments().any()
+    ring.submit_units(q, up, uo)
+    ring.wait()
+    o, v = ring.results()
+    assert np.array_equal(o, eo) and np.array_equal(v, ev)
+    for table in ([1] * 6, [1 << 20] * 6, [3, 7, 20, 100, 0, 5], [2]):
+        ring.set_bucket_segments(table)
+        assert list(ring.bucket_segments()) == (table + [0] * 6)[:6]
+        ring.submit_units(None, up, uo)
+        ring.wait()
+        o2, v2 = ring.results()
+        assert np.array_equal(o2, eo) and np.array_equal(v2, ev), table
+    ring.set_bucket_segments([])
+    base, best, table = ring.autotune_buckets(up, uo, candidates=(2, 8, 64), reps=1)
+    log(f"bucket table {list(table)}: {base:.3f} -> {best:.3f} ms")
+    assert best <= base and all(int(x) in (0, 2, 8, 64) for x in table)
+    ring.submit_units(None, up, uo)
+    ring.wait()
+    o2, v2 = ring.results()
+    assert np.array_equal(o2, eo) and np.array_equal(v2, ev)
+    with pytest.raises(tm.TagMatchError):
+        ring.set_bucket_segments([1] * 7)
+    ring.close()
