@@ -95,6 +95,13 @@ def parse():
                     help="steps timed with full sorted key lists copied to host memory (0: skip)")
     ap.add_argument("--seed", type=lambda x: int(x, 0), default=0x7461676D61746302)
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
+    ap.add_argument("--leg", choices=["headline", "c4", "c5"], default="headline",
+                    help="c4: the learned cost model at 50M filters (plain unit logs, MAPE, throughput against the "
+                         "plain form); c5: the serving loop at 200M filters; both print one JSON line of their own")
+    ap.add_argument("--c4-filters", type=int, default=50_000_000)
+    ap.add_argument("--c4-rows", type=int, default=2_000_000, help="plain unit log rows to train on")
+    ap.add_argument("--c4-log-seconds", type=float, default=420.0, help="time limit of the logging runs")
+    ap.add_argument("--c5-filters", type=int, default=200_000_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run an elementwise device copy of MIB MiB (read + write) three times, "
                          "to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE, and exit")
@@ -354,8 +361,10 @@ def serving_leg(fib, args, n_filters):
     """Poisson arrivals of fixed-size requests through tm_server at 30/70/95%
     of the sustained peak, with inserts interleaved."""
     req = args.serve_request
-    g = tm.Generator(seed=args.seed ^ 0x5E, superset_frac=0.2)
-    pool, _ = g.queries(1 << 17, n_filters)
+    # the table's generator (the seed also fixes each tag's bits), 20% of the
+    # queries supersets of a filter, drawn past the headline's queries
+    g = tm.Generator(seed=args.seed, superset_frac=0.2)
+    pool, _ = g.queries(1 << 17, n_filters, first=1 << 30)
     gi = tm.Generator(seed=args.seed)
     srv = tm.Server(fib, n_slots=args.serve_slots, slot_queries=args.serve_slot_queries,
                     step_graphs=args.serve_graphs)
@@ -453,6 +462,165 @@ def serving_leg(fib, args, n_filters):
             "requests_per_step": st["requests_per_batch_sum"] / max(st["n_batches"], 1)}
 
 
+def big_table(n_filters, args, n_queries):
+    """Generated table of n_filters, canonical and partitioned on the GPU."""
+    t0 = time.time()
+    g = tm.Generator(seed=args.seed)
+    w, ko, k = g.filters(n_filters)
+    q, _ = g.queries(n_queries, n_filters)
+    log(f"generated {n_filters} filters in {time.time() - t0:.1f}s")
+    cw, cko, ck = tm.canonicalize(w, ko, k)
+    del w, ko, k
+    log(f"canonical: {cw.shape[0]} distinct filters, {time.time() - t0:.1f}s")
+    fib, perm, off, masks = tm.build_fib(cw, cko, ck, args.maxp)
+    log(f"partitioned and uploaded: {len(off) - 1} partitions, {time.time() - t0:.1f}s")
+    return cw, cko, ck, q, fib, perm, off, masks, time.time() - t0
+
+
+def oracle_sample(cw, cko, ck, q, n):
+    """The oracle's sorted unique keys of the first n queries (all host threads)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    return orc.match(cw, cko, ck, q[:n], mode=0, threads=min(16, os.cpu_count() or 1))
+
+
+def c4_leg(args):
+    """Learned cost model at 50M filters: the MLP is trained on the plain
+    form's own (partition, 256-query batch) unit logs -- hits, key instances
+    and kernel ticks -- and reports held-out MAPE against a rate model.  It
+    then prices the persistent step (pre-sized result buffers, tile order);
+    throughput is compared with the plain form and with the unpriced step,
+    cold (a new ring's first step, where an unpriced ring retries on
+    overflow) and warm."""
+    nq = args.queries
+    cw, cko, ck, q, fib, perm, off, masks, setup_s = big_table(args.c4_filters, args, nq)
+    res = {"leg": "c4", "filters": args.c4_filters, "distinct_filters": int(cw.shape[0]),
+           "partitions": int(len(off) - 1), "queries": nq, "setup_s": setup_s}
+    # ---- plain unit logs, chunk by chunk, until the row target or the time limit
+    chunk = 16384
+    held_chunks = 2
+    ring = tm.Ring(fib, chunk, schedule=tm.SCHED_PLAIN, batch_size=256, log_units=1)
+    cols, held = [[], [], [], []], [[], [], [], []]
+    rows, t0, c = 0, time.time(), 0
+    plain_s, plain_q = 0.0, 0
+    while c * chunk < nq:
+        qa = q[c * chunk:(c + 1) * chunk]
+        ts = time.perf_counter()
+        ring.submit(qa)
+        ring.wait()
+        plain_s += time.perf_counter() - ts
+        plain_q += len(qa)
+        f, h, k, t, _ = ring.unit_log()
+        dest = held if c < held_chunks else cols
+        for d, x in zip(dest, (f, h, k, t)):
+            d.append(x)
+        if c >= held_chunks:
+            rows += len(f)
+        c += 1
+        log(f"c4 logs: chunk {c}, {rows} training rows, {time.time() - t0:.0f}s")
+        if rows >= args.c4_rows or time.time() - t0 > args.c4_log_seconds:
+            break
+    ring.close()
+    fa, ha, ka, ta = (np.concatenate(x) for x in cols)
+    fb, hb, kb, tb = (np.concatenate(x) for x in held)
+    res.update({"train_rows": int(len(fa)), "heldout_rows": int(len(fb)), "log_s": time.time() - t0,
+                "log_queries": c * chunk, "row_target": args.c4_rows,
+                "plain_queries_per_s": plain_q / plain_s})
+    t1 = time.time()
+    m = tm.MLP(seed=5)
+    epochs = int(max(3, min(150, 600_000 // max(len(fa), 1))))
+    loss = m.train(fa, tm.encode_targets(ha, ka, ta), epochs=epochs, batch=1024 if len(fa) > 100_000 else 256,
+                   lr=2e-3)
+    m.calibrate(fa, ha, ka, ta)
+    ph, pk, pt = tm.decode_targets(m.forward(fb))
+
+    def size(x):
+        return (np.exp2(x[:, 2] * 12) - 1) * (np.exp2(x[:, 0] * 20) - 1)
+
+    rate_h = ha.sum() / size(fa).sum()
+    rate_t = ta.sum() / size(fa).sum()
+    res.update({"epochs": epochs, "train_s": time.time() - t1, "train_loss": loss,
+                "mape_hits": tm.mape(ph, hb, floor=16), "mape_keys": tm.mape(pk, kb, floor=16),
+                "mape_ticks": tm.mape(pt, tb, floor=16),
+                "rate_model_mape_hits": tm.mape(rate_h * size(fb), hb, floor=16),
+                "rate_model_mape_ticks": tm.mape(rate_t * size(fb), tb, floor=16),
+                "total_hits_error": float(ph.sum() / max(hb.sum(), 1) - 1.0)})
+    log(f"c4 model: {res}")
+    del fa, ha, ka, ta
+    # ---- the persistent step over every query: cold and warm, with and without the model
+    out = {}
+    for name, model in (("unpriced", None), ("priced", m)):
+        # every hit of every query is a list entry in HITS mode (tens of
+        # thousands per query at this size): that step runs a prefix
+        for mode, mname, n in ((tm.MODE_KEYS_UNIQUE, "keys_unique", nq), (tm.MODE_HITS, "hits", min(nq, chunk))):
+            r = tm.Ring(fib, n, mode=mode)
+            if model is not None:
+                r.set_cost_model(model)
+            ts = time.perf_counter()
+            r.submit(q[:n])
+            st = r.wait()
+            cold = time.perf_counter() - ts
+            tt = []
+            for i in range(3):
+                ts = time.perf_counter()
+                r.submit_resident(n)
+                st2 = r.wait()
+                tt.append(time.perf_counter() - ts)
+            key = f"{name}_{mname}"
+            out[key] = {"queries": n, "cold_ms": 1e3 * cold, "cold_overflow_retry": int(st.overflow),
+                        "warm_ms": 1e3 * float(np.mean(tt)), "queries_per_s": n / float(np.mean(tt)),
+                        "pricing_ms": st2.model_ms, "pred_hits": st2.pred_hits, "hits": int(st2.n_hits)}
+            if mode == tm.MODE_KEYS_UNIQUE:
+                o, v = r.results()
+                if name == "unpriced":
+                    ref_o, ref_v = o, v
+                else:
+                    out[key]["same_results"] = bool(np.array_equal(o, ref_o) and np.array_equal(v, ref_v))
+                del o, v
+            log(f"c4 {key}: {out[key]}")
+            r.close()
+    res["steps"] = out
+    # the oracle on a sample of the queries
+    n_or = 64
+    eo, ev = oracle_sample(cw, cko, ck, q, n_or)
+    res["oracle_equal_sample"] = bool(np.array_equal(ref_o[:n_or + 1], eo) and
+                                      np.array_equal(ref_v[:int(eo[-1])], ev))
+    res["oracle_sample_queries"] = n_or
+    pq = res["plain_queries_per_s"]
+    res["speedup_priced_vs_plain"] = out["priced_keys_unique"]["queries_per_s"] / pq
+    res["speedup_priced_vs_unpriced"] = (out["priced_keys_unique"]["queries_per_s"] /
+                                         out["unpriced_keys_unique"]["queries_per_s"])
+    res["cold_speedup_priced_vs_unpriced_hits"] = out["unpriced_hits"]["cold_ms"] / out["priced_hits"]["cold_ms"]
+    m.close()
+    fib.close()
+    return res
+
+
+def c5_leg(args):
+    """The serving loop at 200M filters: burst and sustained peaks, then
+    Poisson arrivals at 30/70/95% of the sustained peak with 10k-filter
+    inserts every 100 ms; a request's results alone are checked against the
+    oracle on a sample first."""
+    cw, cko, ck, q, fib, perm, off, masks, setup_s = big_table(args.c5_filters, args, 4096)
+    res = {"leg": "c5", "filters": args.c5_filters, "distinct_filters": int(cw.shape[0]),
+           "partitions": int(len(off) - 1), "setup_s": setup_s, "device_bytes": int(fib.stats()["device_bytes"])}
+    n_or = 32
+    srv = tm.Server(fib, n_slots=args.serve_slots, slot_queries=args.serve_slot_queries,
+                    step_graphs=args.serve_graphs)
+    rid = srv.submit(q[:n_or])
+    srv.wait(rid)
+    _, o, v = srv.result(rid)
+    srv.close()
+    eo, ev = oracle_sample(cw, cko, ck, q, n_or)
+    res["oracle_equal_sample"] = bool(np.array_equal(o, eo) and np.array_equal(v, ev))
+    res["oracle_sample_queries"] = n_or
+    log(f"c5 oracle sample: {res['oracle_equal_sample']}")
+    del cw, cko, ck
+    res["serving"] = serving_leg(fib, args, args.c5_filters)
+    fib.close()
+    return res
+
+
 def copy_calibration(mib):
     """A kernel of known traffic: y = x + 1 over mib MiB of int32 (reads and
     writes mib MiB each), for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -514,6 +682,44 @@ def delivered_steps(fib, q, cfg_kw, steps, warmup, kw):
     return ms, match_ms, last, rings, bufs
 
 
+def packed_steps(fib, q, cfg_kw, steps, warmup, kw):
+    """The headline step: KEY_BITMAP rings whose submissions end with every
+    query's key set in pinned host memory in packed form
+    (tm_ring_set_host_rows_packed): 4 bytes per query, and a row only for
+    the queries whose key set is not the whole dictionary.  Two rings
+    alternate.  Returns (ms per step, match kernel ms per step, packed rows
+    per step, last ring's buffers, rings, buffers)."""
+    nq = q.shape[0]
+    rings, bufs = [], []
+    for _ in range(2):
+        r = tm.Ring(fib, nq, tm.config(mode=tm.MODE_KEY_BITMAP, **cfg_kw))
+        b = tm.PackedRows(nq, kw)
+        r.set_host_rows_packed(b)
+        r.submit(q)
+        r.wait()
+        rings.append(r)
+        bufs.append(b)
+
+    def run(k):
+        match_ms, packed = [], []
+        rings[0].submit_resident(nq)
+        for i in range(k):
+            if i + 1 < k:
+                rings[(i + 1) % 2].submit_resident(nq)
+            st = rings[i % 2].wait()
+            match_ms.append(st.match_ms)
+            packed.append(int(st.n_results))
+        return match_ms, packed
+
+    run(max(warmup, 1))
+    tm.check(tm.load().tm_dev_synchronize(), "sync")
+    t0 = time.perf_counter()
+    match_ms, packed = run(steps)
+    tm.check(tm.load().tm_dev_synchronize(), "sync")
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, match_ms, packed, bufs[(steps - 1) % 2], rings, bufs
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -525,6 +731,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     tm.init(0)
+    if args.leg != "headline":
+        res = c4_leg(args) if args.leg == "c4" else c5_leg(args)
+        line = json.dumps(res)
+        print(line, flush=True)
+        if args.out:
+            with open(args.out, "w") as f:
+                f.write(line + "\n")
+        return
     cw, cko, ck, q, fib, perm, part_off, masks, setup_s = setup(args)
     nq = q.shape[0]
     sched = tm.SCHED_PERSISTENT if args.schedule == "persistent" else tm.SCHED_PLAIN
@@ -571,20 +785,32 @@ def main():
     log(f"device-resident KEYS_UNIQUE step: {device_step_ms:.2f} ms (match kernel {st['match_ms']:.2f} ms, "
         f"{st['n_results']} keys)")
 
-    # ---- headline: every query's key set delivered to host memory
-    kwords = (int(fib.stats()["n_distinct_keys"]) + 31) // 32
-    ms_per_step, hl_match_ms, host_rows, hrings, hbufs = delivered_steps(fib, q, kw, args.steps, args.warmup, kwords)
+    # ---- headline: every query's key set delivered to host memory, packed
+    n_keys = int(fib.stats()["n_distinct_keys"])
+    kwords = (n_keys + 31) // 32
+    ms_per_step, hl_match_ms, packed_rows, last_packed, prings, pbufs = packed_steps(fib, q, kw, args.steps,
+                                                                                    args.warmup, kwords)
     qps = nq / (ms_per_step / 1e3)
     kernel_ms = float(np.mean(hl_match_ms))
-    log(f"host-delivered step: {ms_per_step:.2f} ms -> {qps / 1e6:.1f}M queries/s (match kernel {kernel_ms:.2f} ms)")
+    n_packed = packed_rows[-1]
+    log(f"host-delivered step (packed): {ms_per_step:.2f} ms -> {qps / 1e6:.1f}M queries/s (match kernel "
+        f"{kernel_ms:.2f} ms, {n_packed} rows not the whole dictionary)")
+    # every row copied to the host, for comparison
+    full_ms, full_match_ms, host_rows, hrings, hbufs = delivered_steps(fib, q, kw, args.steps, args.warmup, kwords)
+    full_rows = {"ms_per_step": full_ms, "queries_per_s": nq / (full_ms / 1e3),
+                 "match_ms": float(np.mean(full_match_ms)), "bytes_per_step": float(nq * kwords * 4),
+                 "note": "the same steps delivering every query's full key bitmap row (tm_ring_set_host_rows)"}
+    log(f"host-delivered step (every row): {full_ms:.2f} ms -> {nq / full_ms / 1e3:.1f}M queries/s")
 
     # ---- parity outside the timed region: the delivered rows equal the
     # KEYS_UNIQUE step's rows and the rows of the step that enumerates every
     # match; a sample of queries against the oracle
     parity = {}
     rows_ku = ring.key_bitmaps()
-    delivered = host_rows.rows(nq, kwords)
+    delivered = last_packed.unpack(nq, n_packed, n_keys)
     parity["rows_equal_keys_unique_step"] = bool(np.array_equal(delivered, rows_ku))
+    parity["full_rows_equal_keys_unique_step"] = bool(np.array_equal(host_rows.rows(nq, kwords), rows_ku))
+    del rows_ku
     ring.configure(tm.config(mode=tm.MODE_KEYS_UNIQUE, schedule=sched, full_exit=2, **kw))
     ring.submit_resident(nq)
     ring.wait()
@@ -621,27 +847,34 @@ def main():
     # ---- the same results as full sorted key lists in host memory, and the
     # rows expanded into key lists on the host
     hk = host_key_lists(ring, nq, args.host_key_steps)
-    t = time.time()
+    del delivered
     exp_off = np.zeros(nq + 1, np.uint64)
     lib = tm.load()
-    tm.check(lib.tm_expand_key_bitmaps(tm._ptr(delivered), nq, kwords, tm._ptr(dictionary), dictionary.size,
-                                       tm._ptr(exp_off), None, 0, 0), "expand")
+    row_of = last_packed.row_of.array[:nq]
+    prow = last_packed.rows.array[:max(n_packed, 1) * kwords]
+
+    def expand(values, threads):
+        return lib.tm_expand_key_rows_packed(tm._ptr(row_of), tm._ptr(prow), n_packed, nq, kwords,
+                                             tm._ptr(dictionary), dictionary.size, tm._ptr(exp_off),
+                                             tm._ptr(values) if values is not None else None,
+                                             values.size if values is not None else 0, threads)
+    tm.check(expand(None, 0), "expand")
     exp_vals = np.empty(int(exp_off[-1]), np.uint32)
     exp_vals[::1024] = 0       # touch the pages before timing
     threads = os.cpu_count() or 1
     tt = []
     for _ in range(2):
         t = time.perf_counter()
-        tm.check(lib.tm_expand_key_bitmaps(tm._ptr(delivered), nq, kwords, tm._ptr(dictionary), dictionary.size,
-                                           tm._ptr(exp_off), tm._ptr(exp_vals), exp_vals.size, threads), "expand")
+        tm.check(expand(exp_vals, threads), "expand")
         tt.append(time.perf_counter() - t)
     host_expand = {"queries_per_s": nq / min(tt), "seconds": min(tt), "threads": threads,
-                   "keys": int(exp_off[-1]), "keys_per_s": int(exp_off[-1]) / min(tt)}
+                   "keys": int(exp_off[-1]), "keys_per_s": int(exp_off[-1]) / min(tt),
+                   "note": "the packed delivered rows expanded into sorted key lists (tm_expand_key_rows_packed)"}
     log(f"host expansion of the rows: {host_expand}")
     del exp_vals
-    for r in hrings:
+    for r in hrings + prings:
         r.close()
-    for b in hbufs:
+    for b in hbufs + pbufs:
         b.close()
 
     # ---- roofline of the dominant kernel (k_match_q) and of the step
@@ -671,12 +904,14 @@ def main():
                     "every block of the table once (words, info, key slots).  bruteforce_bytes = what the reference's "
                     "(partition, 256-query batch) units would read; block masks, the rich-filter pass and the "
                     "full-key-set exit skip most of it exactly"}
-    step_bytes = float(nq * kwords * 4)
+    step_bytes = float(nq * 4 + n_packed * kwords * 4)
     step_roof = {"bound": "pcie", "achieved": step_bytes / (ms_per_step / 1e3) / 1e9, "peak": PCIE_PEAK_GBS,
                  "unit": "GB/s", "frac": step_bytes / (ms_per_step / 1e3) / 1e9 / PCIE_PEAK_GBS,
                  "bytes_per_step": step_bytes,
-                 "note": "the host-delivered step moves every query's key bitmap row (4 x ceil(K/32) bytes) to "
-                         "host memory; PCIe 5.0 x16 is 63 GB/s per direction"}
+                 "full_rows_frac": full_rows["bytes_per_step"] / (full_ms / 1e3) / 1e9 / PCIE_PEAK_GBS,
+                 "note": "the headline step moves 4 bytes per query and a key bitmap row (4 x ceil(K/32) bytes) per "
+                         "query whose key set is not the whole dictionary; full_rows_frac is the same for the form "
+                         "that copies every row.  PCIe 5.0 x16 is 63 GB/s per direction"}
     for name, rr in (("roofline", roof), ("step_roofline", step_roof)):
         assert rr["frac"] <= 1.0, f"{name}: fraction {rr['frac']:.3f} above 1"
     for name, v in (("alg_frac", roof["alg_frac"]), ("measured_valu_frac", roof["measured_valu_frac"])):
@@ -742,14 +977,18 @@ def main():
         "config": {
             "workload": f"{args.filters // 1_000_000}M filters (x 192-bit), {nq} queries, partitions <= {args.maxp}, "
                         f"batches of {args.batch} queries per partition; a step ends with every query's sorted "
-                        f"unique key set in host memory (key bitmap row over the sorted dictionary)",
+                        f"unique key set in host memory (packed key bitmap rows over the sorted dictionary)",
             "filters": args.filters, "distinct_filters": int(cw.shape[0]), "queries": nq,
             "partitions": int(len(part_off) - 1), "maxp": args.maxp, "batch": args.batch,
-            "schedule": args.schedule, "graph": bool(args.graph), "result_form": "key bitmap rows in host memory",
+            "schedule": args.schedule, "graph": bool(args.graph),
+            "result_form": "packed key bitmap rows in host memory",
         },
-        "value_note": "queries/s with every query's match_unique result (sorted unique keys) in host memory as a "
-                      f"{kwords * 4}-byte key bitmap row over the FIB's sorted dictionary; two rings alternate so "
-                      "that one step's copy overlaps the next step's match",
+        "value_note": "queries/s with every query's match_unique result (sorted unique keys) in host memory, packed "
+                      f"lossless: a 4-byte entry per query, TM_ROW_FULL when its key set is the whole dictionary, "
+                      f"else the index of its {kwords * 4}-byte key bitmap row over the FIB's sorted dictionary; two "
+                      "rings alternate so that one step's copies overlap the next step's match",
+        "packed_rows_per_step": n_packed,
+        "full_rows_delivered": full_rows,
         "device_step_ms": device_step_ms,
         "device_step_queries_per_s": nq / (device_step_ms / 1e3),
         "device_step_note": "KEYS_UNIQUE step whose sorted key lists stay in HBM",

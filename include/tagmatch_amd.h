@@ -253,7 +253,12 @@ typedef struct tm_match_config {
 	                            once its key set holds every key of the
 	                            table (results unchanged); 0 = default (on),
 	                            1 = on, 2 = off                              */
-	uint32_t reserved[5];
+	uint32_t compact_tiles;  /* PERSISTENT KEYS_UNIQUE / KEY_BITMAP: steps of
+	                            more query tiles than this match only the
+	                            queries the rich-filter pass leaves
+	                            incomplete, packed into tiles of their own;
+	                            0 = default (128)                            */
+	uint32_t reserved[4];
 } tm_match_config;
 
 void tm_match_config_default(tm_match_config * cfg);
@@ -365,6 +370,24 @@ int tm_read_key_bitmaps(tm_ring * ring, uint32_t * rows, uint64_t capacity_words
  * host_rows NULL stops the delivery. */
 int tm_ring_set_host_rows(tm_ring * ring, uint32_t * host_rows, uint64_t capacity_words);
 
+/* Packed host delivery for KEY_BITMAP rings: instead of every row, each
+ * later submission delivers row_of (one entry per query: TM_ROW_FULL when
+ * the query's key set is the whole dictionary, else the index of its row in
+ * `rows`) and, packed in query order, the rows of the queries whose key set
+ * is not the whole dictionary.  Lossless: a TM_ROW_FULL query's row is all
+ * K bits.  row_of and rows as for tm_ring_set_host_rows (pinned, valid until
+ * tm_wait returns; nq entries and nq * kw words of capacity are required).
+ * The entries and part of the rows are copied on the ring's stream; rows
+ * beyond that are copied by tm_wait, so they are complete when tm_wait
+ * returns (tm_poll reporting completion is not enough).  tm_status.n_results
+ * is the number of packed rows.  Setting either host delivery form replaces
+ * the other; row_of NULL stops it.  The reference delivers each query's keys
+ * through its match handler (back_end.cc:288-301, tagmatch_query.hh:99-119);
+ * this form moves 4 bytes per query whose keys are every key. */
+#define TM_ROW_FULL 0xFFFFFFFFu
+int tm_ring_set_host_rows_packed(tm_ring * ring, uint32_t * row_of, uint64_t row_of_capacity, uint32_t * rows,
+								 uint64_t rows_capacity_words);
+
 /* Pinned (page-locked) host memory for result delivery, and its release. */
 int tm_host_alloc(void ** ptr, uint64_t bytes);
 int tm_host_free(void * ptr);
@@ -382,6 +405,12 @@ int tm_fib_key_dictionary(const tm_fib * fib, uint32_t * keys, uint64_t capacity
  * (tagmatch_query.hh:80-119) for a whole submission at once. */
 int tm_expand_key_bitmaps(const uint32_t * rows, uint64_t nq, uint32_t kw, const uint32_t * dict, uint32_t n_keys,
 						  uint64_t * offsets, uint32_t * values, uint64_t capacity, uint32_t threads);
+/* The same from packed delivery (tm_ring_set_host_rows_packed): row_of[q]
+ * TM_ROW_FULL gives the whole dictionary, else rows[row_of[q]] (n_rows rows
+ * of kw words); an index at or above n_rows is TM_ERR_INVALID. */
+int tm_expand_key_rows_packed(const uint32_t * row_of, const uint32_t * rows, uint64_t n_rows, uint64_t nq,
+							  uint32_t kw, const uint32_t * dict, uint32_t n_keys, uint64_t * offsets,
+							  uint32_t * values, uint64_t capacity, uint32_t threads);
 
 /* Per-partition query counts of the finished submission (the on-device
  * lookup's result; the reference's per-partition queue fill,

@@ -30,6 +30,7 @@ MODE_KEY_BITMAP = 3
 STATIC_MAX_QUERIES = 65536
 STATIC_MAX_PARTS = 16384
 SIZE_BUCKETS = 6
+ROW_FULL = 0xFFFFFFFF
 SCHED_PLAIN = 0
 SCHED_PERSISTENT = 1
 
@@ -60,7 +61,7 @@ class MatchConfig(C.Structure):
         ("chunk_blocks", C.c_uint32), ("waves_per_group", C.c_uint32),
         ("groups_per_cu", C.c_uint32), ("hit_capacity", C.c_uint64), ("use_graph", C.c_uint32),
         ("log_units", C.c_uint32), ("order_by_cost", C.c_uint32), ("variant", C.c_uint32),
-        ("full_exit", C.c_uint32), ("reserved", C.c_uint32 * 5),
+        ("full_exit", C.c_uint32), ("compact_tiles", C.c_uint32), ("reserved", C.c_uint32 * 4),
     ]
 
 
@@ -144,6 +145,7 @@ _SIGS = {
     "tm_ring_results_device": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
     "tm_read_key_bitmaps": (C.c_int, [P, P, C.c_uint64, U32P]),
     "tm_ring_set_host_rows": (C.c_int, [P, P, C.c_uint64]),
+    "tm_ring_set_host_rows_packed": (C.c_int, [P, P, C.c_uint64, P, C.c_uint64]),
     "tm_match_rows_static": (C.c_int, [P, P, C.c_uint32, P, P]),
     "tm_host_alloc": (C.c_int, [C.POINTER(P), C.c_uint64]),
     "tm_dev_alloc": (C.c_int, [C.POINTER(P), C.c_uint64]),
@@ -153,6 +155,8 @@ _SIGS = {
     "tm_host_free": (C.c_int, [P]),
     "tm_fib_key_dictionary": (C.c_int, [P, P, C.c_uint64, U64P]),
     "tm_expand_key_bitmaps": (C.c_int, [P, C.c_uint64, C.c_uint32, P, C.c_uint32, P, P, C.c_uint64, C.c_uint32]),
+    "tm_expand_key_rows_packed": (C.c_int, [P, P, C.c_uint64, C.c_uint64, C.c_uint32, P, C.c_uint32, P, P,
+                                            C.c_uint64, C.c_uint32]),
     "tm_ring_lookup_counts": (C.c_int, [P, P, C.c_uint32]),
     "tm_match": (C.c_int, [P, P, C.c_uint32, C.POINTER(MatchConfig), P, P, C.c_uint64, C.POINTER(Status)]),
     "tm_mlp_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(P)]),
@@ -602,6 +606,18 @@ class Ring:
         else:
             check(load().tm_ring_set_host_rows(self.h, C.c_void_p(buf.ptr), buf.words), "tm_ring_set_host_rows")
 
+    def set_host_rows_packed(self, buf):
+        """Deliver each KEY_BITMAP submission packed into `buf` (a
+        PackedRows or None to stop): per query TM_ROW_FULL or its row's
+        index, and the rows that are not the whole dictionary."""
+        self.host_rows = buf
+        if buf is None:
+            check(load().tm_ring_set_host_rows_packed(self.h, None, 0, None, 0), "tm_ring_set_host_rows_packed")
+        else:
+            check(load().tm_ring_set_host_rows_packed(self.h, C.c_void_p(buf.row_of.ptr), buf.row_of.words,
+                                                      C.c_void_p(buf.rows.ptr), buf.rows.words),
+                  "tm_ring_set_host_rows_packed")
+
     def lookup_counts(self):
         n = int(self.fib.stats()["n_parts"])
         c = np.zeros(max(n, 1), np.uint32)
@@ -738,6 +754,52 @@ class PinnedRows:
             self.close()
         except Exception:
             pass
+
+
+class PackedRows:
+    """Pinned host buffers of packed KEY_BITMAP delivery for up to nq
+    queries of kw-word rows."""
+
+    def __init__(self, nq, kw):
+        self.kw = int(kw)
+        self.row_of = PinnedRows(nq)
+        self.rows = PinnedRows(int(nq) * self.kw)
+
+    def unpack(self, nq, n_rows, n_keys):
+        """The full (nq, kw) rows of a delivered submission."""
+        row_of = self.row_of.array[:nq]
+        packed = self.rows.array[:n_rows * self.kw].reshape(n_rows, self.kw)
+        full = np.zeros(self.kw, np.uint32)
+        full[:n_keys // 32] = 0xFFFFFFFF
+        if n_keys % 32:
+            full[n_keys // 32] = (1 << (n_keys % 32)) - 1
+        out = np.empty((nq, self.kw), np.uint32)
+        is_full = row_of == ROW_FULL
+        out[is_full] = full
+        out[~is_full] = packed[row_of[~is_full]]
+        return out
+
+    def close(self):
+        self.row_of.close()
+        self.rows.close()
+
+
+def expand_key_rows_packed(row_of, rows, dictionary, threads=0):
+    """Packed delivery (row_of [nq], rows [n_rows, kw]) -> KEYS_UNIQUE
+    (offsets, values), expanded on the host (tm_expand_key_rows_packed)."""
+    row_of = np.ascontiguousarray(row_of, np.uint32)
+    rows = np.ascontiguousarray(rows, np.uint32)
+    n_rows, kw = rows.shape
+    nq = row_of.size
+    d = np.ascontiguousarray(dictionary, np.uint32)
+    off = np.zeros(nq + 1, np.uint64)
+    lib = load()
+    check(lib.tm_expand_key_rows_packed(_ptr(row_of), _ptr(rows), n_rows, nq, kw, _ptr(d), d.size, _ptr(off), None,
+                                        0, threads), "tm_expand_key_rows_packed")
+    vals = np.zeros(int(off[-1]), np.uint32)
+    check(lib.tm_expand_key_rows_packed(_ptr(row_of), _ptr(rows), n_rows, nq, kw, _ptr(d), d.size, _ptr(off),
+                                        _ptr(vals), vals.size, threads), "tm_expand_key_rows_packed")
+    return off, vals
 
 
 def expand_key_bitmaps(rows, dictionary, threads=0):

@@ -54,19 +54,20 @@ void parallel_ranges(uint64_t n, uint32_t threads, F fn) {
 	for (auto & x : th) x.join();
 }
 
-} // namespace
-
-extern "C" int tm_expand_key_bitmaps(const uint32_t * rows, uint64_t nq, uint32_t kw, const uint32_t * dict,
-									 uint32_t n_keys, uint64_t * offsets, uint32_t * values, uint64_t capacity,
-									 uint32_t threads) {
-	if (!offsets) return tmx::set_error(TM_ERR_INVALID, "offsets is NULL");
-	if (nq && (!rows || (n_keys && !dict))) return tmx::set_error(TM_ERR_INVALID, "rows or dictionary is NULL");
+// Offsets, then values, of nq queries whose rows row_at(q) gives (NULL: the
+// whole dictionary).
+template <typename R>
+int expand(R row_at, uint64_t nq, uint32_t kw, const uint32_t * dict, uint32_t n_keys, uint64_t * offsets,
+		   uint32_t * values, uint64_t capacity, uint32_t threads) {
 	const uint32_t kw_used = std::min<uint32_t>(kw, (n_keys + 31) / 32);
 	const uint32_t last_mask = (n_keys & 31) ? (1u << (n_keys & 31)) - 1u : 0xFFFFFFFFu;
 	if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
 	offsets[0] = 0;
 	parallel_ranges(nq, threads, [&](uint64_t a, uint64_t b) {
-		for (uint64_t q = a; q < b; ++q) offsets[q + 1] = row_count(rows + q * kw, kw_used, last_mask);
+		for (uint64_t q = a; q < b; ++q) {
+			const uint32_t * row = row_at(q);
+			offsets[q + 1] = row ? row_count(row, kw_used, last_mask) : n_keys;
+		}
 	});
 	for (uint64_t q = 0; q < nq; ++q) offsets[q + 1] += offsets[q];
 	if (!values) return TM_OK;
@@ -74,7 +75,38 @@ extern "C" int tm_expand_key_bitmaps(const uint32_t * rows, uint64_t nq, uint32_
 		return tmx::set_error(TM_ERR_OVERFLOW, "%llu keys, capacity %llu", (unsigned long long)offsets[nq],
 							  (unsigned long long)capacity);
 	parallel_ranges(nq, threads, [&](uint64_t a, uint64_t b) {
-		for (uint64_t q = a; q < b; ++q) row_write(rows + q * kw, kw_used, last_mask, dict, values + offsets[q]);
+		for (uint64_t q = a; q < b; ++q) {
+			const uint32_t * row = row_at(q);
+			if (row)
+				row_write(row, kw_used, last_mask, dict, values + offsets[q]);
+			else if (n_keys)
+				std::memcpy(values + offsets[q], dict, (size_t)n_keys * 4);
+		}
 	});
 	return TM_OK;
+}
+
+} // namespace
+
+extern "C" int tm_expand_key_rows_packed(const uint32_t * row_of, const uint32_t * rows, uint64_t n_rows, uint64_t nq,
+										 uint32_t kw, const uint32_t * dict, uint32_t n_keys, uint64_t * offsets,
+										 uint32_t * values, uint64_t capacity, uint32_t threads) {
+	if (!offsets) return tmx::set_error(TM_ERR_INVALID, "offsets is NULL");
+	if (nq && (!row_of || (n_keys && !dict))) return tmx::set_error(TM_ERR_INVALID, "row_of or dictionary is NULL");
+	if (n_rows && !rows) return tmx::set_error(TM_ERR_INVALID, "rows is NULL");
+	for (uint64_t q = 0; q < nq; ++q)
+		if (row_of[q] != TM_ROW_FULL && row_of[q] >= n_rows)
+			return tmx::set_error(TM_ERR_INVALID, "query %llu: row %u of %llu", (unsigned long long)q, row_of[q],
+								  (unsigned long long)n_rows);
+	return expand([&](uint64_t q) { return row_of[q] == TM_ROW_FULL ? nullptr : rows + (uint64_t)row_of[q] * kw; }, nq,
+				  kw, dict, n_keys, offsets, values, capacity, threads);
+}
+
+extern "C" int tm_expand_key_bitmaps(const uint32_t * rows, uint64_t nq, uint32_t kw, const uint32_t * dict,
+									 uint32_t n_keys, uint64_t * offsets, uint32_t * values, uint64_t capacity,
+									 uint32_t threads) {
+	if (!offsets) return tmx::set_error(TM_ERR_INVALID, "offsets is NULL");
+	if (nq && (!rows || (n_keys && !dict))) return tmx::set_error(TM_ERR_INVALID, "rows or dictionary is NULL");
+	return expand([&](uint64_t q) { return rows + q * kw; }, nq, kw, dict, n_keys, offsets, values, capacity,
+				  threads);
 }

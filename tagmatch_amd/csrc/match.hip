@@ -928,6 +928,35 @@ k_bm_count(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t
 	if (lane == 0) qcount[qi] = c;
 }
 
+// KEY_BITMAP, packed delivery: one wave per query flags a row that is not
+// the whole dictionary (bits 0..K-1 set, last_mask in the last word).
+__global__ void __launch_bounds__(256)
+k_row_partial(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t last_mask,
+			  uint32_t * __restrict__ flag) {
+	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+	if (qi >= nq) return;
+	bool partial = false;
+	for (uint32_t w = lane; w < kw; w += 64)
+		partial |= gbm[(size_t)qi * kw + w] != (w + 1 == kw ? last_mask : 0xFFFFFFFFu);
+	const uint64_t any = __ballot(partial);
+	if (lane == 0) flag[qi] = any != 0;
+}
+
+// KEY_BITMAP, packed delivery: each query's entry (TM_ROW_FULL or the index
+// of its row among the flagged ones, in query order) and the flagged rows,
+// packed.  One wave per query.
+__global__ void __launch_bounds__(256)
+k_row_pack(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint32_t * __restrict__ flag,
+		   const uint64_t * __restrict__ base, uint32_t * __restrict__ row_of, uint32_t * __restrict__ prow) {
+	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+	if (qi >= nq) return;
+	const bool partial = flag[qi] != 0;
+	const uint64_t b = base[qi];
+	if (lane == 0) row_of[qi] = partial ? (uint32_t)b : TM_ROW_FULL;
+	if (!partial) return;
+	for (uint32_t w = lane; w < kw; w += 64) prow[b * kw + w] = gbm[(size_t)qi * kw + w];
+}
+
 // KEYS_UNIQUE: one wave per query writes its keys in ascending id order,
 // which is ascending key order (the dictionary is sorted).  Lanes take 64
 // consecutive key ids at a time: a set bit's output slot is the count of set
@@ -1431,7 +1460,10 @@ struct tm_ring {
 		hipGraphExec_t exec = nullptr;
 		uint32_t nq = 0, epoch = 0;
 		const uint32_t * slot_epoch = nullptr;
+		bool weighted = false;       // reads the block weights, refreshed outside the graph
 	};
+	bool capturing = false;          // enqueue_persistent runs inside a stream capture
+	bool step_weighted = false;      // the last enqueue_persistent cut weighted segments
 	std::vector<graph_entry> graphs;
 
 	uint32_t last_nq = 0;
@@ -1453,6 +1485,19 @@ struct tm_ring {
 	uint32_t * h_step = nullptr;      // pinned source of d_step
 	uint32_t * host_rows = nullptr;   // KEY_BITMAP: rows delivered here at the end of each step
 	uint64_t host_rows_cap = 0;       // words
+	// KEY_BITMAP, packed delivery (tm_ring_set_host_rows_packed)
+	uint32_t * host_row_of = nullptr; // per query: TM_ROW_FULL or its row's index in host_prow
+	uint32_t * host_prow = nullptr;   // the rows that are not the whole dictionary, packed
+	uint64_t host_row_of_cap = 0, host_prow_cap = 0;   // entries, words
+	uint32_t * d_rflag = nullptr;     // per query: row is not full
+	uint64_t * d_rbase = nullptr;     // exclusive scan of d_rflag (nq + 1)
+	uint64_t * d_rscan = nullptr;     // scan partials
+	uint32_t * d_row_of = nullptr;
+	uint32_t * d_prow = nullptr;
+	uint64_t pack_q_cap = 0;          // queries the packing buffers hold
+	uint64_t * h_npack = nullptr;     // pinned: packed rows of the last step
+	uint64_t pack_sent = 0;           // rows copied with the step (the rest in tm_wait)
+	uint64_t pack_guess = 0;          // rows to copy with the next step
 	bool pending = false;
 	bool match_events = false;
 	int retries = 0;
@@ -1467,6 +1512,10 @@ void tm_ring::release() {
 	if (h_q) (void)hipHostFree(h_q);
 	if (h_stat) (void)hipHostFree(h_stat);
 	if (h_step) (void)hipHostFree(h_step);
+	if (h_npack) (void)hipHostFree(h_npack);
+	h_npack = nullptr;
+	dfree(d_rflag); dfree(d_rbase); dfree(d_rscan); dfree(d_row_of); dfree(d_prow);
+	pack_q_cap = 0;
 	h_q = nullptr;
 	h_stat = nullptr;
 	h_step = nullptr;
@@ -1565,6 +1614,20 @@ int step_waves(const tm_ring * r, uint64_t tiles, bool compacted) {
 }
 int wgs_for_waves(const tm_ring * r, int nw) { return r->wgs_for[nw == 4 ? 0 : nw == 8 ? 1 : 2]; }
 
+// Block weights for equal-work segments (k_block_weights + prefix sum), kept
+// while the FIB and the submissions' bit density, rounded to a quarter of a
+// binary order, stay the same: they only balance the segments.
+void ensure_weights(tm_ring * r, hipStream_t s) {
+	const uint32_t nblk = r->fib->n_blocks;
+	const float l2 = std::round(r->rho_l2 * 4.f) / 4.f;
+	if (r->bw_version == r->fib->version && r->bw_nb == nblk && r->bw_l2 == l2) return;
+	hipLaunchKernelGGL(k_block_weights, dim3((nblk + 3) / 4), dim3(256), 0, s, r->fib->view(), nblk, l2, r->d_bw);
+	scan_u32_to_u64(r->d_bw, r->d_bprefix, nblk, r->d_bscan, s);
+	r->bw_version = r->fib->version;
+	r->bw_nb = nblk;
+	r->bw_l2 = l2;
+}
+
 int grow_u32(uint32_t ** p, uint64_t & cap, uint64_t need) {
 	if (need <= cap && *p) return TM_OK;
 	dfree(*p);
@@ -1640,7 +1703,8 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	// Compacted step: only the queries whose rows the pass leaves incomplete
 	// are matched (large steps only: the extra kernels cost about 0.1 ms,
 	// more than a step of up to ~8k queries gains)
-	const bool compact = rich_pass && tiles > COMPACT_MIN_TILES && !r->use_iorder && !r->cfg.log_units &&
+	const uint64_t compact_min = r->cfg.compact_tiles ? r->cfg.compact_tiles : COMPACT_MIN_TILES;
+	const bool compact = rich_pass && tiles > compact_min && !r->use_iorder && !r->cfg.log_units &&
 						 (r->variant & 1) == 0;
 	const int nw = step_waves(r, tiles, compact);
 	// Small submissions: cut each tile's work into block segments so that
@@ -1670,16 +1734,11 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	// queries; uniform segments put them in a few long work items, which
 	// then set the step's time.
 	const bool weighted = nseg > 1 && tiles <= WSEG_MAX_TILES && !r->cfg.chunk_blocks && !r->no_weights;
+	r->step_weighted = weighted;
 	if (weighted) {
-		const float l2 = std::round(r->rho_l2 * 16.f) / 16.f;
-		if (r->bw_version != r->fib->version || r->bw_nb != nblk || r->bw_l2 != l2) {
-			hipLaunchKernelGGL(k_block_weights, dim3((nblk + 3) / 4), dim3(256), 0, s, r->fib->view(), nblk, l2,
-							   r->d_bw);
-			scan_u32_to_u64(r->d_bw, r->d_bprefix, nblk, r->d_bscan, s);
-			r->bw_version = r->fib->version;
-			r->bw_nb = nblk;
-			r->bw_l2 = l2;
-		}
+		// a captured step leaves the weights out: the caller refreshes them
+		// before each replay
+		if (!r->capturing) ensure_weights(r, s);
 		hipLaunchKernelGGL(k_seg_bounds, dim3((nseg + 256) / 256), dim3(256), 0, s, r->d_bprefix, nblk, nseg,
 						   r->d_segb);
 	}
@@ -2128,8 +2187,45 @@ int enqueue_units(tm_ring * r, uint32_t nq) {
 // KEY_BITMAP with host delivery: the step's rows into the caller's host
 // buffer, on the ring's stream after the step (outside any captured graph).
 int enqueue_delivery(tm_ring * r, uint32_t nq) {
-	if (r->cfg.mode != TM_MODE_KEY_BITMAP || !r->host_rows || nq == 0 || r->kw == 0) return TM_OK;
-	TM_HIP(hipMemcpyAsync(r->host_rows, r->d_gbm, (size_t)nq * r->kw * 4, hipMemcpyDeviceToHost, r->s));
+	r->pack_sent = 0;
+	if (r->cfg.mode != TM_MODE_KEY_BITMAP || nq == 0 || r->kw == 0) return TM_OK;
+	if (r->host_rows) {
+		TM_HIP(hipMemcpyAsync(r->host_rows, r->d_gbm, (size_t)nq * r->kw * 4, hipMemcpyDeviceToHost, r->s));
+		return TM_OK;
+	}
+	if (!r->host_row_of) return TM_OK;
+	// packed: flag the rows that are not the whole dictionary, number them,
+	// pack them; the entries and as many rows as the last step had go to the
+	// host now, any further rows in tm_wait once the count is known
+	hipStream_t s = r->s;
+	const uint32_t kw = r->kw, bits = r->fib->n_keys % 32;
+	const uint32_t last_mask = bits ? (1u << bits) - 1u : 0xFFFFFFFFu;
+	const uint32_t g = (nq + 3) / 4;
+	hipLaunchKernelGGL(k_row_partial, dim3(g), dim3(256), 0, s, r->d_gbm, kw, nq, last_mask, r->d_rflag);
+	scan_u32_to_u64(r->d_rflag, r->d_rbase, nq, r->d_rscan, s);
+	hipLaunchKernelGGL(k_row_pack, dim3(g), dim3(256), 0, s, r->d_gbm, kw, nq, r->d_rflag, r->d_rbase, r->d_row_of,
+					   r->d_prow);
+	TM_HIP(hipGetLastError());
+	TM_HIP(hipMemcpyAsync(r->host_row_of, r->d_row_of, (size_t)nq * 4, hipMemcpyDeviceToHost, s));
+	TM_HIP(hipMemcpyAsync(r->h_npack, r->d_rbase + nq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+	r->pack_sent = std::min<uint64_t>(r->pack_guess, nq);
+	if (r->pack_sent)
+		TM_HIP(hipMemcpyAsync(r->host_prow, r->d_prow, r->pack_sent * kw * 4, hipMemcpyDeviceToHost, s));
+	return TM_OK;
+}
+
+// The packing buffers for nq queries.
+int ensure_pack(tm_ring * r, uint64_t nq) {
+	if (!r->h_npack) TM_HIP(hipHostMalloc((void **)&r->h_npack, sizeof(uint64_t)));
+	if (nq <= r->pack_q_cap) return TM_OK;
+	dfree(r->d_rflag); dfree(r->d_rbase); dfree(r->d_rscan); dfree(r->d_row_of); dfree(r->d_prow);
+	r->pack_q_cap = 0;
+	TM_CHECK(dalloc(&r->d_rflag, nq));
+	TM_CHECK(dalloc(&r->d_rbase, nq + 1));
+	TM_CHECK(dalloc(&r->d_rscan, scan_tmp_words(nq)));
+	TM_CHECK(dalloc(&r->d_row_of, nq));
+	TM_CHECK(dalloc(&r->d_prow, std::max<uint64_t>(nq * r->kw, 1)));
+	r->pack_q_cap = nq;
 	return TM_OK;
 }
 
@@ -2172,7 +2268,9 @@ int enqueue_step(tm_ring * r, bool allow_graph) {
 			tm_ring::graph_entry e;
 			hipGraph_t g = nullptr;
 			TM_HIP(hipStreamBeginCapture(r->s, hipStreamCaptureModeThreadLocal));
+			r->capturing = true;
 			int rc = enqueue_persistent(r, nq, false);
+			r->capturing = false;
 			hipError_t ce = hipStreamEndCapture(r->s, &g);
 			if (rc != TM_OK) {
 				if (g) (void)hipGraphDestroy(g);
@@ -2185,9 +2283,11 @@ int enqueue_step(tm_ring * r, bool allow_graph) {
 			e.nq = nq;
 			e.epoch = r->epoch;
 			e.slot_epoch = r->fib->slot_epoch;
+			e.weighted = r->step_weighted;
 			r->graphs.push_back(e);
 			ge = &r->graphs.back();
 		}
+		if (ge->weighted) ensure_weights(r, r->s);
 		TM_HIP(hipGraphLaunch(ge->exec, r->s));
 		r->match_events = false;
 	} else if (persistent) {
@@ -2381,6 +2481,13 @@ int submit_common(tm_ring * r, uint32_t nq) {
 	if (r->cfg.mode == TM_MODE_KEY_BITMAP && r->host_rows && (uint64_t)nq * r->kw > r->host_rows_cap)
 		return set_error(TM_ERR_OVERFLOW, "host rows hold %llu words, the submission needs %llu",
 						 (unsigned long long)r->host_rows_cap, (unsigned long long)nq * r->kw);
+	if (r->cfg.mode == TM_MODE_KEY_BITMAP && r->host_row_of) {
+		if (nq > r->host_row_of_cap || (uint64_t)nq * r->kw > r->host_prow_cap)
+			return set_error(TM_ERR_OVERFLOW, "packed host rows hold %llu entries and %llu words, the submission "
+							 "needs %u and up to %llu", (unsigned long long)r->host_row_of_cap,
+							 (unsigned long long)r->host_prow_cap, nq, (unsigned long long)nq * r->kw);
+		TM_CHECK(ensure_pack(r, std::max<uint64_t>(nq, 1)));
+	}
 	if (!r->units_mode) TM_CHECK(price_submission(r, r->run_nq));
 	int rc = enqueue_step(r, true);
 	if (rc != TM_OK) return rc;
@@ -2588,6 +2695,20 @@ int tm_wait(tm_ring * r, tm_status * st) {
 			r->graphs.clear();
 			TM_CHECK(enqueue_step(r, false));
 		}
+		if (r->cfg.mode == TM_MODE_KEY_BITMAP && r->host_row_of && r->last_nq && r->kw) {
+			// packed rows beyond those copied with the step
+			const uint64_t n = *r->h_npack;
+			if (n > r->pack_sent) {
+				const hipError_t e = hipMemcpy(r->host_prow + r->pack_sent * r->kw, r->d_prow + r->pack_sent * r->kw,
+											   (n - r->pack_sent) * r->kw * 4, hipMemcpyDeviceToHost);
+				if (e != hipSuccess) {
+					end_pending(r);
+					return hip_error(e, "packed rows", __FILE__, __LINE__);
+				}
+			}
+			// the next step copies this many rows with itself (with room to grow)
+			r->pack_guess = n + n / 8 + 64;
+		}
 		end_pending(r);
 		float dms = 0.f, mms = 0.f;
 		(void)hipEventElapsedTime(&dms, r->e0, r->e1);
@@ -2605,6 +2726,7 @@ int tm_wait(tm_ring * r, tm_status * st) {
 		r->st.n_block_visits = r->h_stat[5];
 		r->st.n_cand_tests = r->h_stat[6];
 		r->st.n_hit_events = r->h_stat[7];
+		if (r->cfg.mode == TM_MODE_KEY_BITMAP && r->host_row_of) r->st.n_results = r->last_nq ? *r->h_npack : 0;
 	}
 	if (st) *st = r->st;
 	return TM_OK;
@@ -2654,6 +2776,29 @@ int tm_ring_set_host_rows(tm_ring * r, uint32_t * host_rows, uint64_t capacity_w
 		return set_error(TM_ERR_STATE, "host rows are delivered in KEY_BITMAP mode only");
 	r->host_rows = host_rows;
 	r->host_rows_cap = host_rows ? capacity_words : 0;
+	if (host_rows) {   // one delivery form at a time
+		r->host_row_of = r->host_prow = nullptr;
+		r->host_row_of_cap = r->host_prow_cap = 0;
+	}
+	return TM_OK;
+}
+
+int tm_ring_set_host_rows_packed(tm_ring * r, uint32_t * row_of, uint64_t row_of_capacity, uint32_t * rows,
+								 uint64_t rows_capacity_words) {
+	if (!r) return set_error(TM_ERR_INVALID, "ring is NULL");
+	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
+	if (row_of && r->cfg.mode != TM_MODE_KEY_BITMAP)
+		return set_error(TM_ERR_STATE, "host rows are delivered in KEY_BITMAP mode only");
+	if (row_of && !rows) return set_error(TM_ERR_INVALID, "rows is NULL");
+	r->host_row_of = row_of;
+	r->host_prow = row_of ? rows : nullptr;
+	r->host_row_of_cap = row_of ? row_of_capacity : 0;
+	r->host_prow_cap = row_of ? rows_capacity_words : 0;
+	r->pack_guess = 0;
+	if (row_of) {   // one delivery form at a time
+		r->host_rows = nullptr;
+		r->host_rows_cap = 0;
+	}
 	return TM_OK;
 }
 

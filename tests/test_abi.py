@@ -140,3 +140,36 @@ def test_expand_key_bitmaps_host(n_keys):
     rc = lib.tm_expand_key_bitmaps(tm._ptr(rows), nq, kw, tm._ptr(dictionary), n_keys, tm._ptr(o2), tm._ptr(small),
                                    small.size, 2)
     assert rc == tm.TM_ERR_OVERFLOW and np.array_equal(o2, off)
+
+
+@pytest.mark.parametrize("n_keys", [1, 33, 2048])
+def test_expand_key_rows_packed_host(n_keys):
+    """tm_expand_key_rows_packed (host code, no GPU): TM_ROW_FULL entries
+    give the whole dictionary, the others their packed row; equal to the
+    expansion of the unpacked rows; an index past the packed rows is
+    rejected."""
+    rng = np.random.default_rng(100 + n_keys)
+    kw = (n_keys + 31) // 32
+    nq = 2000
+    full = rng.random(nq) < 0.6
+    n_rows = int((~full).sum())
+    packed = rng.integers(0, 2**32, size=(n_rows, kw), dtype=np.uint64).astype(np.uint32)
+    if n_keys % 32:
+        packed[:, -1] &= np.uint32((1 << (n_keys % 32)) - 1)
+    row_of = np.full(nq, tm.ROW_FULL, np.uint32)
+    row_of[~full] = np.arange(n_rows, dtype=np.uint32)
+    rows = np.zeros((nq, kw), np.uint32)
+    rows[full] = 0xFFFFFFFF
+    if n_keys % 32:
+        rows[full, -1] = (1 << (n_keys % 32)) - 1
+    rows[~full] = packed
+    dictionary = np.sort(rng.choice(2**31, n_keys, replace=False)).astype(np.uint32)
+    eo, ev = tm.expand_key_bitmaps(rows, dictionary)
+    for threads in (1, 0):
+        off, vals = tm.expand_key_rows_packed(row_of, packed, dictionary, threads=threads)
+        assert np.array_equal(off, eo) and np.array_equal(vals, ev)
+    assert np.all(np.diff(off)[full] == n_keys)
+    bad = row_of.copy()
+    bad[np.flatnonzero(~full)[0]] = n_rows
+    with pytest.raises(tm.TagMatchError):
+        tm.expand_key_rows_packed(bad, packed, dictionary)

@@ -101,3 +101,70 @@ def test_host_rows_only_for_bitmap_rings(table):
     buf.close()
     with pytest.raises(tm.TagMatchError):
         tm.Server(fib, n_slots=1, slot_queries=64, mode=tm.MODE_KEY_BITMAP)
+
+
+def test_packed_host_rows(gpu):
+    """Packed delivery (tm_ring_set_host_rows_packed): a table where a
+    one-bit filter carries every key, so the queries holding that bit match
+    the whole dictionary and the others do not.  Per step: the entries number
+    the other queries' rows in order, the packed rows are theirs, and the
+    expansion equals the oracle; steps whose packed rows outgrow what the
+    previous step copied with itself, graph replays, switching forms and a
+    capacity error."""
+    cw, cko, ck, q, _ = tm.workload(60000, 20000, seed=1211)
+    every = np.unique(ck)
+    freq = np.array([(q[:, b // 32] >> np.uint32(b % 32) & 1).mean() for b in range(192)])
+    bit = int(np.argmin(np.abs(freq - 0.5)))
+    w1 = np.zeros((1, 6), np.uint32)
+    w1[0, bit // 32] = np.uint32(1 << (bit % 32))
+    words = np.vstack([cw, w1])
+    ko = np.append(cko.astype(np.uint64), np.uint64(cko[-1]) + np.uint64(every.size))
+    cw2, cko2, ck2 = tm.canonicalize(words, ko, np.concatenate([ck, every]))
+    fib, *_ = tm.build_fib(cw2, cko2, ck2, 6000)
+    d = fib.dictionary()
+    kw = (len(d) + 31) // 32
+    log(f"bit {bit} in {freq[bit]:.2f} of the queries, K={len(d)}")
+    # slices whose rows not full grow from step to step
+    steps = [(0, 300), (300, 3000), (3300, 12000), (15300, 40), (15340, 4000)]
+    for graph in (0, 1):
+        ring = tm.Ring(fib, 12000, mode=tm.MODE_KEY_BITMAP, use_graph=graph)
+        buf = tm.PackedRows(12000, kw)
+        ring.set_host_rows_packed(buf)
+        for a, n in steps:
+            qa = q[a:a + n]
+            eo, ev = orc.match(cw2, cko2, ck2, qa, mode=0)
+            ring.submit(qa)
+            st = ring.wait()
+            full = np.diff(eo.astype(np.int64)) == len(d)
+            assert 0.1 < full.mean() < 0.95 or n < 100
+            npk = int(st.n_results)
+            row_of = buf.row_of.array[:n].copy()
+            assert npk == int((~full).sum())
+            assert np.array_equal(row_of == tm.ROW_FULL, full)
+            assert np.array_equal(row_of[~full], np.arange(npk))
+            packed = buf.rows.array[:npk * kw].reshape(npk, kw)
+            assert np.array_equal(buf.unpack(n, npk, len(d)), ring.key_bitmaps())
+            off, vals = tm.expand_key_rows_packed(row_of, packed, d)
+            assert np.array_equal(off, eo) and np.array_equal(vals, ev), (graph, a, n)
+        # the other form replaces this one
+        rows = tm.PinnedRows(12000 * kw)
+        ring.set_host_rows(rows)
+        buf.row_of.array[:] = 7
+        ring.submit(q[:500])
+        ring.wait()
+        assert np.all(buf.row_of.array[:500] == 7)
+        assert np.array_equal(rows.rows(500, kw), ring.key_bitmaps())
+        ring.set_host_rows(None)
+        ring.close()
+        rows.close()
+        buf.close()
+    # capacity below the submission
+    ring = tm.Ring(fib, 100, mode=tm.MODE_KEY_BITMAP)
+    small = tm.PackedRows(50, kw)
+    ring.set_host_rows_packed(small)
+    with pytest.raises(tm.TagMatchError):
+        ring.submit(q[:100])
+    ring.set_host_rows_packed(None)
+    ring.close()
+    small.close()
+    fib.close()
