@@ -100,7 +100,9 @@ def parse():
                          "plain form); c5: the serving loop at 200M filters; both print one JSON line of their own")
     ap.add_argument("--c4-filters", type=int, default=50_000_000)
     ap.add_argument("--c4-rows", type=int, default=2_000_000, help="plain unit log rows to train on")
-    ap.add_argument("--c4-log-seconds", type=float, default=420.0, help="time limit of the logging runs")
+    ap.add_argument("--c4-log-queries", type=int, default=4_000_000,
+                    help="queries the plain form logs units over (drawn apart from the timed ones)")
+    ap.add_argument("--c4-log-seconds", type=float, default=660.0, help="time limit of the logging runs")
     ap.add_argument("--c5-filters", type=int, default=200_000_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run an elementwise device copy of MIB MiB (read + write) three times, "
@@ -496,15 +498,17 @@ def c4_leg(args):
     cw, cko, ck, q, fib, perm, off, masks, setup_s = big_table(args.c4_filters, args, nq)
     res = {"leg": "c4", "filters": args.c4_filters, "distinct_filters": int(cw.shape[0]),
            "partitions": int(len(off) - 1), "queries": nq, "setup_s": setup_s}
-    # ---- plain unit logs, chunk by chunk, until the row target or the time limit
+    # ---- plain unit logs, chunk by chunk, until the row target or the time
+    # limit, over queries of their own (the same generator, past the timed ones)
+    ql, _ = tm.Generator(seed=args.seed).queries(args.c4_log_queries, args.c4_filters, first=1 << 32)
     chunk = 16384
     held_chunks = 2
     ring = tm.Ring(fib, chunk, schedule=tm.SCHED_PLAIN, batch_size=256, log_units=1)
     cols, held = [[], [], [], []], [[], [], [], []]
     rows, t0, c = 0, time.time(), 0
     plain_s, plain_q = 0.0, 0
-    while c * chunk < nq:
-        qa = q[c * chunk:(c + 1) * chunk]
+    while c * chunk < len(ql):
+        qa = ql[c * chunk:(c + 1) * chunk]
         ts = time.perf_counter()
         ring.submit(qa)
         ring.wait()
