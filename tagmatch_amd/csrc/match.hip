@@ -928,33 +928,57 @@ k_bm_count(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t
 	if (lane == 0) qcount[qi] = c;
 }
 
-// KEY_BITMAP, packed delivery: one wave per query flags a row that is not
-// the whole dictionary (bits 0..K-1 set, last_mask in the last word).
+// KEY_BITMAP, packed delivery: flag each query whose row is not the whole
+// dictionary (bits 0..K-1 set, last_mask in the last word).  A wave takes 64
+// queries; a query whose full flag this step set (qfull, when valid) is full
+// without reading its row, and the wave reads the rows of the others one at
+// a time, coalesced.
 __global__ void __launch_bounds__(256)
 k_row_partial(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, uint32_t last_mask,
-			  uint32_t * __restrict__ flag) {
-	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-	if (qi >= nq) return;
+			  const uint32_t * __restrict__ qfull, uint32_t * __restrict__ flag) {
+	const uint32_t lane = threadIdx.x & 63;
+	const uint32_t base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+	if (base >= nq) return;
+	const uint32_t qi = base + lane;
+	const bool valid = qi < nq;
+	const bool known = valid && qfull && qfull[qi] != 0u;
+	uint64_t check = __ballot(valid && !known);
 	bool partial = false;
-	for (uint32_t w = lane; w < kw; w += 64)
-		partial |= gbm[(size_t)qi * kw + w] != (w + 1 == kw ? last_mask : 0xFFFFFFFFu);
-	const uint64_t any = __ballot(partial);
-	if (lane == 0) flag[qi] = any != 0;
+	while (check) {
+		const uint32_t j = (uint32_t)__builtin_ctzll(check);
+		check &= check - 1;
+		const uint32_t * row = gbm + (size_t)(base + j) * kw;
+		bool diff = false;
+		for (uint32_t w = lane; w < kw; w += 64) diff |= row[w] != (w + 1 == kw ? last_mask : 0xFFFFFFFFu);
+		const bool p = __ballot(diff) != 0;
+		if (lane == j) partial = p;
+	}
+	if (valid) flag[qi] = partial;
 }
 
 // KEY_BITMAP, packed delivery: each query's entry (TM_ROW_FULL or the index
 // of its row among the flagged ones, in query order) and the flagged rows,
-// packed.  One wave per query.
+// packed.  A wave takes 64 queries and copies their flagged rows one at a
+// time, coalesced.
 __global__ void __launch_bounds__(256)
 k_row_pack(const uint32_t * __restrict__ gbm, uint32_t kw, uint32_t nq, const uint32_t * __restrict__ flag,
-		   const uint64_t * __restrict__ base, uint32_t * __restrict__ row_of, uint32_t * __restrict__ prow) {
-	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-	if (qi >= nq) return;
-	const bool partial = flag[qi] != 0;
-	const uint64_t b = base[qi];
-	if (lane == 0) row_of[qi] = partial ? (uint32_t)b : TM_ROW_FULL;
-	if (!partial) return;
-	for (uint32_t w = lane; w < kw; w += 64) prow[b * kw + w] = gbm[(size_t)qi * kw + w];
+		   const uint64_t * __restrict__ base_of, uint32_t * __restrict__ row_of, uint32_t * __restrict__ prow) {
+	const uint32_t lane = threadIdx.x & 63;
+	const uint32_t base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+	if (base >= nq) return;
+	const uint32_t qi = base + lane;
+	const bool valid = qi < nq;
+	const bool partial = valid && flag[qi] != 0u;
+	const uint64_t b = valid ? base_of[qi] : 0;
+	if (valid) row_of[qi] = partial ? (uint32_t)b : TM_ROW_FULL;
+	uint64_t copy = __ballot(partial);
+	while (copy) {
+		const uint32_t j = (uint32_t)__builtin_ctzll(copy);
+		copy &= copy - 1;
+		const uint64_t dst = __shfl(b, (int)j, 64);
+		const uint32_t * src = gbm + (size_t)(base + j) * kw;
+		for (uint32_t w = lane; w < kw; w += 64) prow[dst * kw + w] = src[w];
+	}
 }
 
 // KEYS_UNIQUE: one wave per query writes its keys in ascending id order,
@@ -1145,74 +1169,91 @@ k_seg_bounds(const uint64_t * __restrict__ prefix, uint32_t nb, uint32_t nseg, u
 	segb[k] = lo;
 }
 
-// Rich-filter pass, one wave per query: the key bitmaps of the rich
-// filters that the query contains, ORed into its row, which this writes
-// whole; flag 2 when the row then holds every key (the match kernel's
-// full-key-set exit drops the query), else 0.  Exact: a filter contained in
-// the query matches it, whatever the partition lookup.
+// Rich-filter pass: the key bitmaps of the rich filters that a query
+// contains, ORed into its row, which this writes whole; flag 2 when the row
+// then holds every key (the match kernel's full-key-set exit drops the
+// query), else 0.  Exact: a filter contained in the query matches it,
+// whatever the partition lookup.  A wave takes 64 queries: first one lane
+// per query against the rich filters that hold every key (most queries
+// contain one), whose rows the wave then writes full, row by row; then the
+// other queries one at a time with the whole wave, 64 rich filters per test
+// and eight bitmaps in flight.
 __global__ void __launch_bounds__(256)
 k_rich_rows(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ rw,
 			const uint32_t * __restrict__ rbm, uint32_t nr, uint32_t nfull, uint32_t kw, uint32_t last_mask,
 			uint32_t * __restrict__ gbm, uint32_t * __restrict__ qfull) {
 	const uint32_t lane = threadIdx.x & 63;
-	const uint32_t qi = blockIdx.x * 4 + (threadIdx.x >> 6);
-	if (qi >= nq) return;
-	const uint4 x = *reinterpret_cast<const uint4 *>(q + (size_t)qi * 8);
-	const uint2 y = *reinterpret_cast<const uint2 *>(q + (size_t)qi * 8 + 4);
-	uint32_t * row = gbm + (size_t)qi * kw;
+	const uint32_t base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+	if (base >= nq) return;
+	const uint32_t qi = base + lane;
+	const bool valid = qi < nq;
+	uint4 x = make_uint4(0, 0, 0, 0);
+	uint2 y = make_uint2(0, 0);
+	if (valid) {
+		x = *reinterpret_cast<const uint4 *>(q + (size_t)qi * 8);
+		y = *reinterpret_cast<const uint2 *>(q + (size_t)qi * 8 + 4);
+	}
 	// the first nfull rich filters hold every key: one contained is enough
-	bool any_full = false;
-	for (uint32_t fb = 0; fb < nfull && !any_full; fb += 64) {
-		const uint32_t f = fb + lane;
-		bool hit = false;
-		if (f < nfull) {
-			const uint4 a = *reinterpret_cast<const uint4 *>(rw + (size_t)f * 8);
-			const uint2 b = *reinterpret_cast<const uint2 *>(rw + (size_t)f * 8 + 4);
-			hit = ((a.x & ~x.x) | (a.y & ~x.y) | (a.z & ~x.z) | (a.w & ~x.w) | (b.x & ~y.x) | (b.y & ~y.y)) == 0;
-		}
-		any_full = __ballot(hit) != 0;
+	bool full = false;
+	for (uint32_t f = 0; f < nfull; ++f) {
+		const uint4 a = sld4(rw + (size_t)f * 8);
+		const uint2 b = sld2(rw + (size_t)f * 8 + 4);
+		full |= ((a.x & ~x.x) | (a.y & ~x.y) | (a.z & ~x.z) | (a.w & ~x.w) | (b.x & ~y.x) | (b.y & ~y.y)) == 0;
+		if ((f & 7) == 7 && __ballot(valid && !full) == 0) break;
 	}
-	if (any_full) {
+	uint64_t m = __ballot(valid && full);
+	while (m) {
+		const uint32_t j = (uint32_t)__builtin_ctzll(m);
+		m &= m - 1;
+		uint32_t * row = gbm + (size_t)(base + j) * kw;
 		for (uint32_t w = lane; w < kw; w += 64) row[w] = w + 1 == kw ? last_mask : 0xFFFFFFFFu;
-		if (lane == 0) qfull[qi] = 2u;
-		return;
 	}
-	uint32_t all = 0xFFFFFFFFu;
-	for (uint32_t wb = 0; wb < kw; wb += 64) {   // 64 words (2048 keys) per pass
-		const uint32_t w = wb + lane;
-		uint32_t acc = 0;
-		for (uint32_t fb = 0; fb < nr; fb += 64) {
-			const uint32_t f = fb + lane;
-			bool hit = false;
-			if (f < nr) {
-				const uint4 a = *reinterpret_cast<const uint4 *>(rw + (size_t)f * 8);
-				const uint2 b = *reinterpret_cast<const uint2 *>(rw + (size_t)f * 8 + 4);
-				hit = ((a.x & ~x.x) | (a.y & ~x.y) | (a.z & ~x.z) | (a.w & ~x.w) | (b.x & ~y.x) | (b.y & ~y.y)) == 0;
+	bool done_full = full;
+	uint64_t rest = __ballot(valid && !full);
+	while (rest) {
+		const uint32_t j = (uint32_t)__builtin_ctzll(rest);
+		rest &= rest - 1;
+		const uint32_t qx = __shfl(x.x, (int)j, 64), qy = __shfl(x.y, (int)j, 64), qz = __shfl(x.z, (int)j, 64),
+					   qw = __shfl(x.w, (int)j, 64), q4 = __shfl(y.x, (int)j, 64), q5 = __shfl(y.y, (int)j, 64);
+		uint32_t * row = gbm + (size_t)(base + j) * kw;
+		uint32_t all = 0xFFFFFFFFu;
+		for (uint32_t wb = 0; wb < kw; wb += 64) {   // 64 words (2048 keys) per pass
+			const uint32_t w = wb + lane;
+			uint32_t acc = 0;
+			for (uint32_t fb = 0; fb < nr; fb += 64) {
+				const uint32_t f = fb + lane;
+				bool hit = false;
+				if (f < nr) {
+					const uint4 a = *reinterpret_cast<const uint4 *>(rw + (size_t)f * 8);
+					const uint2 b = *reinterpret_cast<const uint2 *>(rw + (size_t)f * 8 + 4);
+					hit = ((a.x & ~qx) | (a.y & ~qy) | (a.z & ~qz) | (a.w & ~qw) | (b.x & ~q4) | (b.y & ~q5)) == 0;
+				}
+				uint64_t hm = __ballot(hit);
+				while (hm) {   // eight bitmaps in flight per round (a repeat ORs twice)
+					uint32_t jj[8];
+#pragma unroll
+					for (int k = 0; k < 8; ++k) {
+						jj[k] = hm ? (uint32_t)__builtin_ctzll(hm) : jj[0];
+						hm &= hm - 1;
+					}
+					if (w < kw) {
+						uint32_t v[8];
+#pragma unroll
+						for (int k = 0; k < 8; ++k) v[k] = rbm[(size_t)(fb + jj[k]) * kw + w];
+#pragma unroll
+						for (int k = 0; k < 8; ++k) acc |= v[k];
+					}
+				}
 			}
-			uint64_t m = __ballot(hit);
-			while (m) {   // eight bitmaps in flight per round (a repeat ORs twice)
-				uint32_t j[8];
-#pragma unroll
-				for (int k = 0; k < 8; ++k) {
-					j[k] = m ? (uint32_t)__builtin_ctzll(m) : j[0];
-					m &= m - 1;
-				}
-				if (w < kw) {
-					uint32_t v[8];
-#pragma unroll
-					for (int k = 0; k < 8; ++k) v[k] = rbm[(size_t)(fb + j[k]) * kw + w];
-#pragma unroll
-					for (int k = 0; k < 8; ++k) acc |= v[k];
-				}
+			if (w < kw) {
+				row[w] = acc;
+				all &= w + 1 == kw ? (acc | ~last_mask) : acc;
 			}
 		}
-		if (w < kw) {
-			row[w] = acc;
-			all &= w + 1 == kw ? (acc | ~last_mask) : acc;
-		}
+		const bool f = __ballot(all != 0xFFFFFFFFu) == 0;
+		if (lane == j) done_full = f;
 	}
-	const bool full = __ballot(all != 0xFFFFFFFFu) == 0;
-	if (lane == 0) qfull[qi] = full ? 2u : 0u;
+	if (valid) qfull[qi] = done_full ? 2u : 0u;
 }
 
 // Queries whose rows are not yet full after the rich-filter pass, listed
@@ -1296,8 +1337,11 @@ __global__ void k_init_queues_active(uint32_t * __restrict__ act, uint32_t nqg, 
 	}
 }
 
-// Queries per partition (the lookup's counts) with per-block counters in
-// LDS: one global atomic per partition per block.  P counters of LDS.
+// Queries per partition (the lookup's counts).  Lanes are partitions, 64 at
+// a time; the wave's share of the queries is read with scalar loads, and
+// each lane counts the queries that select its partition in a register.
+// Counts meet in LDS (P counters), then one global atomic per partition per
+// workgroup.
 __global__ void __launch_bounds__(256)
 k_lookup_hist(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ part_info, uint32_t P,
 			  uint32_t * __restrict__ pcount) {
@@ -1305,19 +1349,25 @@ k_lookup_hist(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __r
 	for (uint32_t p = threadIdx.x; p < P; p += 256) s_cnt[p] = 0;
 	__syncthreads();
 	const uint32_t lane = threadIdx.x & 63;
-	for (uint32_t i0 = blockIdx.x * 256; i0 < nq; i0 += gridDim.x * 256) {
-		const uint32_t i = i0 + threadIdx.x;
-		const bool valid = i < nq;
-		uint32_t n[6] = {~0u, ~0u, ~0u, ~0u, ~0u, ~0u};
-		if (valid)
-			for (int j = 0; j < 6; ++j) n[j] = ~q[(size_t)i * 8 + j];
-		for (uint32_t p = 0; p < P; ++p) {
-			const uint4 ma = sld4(part_info + (size_t)p * 8);
-			const uint2 mb = sld2(part_info + (size_t)p * 8 + 4);
-			const uint32_t t = (ma.x & n[0]) | (ma.y & n[1]) | (ma.z & n[2]) | (ma.w & n[3]) | (mb.x & n[4]) | (mb.y & n[5]);
-			const uint64_t b = __ballot(valid && t == 0);
-			if (b && lane == 0) atomicAdd(&s_cnt[p], (uint32_t)__popcll(b));
+	const uint32_t gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+	const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+	const uint32_t per = (nq + nwaves - 1) / nwaves;
+	const uint32_t qa = min(gw * per, nq), qb = min(qa + per, nq);
+	for (uint32_t p0 = 0; p0 < P; p0 += 64) {
+		const uint32_t p = p0 + lane;
+		uint32_t m[6] = {0, 0, 0, 0, 0, 0};
+		if (p < P)
+			for (int j = 0; j < 6; ++j) m[j] = part_info[(size_t)p * 8 + j];
+		uint32_t cnt = 0;
+#pragma unroll 4
+		for (uint32_t i = qa; i < qb; ++i) {
+			const uint4 x = sld4(q + (size_t)i * 8);
+			const uint2 y = sld2(q + (size_t)i * 8 + 4);
+			const uint32_t t = (m[0] & ~x.x) | (m[1] & ~x.y) | (m[2] & ~x.z) | (m[3] & ~x.w) | (m[4] & ~y.x) |
+							   (m[5] & ~y.y);
+			cnt += t == 0;
 		}
+		if (p < P && cnt) atomicAdd(&s_cnt[p], cnt);
 	}
 	__syncthreads();
 	for (uint32_t p = threadIdx.x; p < P; p += 256)
@@ -1461,8 +1511,10 @@ struct tm_ring {
 		uint32_t nq = 0, epoch = 0;
 		const uint32_t * slot_epoch = nullptr;
 		bool weighted = false;       // reads the block weights, refreshed outside the graph
+		bool qfull_valid = false;    // writes d_qfull
 	};
 	bool capturing = false;          // enqueue_persistent runs inside a stream capture
+	bool qfull_valid = false;        // the last step wrote d_qfull: a set flag means a full row
 	bool step_weighted = false;      // the last enqueue_persistent cut weighted segments
 	std::vector<graph_entry> graphs;
 
@@ -1747,7 +1799,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	if (rich_pass) {
 		// rows start with the keys of the rich filters each query contains
 		// (written whole, with the flags); full rows need no test at all
-		hipLaunchKernelGGL(k_rich_rows, dim3((nq + 3) / 4), dim3(256), 0, s, r->d_q, nq, r->fib->d_rich_words,
+		hipLaunchKernelGGL(k_rich_rows, dim3((nq + 255) / 256), dim3(256), 0, s, r->d_q, nq, r->fib->d_rich_words,
 						   r->fib->d_rich_bm, r->fib->n_rich, r->fib->n_rich_full, r->kw, last_mask, r->d_gbm,
 						   r->d_qfull);
 	} else if (nseg > 1 && keyrows(r->cfg)) {
@@ -1761,7 +1813,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	if (compact) {
 		fill_u32(r->d_nact, 0, 4, s);
 		if (P <= 16384) {
-			const uint32_t gh = (uint32_t)std::min<uint64_t>((nq + 255) / 256, 512);
+			const uint32_t gh = (uint32_t)std::min<uint64_t>((nq + 255) / 256, 2048);
 			hipLaunchKernelGGL(k_lookup_hist, dim3(gh), dim3(256), (size_t)P * 4, s, r->d_q, nq, r->fib->part_info, P,
 							   r->d_pcount);
 		} else {
@@ -1811,6 +1863,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.gbm = r->d_gbm;
 	// flags hold this submission's state: cleared or set by the pass above
 	a.qfull = (nseg > 1 || rich_pass) ? r->d_qfull : nullptr;
+	r->qfull_valid = a.qfull != nullptr;   // set flags mean full rows this step
 	a.hits = r->d_hits;
 	a.hit_cap = r->hit_cap;
 	a.counters = r->d_cnt;
@@ -2200,8 +2253,9 @@ int enqueue_delivery(tm_ring * r, uint32_t nq) {
 	hipStream_t s = r->s;
 	const uint32_t kw = r->kw, bits = r->fib->n_keys % 32;
 	const uint32_t last_mask = bits ? (1u << bits) - 1u : 0xFFFFFFFFu;
-	const uint32_t g = (nq + 3) / 4;
-	hipLaunchKernelGGL(k_row_partial, dim3(g), dim3(256), 0, s, r->d_gbm, kw, nq, last_mask, r->d_rflag);
+	const uint32_t g = (nq + 255) / 256;   // a wave per 64 queries
+	hipLaunchKernelGGL(k_row_partial, dim3(g), dim3(256), 0, s, r->d_gbm, kw, nq, last_mask,
+					   r->qfull_valid ? r->d_qfull : nullptr, r->d_rflag);
 	scan_u32_to_u64(r->d_rflag, r->d_rbase, nq, r->d_rscan, s);
 	hipLaunchKernelGGL(k_row_pack, dim3(g), dim3(256), 0, s, r->d_gbm, kw, nq, r->d_rflag, r->d_rbase, r->d_row_of,
 					   r->d_prow);
@@ -2231,6 +2285,7 @@ int ensure_pack(tm_ring * r, uint64_t nq) {
 
 int enqueue_step(tm_ring * r, bool allow_graph) {
 	const uint32_t nq = r->run_nq;
+	r->qfull_valid = false;   // set by the persistent path that writes the flags
 	TM_HIP(hipEventRecord(r->e0, r->s));
 	if (r->units_mode) {
 		TM_CHECK(enqueue_units(r, nq));
@@ -2284,10 +2339,12 @@ int enqueue_step(tm_ring * r, bool allow_graph) {
 			e.epoch = r->epoch;
 			e.slot_epoch = r->fib->slot_epoch;
 			e.weighted = r->step_weighted;
+			e.qfull_valid = r->qfull_valid;
 			r->graphs.push_back(e);
 			ge = &r->graphs.back();
 		}
 		if (ge->weighted) ensure_weights(r, r->s);
+		r->qfull_valid = ge->qfull_valid;
 		TM_HIP(hipGraphLaunch(ge->exec, r->s));
 		r->match_events = false;
 	} else if (persistent) {

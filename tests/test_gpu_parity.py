@@ -418,3 +418,26 @@ def test_cpp_api_reference_cases(gpu, golden):
     r = subprocess.run([exe, path], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
+
+
+@pytest.mark.parametrize("nq", [100, 20000])
+def test_lookup_counts_vs_bruteforce(gpu, nq):
+    """Per-partition query counts of the lookup (tm_ring_lookup_counts) and
+    the (query, partition) pair count against a brute force over the
+    partition masks: a small step, and a compacted one (counted by
+    k_lookup_hist over every query), in each output mode and schedule."""
+    cw, cko, ck, q, _ = tm.workload(500000, nq, seed=0x7461676D61746302)
+    fib, perm, off, masks = tm.build_fib(cw, cko, ck, 64 * 1024)
+    m = np.asarray(masks, np.uint32).reshape(-1, 6)
+    exp = ((m[None, :, :] & ~q[:, None, :]) == 0).all(axis=2).sum(axis=0)
+    for mode in (tm.MODE_KEYS_UNIQUE, tm.MODE_KEY_BITMAP, tm.MODE_HITS):
+        for sched in (tm.SCHED_PERSISTENT, tm.SCHED_PLAIN):
+            ring = tm.Ring(fib, nq, mode=mode, schedule=sched)
+            ring.submit(q)
+            st = ring.wait()
+            got = ring.lookup_counts()
+            assert np.array_equal(got[:len(exp)], exp), (mode, sched)
+            assert not got[len(exp):].any()
+            assert st.n_pairs == int(exp.sum()), (mode, sched)
+            ring.close()
+    fib.close()
