@@ -50,8 +50,10 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # int32 lane-ops/s, non-packed
 OPS_PER_PAIR = 7                            # 6 and-not/or + 1 compare per pair-test
 PCIE_PEAK_GBS = 63.0                        # PCIe 5.0 x16, one direction
-# (queries per work item, waves per workgroup, blocks per segment; 0 = automatic)
-AUTOTUNE_CANDIDATES = [(64, 4, 0), (64, 8, 0), (64, 4, 2048), (64, 4, 512), (128, 4, 0), (128, 8, 0)]
+# (queries per work item, waves per workgroup, blocks per segment (0 = automatic), replay from a
+# captured graph)
+AUTOTUNE_CANDIDATES = [(64, 4, 0, 0), (64, 8, 0, 0), (64, 4, 2048, 0), (64, 4, 512, 0), (128, 4, 0, 0),
+                       (128, 8, 0, 0), (64, 4, 0, 1), (64, 8, 0, 1)]
 
 
 def log(*a):
@@ -119,11 +121,13 @@ def setup(args):
     cw, cko, ck = tm.canonicalize(w, ko, k)
     del w, ko, k
     t2 = time.time()
-    fib, perm, off, masks = tm.build_fib(cw, cko, ck, args.maxp)
+    times = {}
+    fib, perm, off, masks = tm.build_fib(cw, cko, ck, args.maxp, times=times)
     t3 = time.time()
-    log(f"setup: generate {t1 - t0:.1f}s canonicalize {t2 - t1:.1f}s partition+upload {t3 - t2:.1f}s; "
-        f"{cw.shape[0]} distinct filters, {len(off) - 1} partitions")
-    return cw, cko, ck, q, fib, perm, off, masks, t3 - t0
+    log(f"setup: generate {t1 - t0:.1f}s canonicalize {t2 - t1:.1f}s partition (GPU) {times['partition']:.2f}s "
+        f"upload {times['upload']:.2f}s; {cw.shape[0]} distinct filters, {len(off) - 1} partitions")
+    return cw, cko, ck, q, fib, perm, off, masks, {"total": t3 - t0, "generate": t1 - t0, "canonicalize": t2 - t1,
+                                                    **times}
 
 
 def _sized(fn, q, budget_s, n=16):
@@ -150,7 +154,7 @@ def cpu_info():
     return {"hardware_concurrency": os.cpu_count(), "cpu_model": model}
 
 
-def cpu_baseline(cw, cko, ck, perm, part_off, masks, q, budget_s):
+def cpu_baseline(cw, cko, ck, perm, part_off, masks, q, budget_s, maxp):
     """CPU baselines on bounded samples of the same queries, on the host's
     cores, all with the reference's match_unique output (each query's keys
     collected, sorted and deduplicated):
@@ -183,6 +187,19 @@ def cpu_baseline(cw, cko, ck, perm, part_off, masks, q, budget_s):
            "bruteforce": {"value": n2 / dt2, "unit": "queries/s", "cores": threads, "queries": n2, "seconds": dt2},
            "bruteforce_single_thread": {"value": n3 / dt3, "unit": "queries/s", "cores": 1, "queries": n3,
                                         "seconds": dt3}}
+    # the partitioning of the whole table on the host: the oracle's
+    # restatement (one thread) and, when built, the reference's own program
+    # (all threads; its time includes writing the table to a file and
+    # reading the result back)
+    t = time.time()
+    _, _, poff, _ = orc.partition(cw, maxp)
+    out["partition"] = {"port_seconds": time.time() - t, "port_threads": 1, "partitions": int(len(poff) - 1)}
+    if orc.ref_available("ref_partition"):
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            t = time.time()
+            orc.ref_partition(cw, maxp, d, threads=threads)
+            out["partition"].update({"reference_seconds": time.time() - t, "reference_threads": threads})
     if orc.ref_available("libref_patricia.so"):
         t = time.time()
         trie = orc.RefPatricia(cw, cko, ck)
@@ -474,9 +491,11 @@ def big_table(n_filters, args, n_queries):
     cw, cko, ck = tm.canonicalize(w, ko, k)
     del w, ko, k
     log(f"canonical: {cw.shape[0]} distinct filters, {time.time() - t0:.1f}s")
-    fib, perm, off, masks = tm.build_fib(cw, cko, ck, args.maxp)
-    log(f"partitioned and uploaded: {len(off) - 1} partitions, {time.time() - t0:.1f}s")
-    return cw, cko, ck, q, fib, perm, off, masks, time.time() - t0
+    times = {}
+    fib, perm, off, masks = tm.build_fib(cw, cko, ck, args.maxp, times=times)
+    log(f"partitioned (GPU {times['partition']:.2f}s) and uploaded ({times['upload']:.2f}s): {len(off) - 1} "
+        f"partitions, {time.time() - t0:.1f}s")
+    return cw, cko, ck, q, fib, perm, off, masks, {"total": time.time() - t0, **times}
 
 
 def oracle_sample(cw, cko, ck, q, n):
@@ -497,7 +516,7 @@ def c4_leg(args):
     nq = args.queries
     cw, cko, ck, q, fib, perm, off, masks, setup_s = big_table(args.c4_filters, args, nq)
     res = {"leg": "c4", "filters": args.c4_filters, "distinct_filters": int(cw.shape[0]),
-           "partitions": int(len(off) - 1), "queries": nq, "setup_s": setup_s}
+           "partitions": int(len(off) - 1), "queries": nq, "setup_seconds": setup_s}
     # ---- plain unit logs, chunk by chunk, until the row target or the time
     # limit, over queries of their own (the same generator, past the timed ones)
     ql, _ = tm.Generator(seed=args.seed).queries(args.c4_log_queries, args.c4_filters, first=1 << 32)
@@ -607,7 +626,7 @@ def c5_leg(args):
     oracle on a sample first."""
     cw, cko, ck, q, fib, perm, off, masks, setup_s = big_table(args.c5_filters, args, 4096)
     res = {"leg": "c5", "filters": args.c5_filters, "distinct_filters": int(cw.shape[0]),
-           "partitions": int(len(off) - 1), "setup_s": setup_s, "device_bytes": int(fib.stats()["device_bytes"])}
+           "partitions": int(len(off) - 1), "setup_seconds": setup_s, "device_bytes": int(fib.stats()["device_bytes"])}
     n_or = 32
     srv = tm.Server(fib, n_slots=args.serve_slots, slot_queries=args.serve_slot_queries,
                     step_graphs=args.serve_graphs)
@@ -759,13 +778,14 @@ def main():
     tuned = None
     if args.autotune and sched == tm.SCHED_PERSISTENT and not args.tile:
         # the autotuner (tm_ring_autotune) picks the tile size, waves per
-        # workgroup and segment size on this workload, before the timed steps
-        cands = [dict(batch_size=b, waves_per_group=w, chunk_blocks=c, use_graph=int(args.graph))
-                 for b, w, c in AUTOTUNE_CANDIDATES]
+        # workgroup, segment size and graph replay on this workload, before
+        # the timed steps
+        cands = [dict(batch_size=b, waves_per_group=w, chunk_blocks=c, use_graph=int(g or args.graph))
+                 for b, w, c, g in AUTOTUNE_CANDIDATES]
         best, times = ring.autotune(nq, cands, reps=1)
         cfg = ring.cfg
         kw.update(batch_size=cands[best]["batch_size"], waves_per_group=cands[best]["waves_per_group"],
-                  chunk_blocks=cands[best]["chunk_blocks"])
+                  chunk_blocks=cands[best]["chunk_blocks"], use_graph=cands[best]["use_graph"])
         tuned = {"candidates": cands, "step_ms": [float(x) for x in times], "best": cands[best]}
         log(f"autotuned: {cands[best]} ({times[best]:.1f} ms); all {np.round(times, 1).tolist()}")
     for i in range(max(0, args.warmup - 1)):
@@ -967,7 +987,7 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(cw, cko, ck, perm, part_off, masks, q, args.cpu_seconds)
+        cpu = cpu_baseline(cw, cko, ck, perm, part_off, masks, q, args.cpu_seconds, args.maxp)
 
     out = {
         "metric": METRIC,
@@ -989,7 +1009,7 @@ def main():
                         f"unique key set in host memory (packed key bitmap rows over the sorted dictionary)",
             "filters": args.filters, "distinct_filters": int(cw.shape[0]), "queries": nq,
             "partitions": int(len(part_off) - 1), "maxp": args.maxp, "batch": args.batch,
-            "schedule": args.schedule, "graph": bool(args.graph),
+            "schedule": args.schedule, "graph": bool(kw.get("use_graph")),
             "result_form": "packed key bitmap rows in host memory",
         },
         "value_note": "queries/s with every query's match_unique result (sorted unique keys) in host memory, packed "
@@ -1030,7 +1050,7 @@ def main():
                       "batch, no rich-filter pass or full-key-set exit) on a prefix of the queries; the same-"
                       "algorithm ratio compares it with the persistent step without the exit",
         "cpu_baseline": cpu,
-        "setup_s": setup_s,
+        "setup_seconds": setup_s,
         "step_stats": stats[-1],
     }
     line = json.dumps(out)
