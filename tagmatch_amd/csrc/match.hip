@@ -399,7 +399,7 @@ template <int MODE, bool LDS_BM, int TILES, int NW, int VAR>
 // (8 waves per SIMD): at 10M filters the step that enumerates every match
 // takes 682 ms instead of 713 and the compacted match 2.54-2.67 ms instead
 // of 2.70, despite a few spilled registers.  The same bound on the 8- and
-// 16-wave instances made small steps slower (a 64-query step 0.074 ->
+// 16-wave instances made small steps slower (a 64-query step 0.101 ->
 // 0.111 ms), so they keep the compiler's choice.
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(TILES == 1 && NW == 4 ? 8 : 1)))
 k_match_q(mq_args a) {
@@ -1179,20 +1179,22 @@ k_seg_bounds(const uint64_t * __restrict__ prefix, uint32_t nb, uint32_t nseg, u
 // contains, ORed into its row, which this writes whole; flag 2 when the row
 // then holds every key (the match kernel's full-key-set exit drops the
 // query), else 0.  Exact: a filter contained in the query matches it,
-// whatever the partition lookup.  A wave takes 64 queries: first one lane
-// per query against the rich filters that hold every key (most queries
-// contain one), whose rows the wave then writes full, row by row; then the
-// other queries one at a time with the whole wave, 64 rich filters per test
-// and eight bitmaps in flight.
+// whatever the partition lookup.  A wave takes qpw queries (at most 64):
+// first one lane per query against the rich filters that hold every key
+// (most queries contain one), whose rows the wave then writes full, row by
+// row; then the other queries one at a time with the whole wave, 64 rich
+// filters per test and eight bitmaps in flight.  Large steps use 64 queries
+// per wave; small ones fewer, so that their few incomplete queries are
+// spread over more waves.
 __global__ void __launch_bounds__(256)
-k_rich_rows(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ rw,
+k_rich_rows(const uint32_t * __restrict__ q, uint32_t nq, uint32_t qpw, const uint32_t * __restrict__ rw,
 			const uint32_t * __restrict__ rbm, uint32_t nr, uint32_t nfull, uint32_t kw, uint32_t last_mask,
 			uint32_t * __restrict__ gbm, uint32_t * __restrict__ qfull) {
 	const uint32_t lane = threadIdx.x & 63;
-	const uint32_t base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+	const uint32_t base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * qpw;
 	if (base >= nq) return;
 	const uint32_t qi = base + lane;
-	const bool valid = qi < nq;
+	const bool valid = lane < qpw && qi < nq;
 	uint4 x = make_uint4(0, 0, 0, 0);
 	uint2 y = make_uint2(0, 0);
 	if (valid) {
@@ -1201,11 +1203,30 @@ k_rich_rows(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __res
 	}
 	// the first nfull rich filters hold every key: one contained is enough
 	bool full = false;
-	for (uint32_t f = 0; f < nfull; ++f) {
-		const uint4 a = sld4(rw + (size_t)f * 8);
-		const uint2 b = sld2(rw + (size_t)f * 8 + 4);
-		full |= ((a.x & ~x.x) | (a.y & ~x.y) | (a.z & ~x.z) | (a.w & ~x.w) | (b.x & ~y.x) | (b.y & ~y.y)) == 0;
-		if ((f & 7) == 7 && __ballot(valid && !full) == 0) break;
+	if (qpw == 1) {
+		// one query (lane 0's): lanes are rich filters, 64 at a time
+		const uint32_t qx = __shfl(x.x, 0, 64), qy = __shfl(x.y, 0, 64), qz = __shfl(x.z, 0, 64),
+					   qw = __shfl(x.w, 0, 64), q4 = __shfl(y.x, 0, 64), q5 = __shfl(y.y, 0, 64);
+		bool any_full = false;
+		for (uint32_t fb = 0; fb < nfull && !any_full; fb += 64) {
+			const uint32_t f = fb + lane;
+			bool hit = false;
+			if (f < nfull) {
+				const uint4 a = *reinterpret_cast<const uint4 *>(rw + (size_t)f * 8);
+				const uint2 b = *reinterpret_cast<const uint2 *>(rw + (size_t)f * 8 + 4);
+				hit = ((a.x & ~qx) | (a.y & ~qy) | (a.z & ~qz) | (a.w & ~qw) | (b.x & ~q4) | (b.y & ~q5)) == 0;
+			}
+			any_full = __ballot(hit) != 0;
+		}
+		full = valid && any_full;
+	} else {
+		// one lane per query; the filters' words are the same for every lane
+		for (uint32_t f = 0; f < nfull; ++f) {
+			const uint4 a = sld4(rw + (size_t)f * 8);
+			const uint2 b = sld2(rw + (size_t)f * 8 + 4);
+			full |= ((a.x & ~x.x) | (a.y & ~x.y) | (a.z & ~x.z) | (a.w & ~x.w) | (b.x & ~y.x) | (b.y & ~y.y)) == 0;
+			if ((f & 7) == 7 && __ballot(valid && !full) == 0) break;
+		}
 	}
 	uint64_t m = __ballot(valid && full);
 	while (m) {
@@ -1805,7 +1826,12 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	if (rich_pass) {
 		// rows start with the keys of the rich filters each query contains
 		// (written whole, with the flags); full rows need no test at all
-		hipLaunchKernelGGL(k_rich_rows, dim3((nq + 255) / 256), dim3(256), 0, s, r->d_q, nq, r->fib->d_rich_words,
+		// queries per wave: 64 for large steps, fewer (down to 1) below
+		// 4096 waves' worth
+		uint32_t qpw = 64;
+		while (qpw > 1 && (uint64_t)nq < (uint64_t)qpw * 4096) qpw >>= 1;
+		const uint32_t waves = (nq + qpw - 1) / qpw;
+		hipLaunchKernelGGL(k_rich_rows, dim3((waves + 3) / 4), dim3(256), 0, s, r->d_q, nq, qpw, r->fib->d_rich_words,
 						   r->fib->d_rich_bm, r->fib->n_rich, r->fib->n_rich_full, r->kw, last_mask, r->d_gbm,
 						   r->d_qfull);
 	} else if (nseg > 1 && keyrows(r->cfg)) {

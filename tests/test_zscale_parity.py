@@ -161,6 +161,35 @@ def test_c2_full_run_cross_check(c2):
     assert np.array_equal(v, c2["default_prefix"])
 
 
+def test_c2_packed_delivery(c2):
+    """The headline's result form at full size: packed key bitmap rows in
+    pinned host memory (graph replay and not), expanded on the host, equal
+    the default KEYS_UNIQUE step on every query."""
+    q, fib = c2["q"], c2["fib"]
+    if "default_digest" not in c2:
+        off, vals, _ = step(c2["ring"], len(q))
+        c2["default_digest"] = digest(off, vals)
+        del off, vals
+    d = fib.dictionary()
+    kw = (len(d) + 31) // 32
+    for graph in (0, 1):
+        ring = tm.Ring(fib, len(q), tm.config(mode=tm.MODE_KEY_BITMAP, use_graph=graph))
+        buf = tm.PackedRows(len(q), kw)
+        ring.set_host_rows_packed(buf)
+        for _ in range(2):                    # the second step copies its rows with itself
+            ring.submit(q)
+            st = ring.wait()
+        n = int(st.n_results)
+        row_of = buf.row_of.array[:len(q)]
+        assert int((row_of != tm.ROW_FULL).sum()) == n
+        off, vals = tm.expand_key_rows_packed(row_of, buf.rows.array[:n * kw].reshape(n, kw), d)
+        log(f"C2 packed (graph {graph}): {n} rows not the whole dictionary, {int(off[-1])} keys")
+        assert digest(off, vals) == c2["default_digest"], f"C2 packed delivery (graph {graph})"
+        del off, vals
+        ring.close()
+        buf.close()
+
+
 def test_c2_hits_sample_vs_oracle(c2):
     q = c2["q"]
     idx = np.arange(7, len(q), 2000)                     # 500 queries
