@@ -395,13 +395,7 @@ struct hit_sink {
 // tests per iteration; VAR 1: from pre-inverted query words in HBM through
 // scalar loads, four independent tests per iteration.
 template <int MODE, bool LDS_BM, int TILES, int NW, int VAR>
-// The one-tile, 4-wave instance (large steps) is held to at most 64 VGPRs
-// (8 waves per SIMD): at 10M filters the step that enumerates every match
-// takes 682 ms instead of 713 and the compacted match 2.54-2.67 ms instead
-// of 2.70, despite a few spilled registers.  The same bound on the 8- and
-// 16-wave instances made small steps slower (a 64-query step 0.101 ->
-// 0.111 ms), so they keep the compiler's choice.
-__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(TILES == 1 && NW == 4 ? 8 : 1)))
+__global__ void __launch_bounds__(NW * 64)
 k_match_q(mq_args a) {
 	constexpr int NQ = TILES * 64;
 	constexpr int NT = NW * 64;
