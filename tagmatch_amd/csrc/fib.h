@@ -31,6 +31,10 @@ namespace tmx {
 //   part_stat [P][8]      u32   OR of the member filters (6 words), member
 //                               count, total popcount of the members (cost
 //                               model features only).
+//   sup_info  [nb/8][8]   u32   AND of the masks of blocks 8g..8g+7 (6
+//                               words; empty blocks count as all ones): a
+//                               query that does not contain it is a
+//                               candidate for none of those blocks.
 struct dev_fib_view {
 	const uint32_t * __restrict__ blk_words;
 	const uint32_t * __restrict__ blk_info;
@@ -41,6 +45,7 @@ struct dev_fib_view {
 	const uint32_t * __restrict__ key_dict;
 	const uint32_t * __restrict__ part_info;
 	const uint32_t * __restrict__ part_stat;
+	const uint32_t * __restrict__ sup_info;
 	uint32_t n_parts;
 	uint32_t n_blocks;
 	uint32_t n_keys;   // K, distinct keys
@@ -48,6 +53,7 @@ struct dev_fib_view {
 };
 
 constexpr uint32_t BLOCK_WORDS = 6 * 64;
+constexpr uint32_t SUP_BLOCKS = 8;   // blocks per group of sup_info
 constexpr uint32_t KEY_NONE = 0xFFFFu;
 constexpr uint32_t MAX_RICH = 1024;
 constexpr uint32_t RICH_MIN_KEYS = 32;
@@ -72,6 +78,8 @@ struct tm_fib {
 	uint32_t * key_dict = nullptr;
 	uint32_t * part_info = nullptr;
 	uint32_t * part_stat = nullptr;
+	uint32_t * sup_info = nullptr;
+	uint64_t sup_groups = 0;             // groups sup_info holds
 	uint32_t n_parts = 0, n_blocks = 0, n_keys = 0;
 	uint64_t n_filters = 0, n_key_instances = 0, device_bytes = 0, max_part = 0;
 	// host copies used by the schedulers and by tm_fib_insert
@@ -119,6 +127,7 @@ struct tm_fib {
 		v.key_dict = key_dict;
 		v.part_info = part_info;
 		v.part_stat = part_stat;
+		v.sup_info = sup_info;
 		v.n_parts = n_parts;
 		v.n_blocks = n_blocks;
 		v.n_keys = n_keys;

@@ -31,6 +31,8 @@ void tm_fib::release() {
 	dfree(key_dict);
 	dfree(part_info);
 	dfree(part_stat);
+	dfree(sup_info);
+	sup_groups = 0;
 	dfree(slot_epoch);
 	dfree(d_rich_words);
 	dfree(d_rich_bm);
@@ -241,8 +243,8 @@ struct ins_rec {
 // AND mask and valid-lane count.
 __global__ void __launch_bounds__(256)
 k_insert_lanes(const ins_rec * __restrict__ recs, uint32_t n, uint32_t n_blocks, uint32_t * __restrict__ blk_words,
-			   uint32_t * __restrict__ blk_info, uint64_t * __restrict__ slot_keys, uint32_t * __restrict__ slot_fid,
-			   uint32_t * __restrict__ slot_epoch, uint32_t epoch) {
+			   uint32_t * __restrict__ blk_info, uint32_t * __restrict__ sup_info, uint64_t * __restrict__ slot_keys,
+			   uint32_t * __restrict__ slot_fid, uint32_t * __restrict__ slot_epoch, uint32_t epoch) {
 	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
 	if (i >= n) return;
 	const ins_rec r = recs[i];
@@ -255,10 +257,26 @@ k_insert_lanes(const ins_rec * __restrict__ recs, uint32_t n, uint32_t n_blocks,
 	for (int j = 0; j < 6; ++j) {
 		blk_words[(size_t)r.block * BLOCK_WORDS + j * 64 + r.lane] = r.w[j];
 		atomicAnd(&blk_info[(size_t)r.block * 8 + j], r.w[j]);
+		if (sup_info) atomicAnd(&sup_info[(size_t)(r.block / SUP_BLOCKS) * 8 + j], r.w[j]);
 	}
 	atomicMax(&blk_info[(size_t)r.block * 8 + 6], r.lane + 1);
 	slot_keys[(size_t)r.block * 64 + r.lane] = r.kv;
 	slot_fid[(size_t)r.block * 64 + r.lane] = r.fid;
+}
+
+// Group masks: AND of the masks of blocks [8g, 8g + 8) (words 0-5; empty
+// blocks hold all ones), one thread per group.
+__global__ void __launch_bounds__(256)
+k_sup_masks(const uint32_t * __restrict__ blk_info, uint32_t nb, uint32_t * __restrict__ sup_info) {
+	const uint32_t gi = blockIdx.x * 256 + threadIdx.x;
+	if ((uint64_t)gi * SUP_BLOCKS >= nb) return;
+	uint32_t m[6] = {~0u, ~0u, ~0u, ~0u, ~0u, ~0u};
+	const uint32_t b1 = min(gi * SUP_BLOCKS + SUP_BLOCKS, nb);
+	for (uint32_t b = gi * SUP_BLOCKS; b < b1; ++b)
+		for (int j = 0; j < 6; ++j) m[j] &= blk_info[(size_t)b * 8 + j];
+	for (int j = 0; j < 6; ++j) sup_info[(size_t)gi * 8 + j] = m[j];
+	sup_info[(size_t)gi * 8 + 6] = 0;
+	sup_info[(size_t)gi * 8 + 7] = 0;
 }
 
 // Lane epochs of blocks [b0, nb): lanes already valid get `visible`, the
@@ -286,6 +304,26 @@ k_remap_keys(uint64_t * __restrict__ slot_keys, uint64_t n, const uint32_t * __r
 		o |= (id == KEY_NONE ? (uint64_t)KEY_NONE : (uint64_t)map[id]) << (16 * x);
 	}
 	slot_keys[i] = o;
+}
+
+// Group masks of every block, into a fresh array when the block count
+// outgrew the current one.  `fresh` says whether sup_info moved (rings must
+// then re-read the FIB, as for any other array that moved).
+int refresh_sup(tm_fib * fib, bool & fresh) {
+	const uint64_t ng = ((uint64_t)fib->n_blocks + SUP_BLOCKS - 1) / SUP_BLOCKS;
+	fresh = false;
+	if (!fib->sup_info || ng > fib->sup_groups) {
+		dfree(fib->sup_info);
+		fib->sup_groups = 0;
+		TM_CHECK(dalloc(&fib->sup_info, std::max<uint64_t>(ng, 1) * 8));
+		fib->sup_groups = std::max<uint64_t>(ng, 1);
+		fresh = true;
+	}
+	if (ng) hipLaunchKernelGGL(k_sup_masks, dim3((uint32_t)((ng + 255) / 256)), dim3(256), 0, 0, fib->blk_info,
+							   fib->n_blocks, fib->sup_info);
+	TM_HIP(hipGetLastError());
+	TM_HIP(hipDeviceSynchronize());
+	return TM_OK;
 }
 
 } // namespace
@@ -485,6 +523,11 @@ int tm_fib_upload(tm_fib * fib, const uint32_t * words, uint32_t n,
 	fib->n_rich_full = n_rich_full;
 	fib->n_parts = np;
 	fib->n_blocks = bb.nb;
+	{
+		bool fresh;
+		TM_CHECK(refresh_sup(fib, fresh));
+		bytes += fib->sup_groups * 32;
+	}
 	fib->n_keys = K;
 	fib->n_filters = n;
 	fib->n_key_instances = nkeys;
@@ -642,7 +685,7 @@ int live_commit_locked(tm_fib * fib, live_plan * pl) {
 	if (n) {
 		TM_HIP(hipMemcpyAsync(fib->d_recs, pl->recs.data(), sizeof(ins_rec) * n, hipMemcpyHostToDevice, s));
 		hipLaunchKernelGGL(k_insert_lanes, dim3((n + 255) / 256), dim3(256), 0, s, (const ins_rec *)fib->d_recs, n,
-						   fib->n_blocks, fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid, fib->slot_epoch, pl->epoch);
+						   fib->n_blocks, fib->blk_words, fib->blk_info, fib->sup_info, fib->slot_keys, fib->slot_fid, fib->slot_epoch, pl->epoch);
 		TM_HIP(hipGetLastError());
 	}
 	TM_HIP(hipMemcpyAsync(fib->part_info, pl->part_info.data(), pl->part_info.size() * 4, hipMemcpyHostToDevice, s));
@@ -896,7 +939,7 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 		int rc = put(d_recs, recs);
 		if (rc == TM_OK) {
 			hipLaunchKernelGGL(k_insert_lanes, dim3((uint32_t)((recs.size() + 255) / 256)), dim3(256), 0, 0, d_recs,
-							   (uint32_t)recs.size(), fib->n_blocks, fib->blk_words, fib->blk_info, fib->slot_keys, fib->slot_fid,
+							   (uint32_t)recs.size(), fib->n_blocks, fib->blk_words, fib->blk_info, fib->sup_info, fib->slot_keys, fib->slot_fid,
 							   fib->slot_epoch, e);
 			hipError_t e = hipGetLastError();
 			if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -906,6 +949,12 @@ int tm_fib_insert(tm_fib * fib, const uint32_t * words, uint32_t n, const uint32
 		TM_CHECK(rc);
 		TM_CHECK(put(fib->part_info, fib->h_part_info, (size_t)P * 8));
 		TM_CHECK(put(fib->part_stat, fib->h_part_stat, (size_t)P * 8));
+	}
+	// group masks of the new blocks (and of the blocks the lanes above joined)
+	{
+		bool fresh;
+		TM_CHECK(refresh_sup(fib, fresh));
+		if (fresh) bump = true;
 	}
 	fib->n_filters += n;
 	fib->n_key_instances += nkeys;
@@ -1140,6 +1189,11 @@ int tm_fib_load(tm_fib * fib, const char * path) {
 	fib->n_rich_full = (uint32_t)h.n_rich_full;
 	fib->n_parts = (uint32_t)P;
 	fib->n_blocks = (uint32_t)nb;
+	{
+		bool fresh;
+		TM_CHECK(refresh_sup(fib, fresh));
+		bytes += fib->sup_groups * 32;
+	}
 	fib->n_keys = (uint32_t)h.n_keys;
 	fib->n_filters = h.n_filters;
 	fib->n_key_instances = h.n_key_instances;

@@ -233,6 +233,8 @@ struct mq_args {
 	uint32_t epoch;                           // lanes with a later epoch are not seen
 	uint32_t full_exit;                       // KEYS_UNIQUE + LDS bitmaps: skip queries whose
 	                                          // bitmap holds every key
+	uint32_t groups;                          // walk blocks in groups, skipping a group whose
+	                                          // AND mask no query holds (compacted steps)
 	uint32_t last_mask;                       // valid bits of a bitmap row's last word
 	uint32_t * qfull;                         // nseg > 1 with full_exit: per query, set once its
 	                                          // merged gbm row holds every key
@@ -806,11 +808,12 @@ k_match_q(mq_args a) {
 					test_block(b, c_f, c_kv, cand, mb.z);
 				}
 			} else {
-				for (uint32_t b = fb + wave; b < be; b += NW) {
+				// one block: its candidates from its mask, then its tests
+				auto visit = [&](uint32_t b) {
 					const uint32_t * bi = a.f.blk_info + (size_t)b * 8;
 					const uint4 ma = sld4(bi), mb = sld4(bi + 4);
 					uint64_t cand[TILES];
-					if (block_cands(ma, mb, cand) == 0) continue;
+					if (block_cands(ma, mb, cand) == 0) return;
 					++n_visit;
 					uint32_t c_f[6];
 					const uint32_t * bw = a.f.blk_words + (size_t)b * BLOCK_WORDS + lane;
@@ -818,6 +821,28 @@ k_match_q(mq_args a) {
 					for (int j = 0; j < 6; ++j) c_f[j] = bw[j * 64];
 					const uint64_t c_kv = MODE == 0 ? a.f.slot_keys[(size_t)b * 64 + lane] : 0;
 					test_block(b, c_f, c_kv, cand, mb.z);
+				};
+				if (a.groups && a.f.sup_info) {
+					// Compacted steps: groups of SUP_BLOCKS blocks (aligned in
+					// the table), one group per wave at a time; a group whose
+					// AND of block masks no selected query contains is skipped
+					// whole (at 10M filters, 21M block scans per compacted
+					// launch become 14M group and block scans in a CPU model of
+					// the same layout, and the kernel 2.65 -> 2.42 ms).  Other
+					// steps keep one block per wave in turn: their items are
+					// short, and a group per wave leaves waves idle (a
+					// 1024-query step took 0.27 ms instead of 0.20).
+					const uint32_t g0 = fb / SUP_BLOCKS, g1 = (be + SUP_BLOCKS - 1) / SUP_BLOCKS;
+					for (uint32_t gi = g0 + wave; gi < g1; gi += NW) {
+						const uint32_t * si = a.f.sup_info + (size_t)gi * 8;
+						const uint4 sa = sld4(si), sb = sld4(si + 4);
+						uint64_t scand[TILES];
+						if (block_cands(sa, sb, scand) == 0) continue;
+						const uint32_t b1 = min(gi * SUP_BLOCKS + SUP_BLOCKS, be);
+						for (uint32_t b = max(gi * SUP_BLOCKS, fb); b < b1; ++b) visit(b);
+					}
+				} else {
+					for (uint32_t b = fb + wave; b < be; b += NW) visit(b);
 				}
 			}
 		}
@@ -1871,6 +1896,7 @@ int enqueue_persistent(tm_ring * r, uint32_t nq, bool events) {
 	a.porder = r->have_order ? r->d_porder : nullptr;
 	a.iorder = r->use_iorder ? r->d_iorder : nullptr;
 	a.full_exit = exit_on;
+	a.groups = compact;
 	a.last_mask = last_mask;
 	a.slot_epoch = r->fib->slot_epoch;
 	a.epoch = r->epoch;
