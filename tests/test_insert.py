@@ -272,3 +272,72 @@ def test_image_errors(gpu, tmp_path):
     with pytest.raises(tm.TagMatchError) as e:
         empty.save(str(tmp_path / "e.fib"))
     assert e.value.code == tm.TM_ERR_STATE
+
+
+def test_inserts_reach_compacted_steps(gpu, tmp_path):
+    """Compacted steps skip groups of 8 blocks by the AND of their masks.
+    Filters inserted live, then at a barrier, then through a saved and
+    loaded image are made to match queries whose key sets are still
+    incomplete, each carrying a key its query lacks: every compacted step
+    after an insert must find those keys (a group mask left stale would hide
+    them), and equal the oracle on the grown table."""
+    cw, ck, cv, _, _ = tm.workload(100000, 1, seed=440)
+    K = np.unique(cv)
+    # one filter holding every key, so that the rich-filter pass (and with
+    # it compaction) is on
+    extra = np.zeros((1, 6), np.uint32)
+    extra[0, 2] = 0b110101
+    w = np.concatenate([cw, extra])
+    ko = np.concatenate([ck, [ck[-1] + len(K)]]).astype(np.uint64)
+    kv = np.concatenate([cv, K]).astype(np.uint32)
+    w, ko, kv = tm.canonicalize(w, ko, kv)
+    fib, perm, off, masks = tm.build_fib(w, ko, kv, 8000)
+    m = np.asarray(masks, np.uint32).reshape(-1, 6)
+    q, _ = tm.Generator(seed=440).queries(20000, 100000, first=1 << 20)
+    nq = len(q)                                   # 313 tiles: a compacted step
+    rng = np.random.default_rng(441)
+
+    def step():
+        o, v, st = tm.match(fib, q, mode=tm.MODE_KEYS_UNIQUE)
+        return o, v
+
+    def batch(o, v, n):
+        """n filters, each contained in a query whose key set is incomplete
+        and carrying one key that query lacks."""
+        cnt = np.diff(o.astype(np.int64))
+        partial = np.nonzero(cnt < len(K))[0]
+        assert len(partial) >= n, len(partial)
+        fw, fk = [], []
+        for i in rng.choice(partial, n, replace=False):
+            have = v[int(o[i]):int(o[i + 1])]
+            missing = np.setdiff1d(K, have)
+            sel = np.nonzero(((m & ~q[i][None, :]) == 0).all(axis=1))[0]   # partitions the query selects
+            f = m[rng.choice(sel)].copy()
+            extra_bits = q[i] & rng.integers(0, 2**32, 6, dtype=np.uint64).astype(np.uint32)
+            f |= extra_bits & rng.integers(0, 2**32, 6, dtype=np.uint64).astype(np.uint32)
+            assert ((f & ~q[i]) == 0).all()
+            fw.append(f)
+            fk.append(int(rng.choice(missing)))
+        fko = np.arange(n + 1, dtype=np.uint32)
+        return np.array(fw, np.uint32), fko, np.array(fk, np.uint32)
+
+    o, v = step()
+    eo, ev = orc.match(w, ko, kv, q, mode=0)
+    assert np.array_equal(o, eo) and np.array_equal(v, ev)
+    # live, at a barrier, then through an image
+    for how in ("live", "barrier", "image"):
+        bw, bko, bk = batch(o, v, 150)
+        if how == "live":
+            assert fib.insert_live(bw, bko, bk)
+        elif how == "barrier":
+            fib.insert(bw, bko, bk, maxp=8000)
+        w, ko, kv = concat(w, ko, kv, bw, bko, bk)
+        if how == "image":
+            path = tmp_path / "g.fib"
+            fib.insert(bw, bko, bk, maxp=8000)
+            fib.save(str(path))
+            fib = tm.Fib.from_image(str(path))
+        o, v = step()
+        eo, ev = orc.match(w, ko, kv, q, mode=0)
+        log(f"{how}: {int(o[-1])} keys, {int(eo[-1])} expected")
+        assert np.array_equal(o, eo) and np.array_equal(v, ev), how
