@@ -351,7 +351,7 @@ def uneven_leg(cw, cko, ck, args):
         ring.configure(schedule=sched)
         if name == "persistent_tuned":
             # blocks per work item by partition-size bucket, tuned on these units
-            b0, b1, table = ring.autotune_buckets(up, uo)
+            b0, b1, table = ring.autotune_buckets(up, uo, reps=2)
             res["bucket_table"] = [int(x) for x in table]
             res["bucket_tune_device_ms"] = [b0, b1]
             log(f"uneven bucket table {res['bucket_table']}: device {b0:.2f} -> {b1:.2f} ms")
