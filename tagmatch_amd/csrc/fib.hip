@@ -135,6 +135,20 @@ int build_blocks(const uint32_t * words, const uint32_t * part_offsets, const ui
 	const uint32_t nb = pblock[n_parts];
 	std::vector<uint32_t> member(n);  // member[k] = input index at partitioned position k
 	std::iota(member.begin(), member.end(), 0u);
+	// sort keys: the filter's bits in significance order, most significant
+	// first; a table per input byte gives that byte's share of the key
+	std::vector<std::array<uint64_t, 3>> tab((size_t)24 * 256, std::array<uint64_t, 3>{0, 0, 0});
+	{
+		std::vector<int> rank(192, 0);
+		for (int r = 0; r < 192 && r < (int)order.size(); ++r) rank[order[r]] = r;
+		for (int bp = 0; bp < 24; ++bp)
+			for (int v = 0; v < 256; ++v)
+				for (int i = 0; i < 8; ++i)
+					if ((v >> i) & 1) {
+						const int r = rank[bp * 8 + i];
+						tab[(size_t)bp * 256 + v][r >> 6] |= 1ULL << (63 - (r & 63));
+					}
+	}
 	parallel_ranges(n_parts, [&](size_t pb, size_t pe) {
 		std::vector<std::array<uint64_t, 3>> key;
 		for (size_t p = pb; p < pe; ++p) {
@@ -143,11 +157,12 @@ int build_blocks(const uint32_t * words, const uint32_t * part_offsets, const ui
 			key.resize(e - b);
 			for (uint32_t i = b; i < e; ++i) {
 				std::array<uint64_t, 3> k{0, 0, 0};
-				const uint32_t * w = words + (size_t)i * 6;
-				for (int r = 0; r < 192; ++r) {
-					int bit = order[r];
-					if ((w[bit >> 5] >> (bit & 31)) & 1)
-						k[r >> 6] |= 1ULL << (63 - (r & 63));
+				const uint8_t * wb = reinterpret_cast<const uint8_t *>(words + (size_t)i * 6);
+				for (int bp = 0; bp < 24; ++bp) {
+					const auto & t = tab[(size_t)bp * 256 + wb[bp]];
+					k[0] |= t[0];
+					k[1] |= t[1];
+					k[2] |= t[2];
 				}
 				key[i - b] = k;
 			}

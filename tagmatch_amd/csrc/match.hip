@@ -1383,11 +1383,11 @@ __global__ void k_init_queues_active(uint32_t * __restrict__ act, uint32_t nqg, 
 	}
 }
 
-// Queries per partition (the lookup's counts).  A wave takes 64 queries at
-// a time, one per lane in registers (inverted words); lanes are then
-// partitions, 64 at a time, and each query's words reach them with
-// readlane, so the loop issues no memory access.  Counts meet in LDS (P
-// counters), then one global atomic per partition per workgroup.
+// Queries per partition (the lookup's counts).  Lanes are partitions, 64 at
+// a time; the wave's share of the queries is read with scalar loads, and
+// each lane counts the queries that select its partition in a register.
+// Counts meet in LDS (P counters), then one global atomic per partition per
+// workgroup.
 __global__ void __launch_bounds__(256)
 k_lookup_hist(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __restrict__ part_info, uint32_t P,
 			  uint32_t * __restrict__ pcount) {
@@ -1397,32 +1397,23 @@ k_lookup_hist(const uint32_t * __restrict__ q, uint32_t nq, const uint32_t * __r
 	const uint32_t lane = threadIdx.x & 63;
 	const uint32_t gw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
 	const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-	for (uint32_t c0 = gw * 64; c0 < nq; c0 += nwaves * 64) {
-		const uint32_t qi = c0 + lane;
-		uint32_t n0 = ~0u, n1 = ~0u, n2 = ~0u, n3 = ~0u, n4 = ~0u, n5 = ~0u;
-		if (qi < nq) {
-			const uint4 x = *reinterpret_cast<const uint4 *>(q + (size_t)qi * 8);
-			const uint2 y = *reinterpret_cast<const uint2 *>(q + (size_t)qi * 8 + 4);
-			n0 = ~x.x; n1 = ~x.y; n2 = ~x.z; n3 = ~x.w; n4 = ~y.x; n5 = ~y.y;
+	const uint32_t per = (nq + nwaves - 1) / nwaves;
+	const uint32_t qa = min(gw * per, nq), qb = min(qa + per, nq);
+	for (uint32_t p0 = 0; p0 < P; p0 += 64) {
+		const uint32_t p = p0 + lane;
+		uint32_t m[6] = {0, 0, 0, 0, 0, 0};
+		if (p < P)
+			for (int j = 0; j < 6; ++j) m[j] = part_info[(size_t)p * 8 + j];
+		uint32_t cnt = 0;
+#pragma unroll 4
+		for (uint32_t i = qa; i < qb; ++i) {
+			const uint4 x = sld4(q + (size_t)i * 8);
+			const uint2 y = sld2(q + (size_t)i * 8 + 4);
+			const uint32_t t = (m[0] & ~x.x) | (m[1] & ~x.y) | (m[2] & ~x.z) | (m[3] & ~x.w) | (m[4] & ~y.x) |
+							   (m[5] & ~y.y);
+			cnt += t == 0;
 		}
-		const uint32_t nj = min(64u, nq - c0);
-		for (uint32_t p0 = 0; p0 < P; p0 += 64) {
-			const uint32_t p = p0 + lane;
-			uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0, m4 = 0, m5 = 0;
-			if (p < P) {
-				const uint4 ma = *reinterpret_cast<const uint4 *>(part_info + (size_t)p * 8);
-				const uint2 mb = *reinterpret_cast<const uint2 *>(part_info + (size_t)p * 8 + 4);
-				m0 = ma.x; m1 = ma.y; m2 = ma.z; m3 = ma.w; m4 = mb.x; m5 = mb.y;
-			}
-			uint32_t cnt = 0;
-			for (uint32_t j = 0; j < nj; ++j) {
-				const uint32_t t = (m0 & __builtin_amdgcn_readlane(n0, j)) | (m1 & __builtin_amdgcn_readlane(n1, j)) |
-								   (m2 & __builtin_amdgcn_readlane(n2, j)) | (m3 & __builtin_amdgcn_readlane(n3, j)) |
-								   (m4 & __builtin_amdgcn_readlane(n4, j)) | (m5 & __builtin_amdgcn_readlane(n5, j));
-				cnt += t == 0;
-			}
-			if (p < P && cnt) atomicAdd(&s_cnt[p], cnt);
-		}
+		if (p < P && cnt) atomicAdd(&s_cnt[p], cnt);
 	}
 	__syncthreads();
 	for (uint32_t p = threadIdx.x; p < P; p += 256)
