@@ -446,7 +446,7 @@ def serving_leg(fib, args, n_filters):
     base_p99 = None
     peak = 0.2 * burst_peak   # when not even the lowest rung is sustained
     for frac in (0.2, 0.25, 0.3, 0.35, 0.4, 0.45, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0):
-        lat, _, ach, _ = poisson(frac * burst_peak, 0.6, False)
+        lat, _, ach, _ = poisson(frac * burst_peak, 1.0, False)
         p99 = float(np.percentile(lat, 99))
         base_p99 = base_p99 or p99
         ok = ach >= 0.98 * frac * burst_peak and p99 <= 2 * base_p99
