@@ -443,3 +443,30 @@ def test_lookup_counts_vs_bruteforce(gpu, nq):
             assert st.n_pairs == int(got.sum()), (mode, sched)
             ring.close()
     fib.close()
+
+
+def test_compact_tiles_threshold(gpu):
+    """compact_tiles: steps of more tiles than the threshold match only the
+    queries the rich-filter pass leaves incomplete.  A 2000-query step (32
+    tiles, below the default 128) compacted and not, in both row modes,
+    equals the oracle."""
+    cw, cko, ck, q, _ = tm.workload(200000, 2000, seed=0x7461676D61746302)
+    # a filter holding every key: the rich-filter pass (and compaction) is on
+    K = np.unique(ck)
+    extra = np.zeros((1, 6), np.uint32)
+    extra[0, 4] = 0b1001101
+    w = np.concatenate([cw, extra])
+    ko = np.concatenate([cko, [cko[-1] + len(K)]]).astype(np.uint64)
+    kv = np.concatenate([ck, K]).astype(np.uint32)
+    cw, cko, ck = tm.canonicalize(w, ko, kv)
+    fib, *_ = tm.build_fib(cw, cko, ck, 20000)
+    eo, ev = orc.match(cw, cko, ck, q, mode=0)
+    for ct in (0, 1, 1000):
+        o, v, st = tm.match(fib, q, compact_tiles=ct)
+        assert np.array_equal(o, eo) and np.array_equal(v, ev), ct
+        ring = tm.Ring(fib, len(q), mode=tm.MODE_KEY_BITMAP, compact_tiles=ct)
+        ring.submit(q)
+        ring.wait()
+        off, vals = tm.expand_key_bitmaps(ring.key_bitmaps(), fib.dictionary())
+        assert np.array_equal(off, eo) and np.array_equal(vals, ev), ct
+        ring.close()
