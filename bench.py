@@ -63,8 +63,8 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--filters", type=int, default=10_000_000)
     ap.add_argument("--queries", type=int, default=1_000_000)
     ap.add_argument("--maxp", type=int, default=256 * 1024)
