@@ -2926,16 +2926,25 @@ int tm_ring_autotune(tm_ring * r, uint32_t nq, const tm_match_config * cands, ui
 	if (r->pending) return set_error(TM_ERR_BUSY, "ring has a submission in flight");
 	if (reps == 0) reps = 1;
 	TM_CHECK(require_device());
+	for (uint32_t c = 0; c < n_cands; ++c)
+		if (cands[c].mode != r->cfg.mode) return set_error(TM_ERR_INVALID, "candidate %u changes the mode", c);
+	const tm_match_config saved = r->cfg;
+	auto fail = [&](int rc) {   // a failed run leaves the ring as it was configured
+		const std::string err = tm_last_error();
+		(void)tm_ring_config(r, &saved);
+		return set_error(rc, "%s", err.c_str());
+	};
 	double best_ms = 1e300;
 	uint32_t bi = 0;
 	for (uint32_t c = 0; c < n_cands; ++c) {
-		if (cands[c].mode != r->cfg.mode) return set_error(TM_ERR_INVALID, "candidate %u changes the mode", c);
-		TM_CHECK(tm_ring_config(r, &cands[c]));
+		int rc = tm_ring_config(r, &cands[c]);
+		if (rc != TM_OK) return fail(rc);
 		double t = 1e300;
 		for (uint32_t k = 0; k <= reps; ++k) {   // the first run warms up (capacities, graph)
-			TM_CHECK(tm_submit_resident(r, nq));
+			rc = tm_submit_resident(r, nq);
 			tm_status st;
-			TM_CHECK(tm_wait(r, &st));
+			if (rc == TM_OK) rc = tm_wait(r, &st);
+			if (rc != TM_OK) return fail(rc);
 			if (k > 0) t = std::min(t, st.device_ms);
 		}
 		if (times_ms) times_ms[c] = t;
@@ -2987,8 +2996,15 @@ int tm_ring_autotune_buckets(tm_ring * r, const uint32_t * unit_part, const uint
 		if (unit_part[i] >= r->fib->n_parts) return set_error(TM_ERR_INVALID, "unit %u: partition out of range", i);
 		present[size_bucket(r->fib->h_part_info[(size_t)unit_part[i] * 8 + 7])] = true;
 	}
+	uint32_t saved[TM_SIZE_BUCKETS];
+	std::copy(r->seg_bucket, r->seg_bucket + TM_SIZE_BUCKETS, saved);
+	auto fail = [&](int rc) {   // a failed run leaves the table as it was
+		std::copy(saved, saved + TM_SIZE_BUCKETS, r->seg_bucket);
+		return rc;
+	};
 	double cur;
-	TM_CHECK(run(&cur));
+	int rc = run(&cur);
+	if (rc != TM_OK) return fail(rc);
 	if (base_ms) *base_ms = cur;
 	for (uint32_t b = 0; b < TM_SIZE_BUCKETS; ++b) {
 		if (!present[b]) continue;
@@ -2998,7 +3014,8 @@ int tm_ring_autotune_buckets(tm_ring * r, const uint32_t * unit_part, const uint
 			if (cands[c] == keep) continue;
 			r->seg_bucket[b] = cands[c];
 			double t;
-			TM_CHECK(run(&t));
+			rc = run(&t);
+			if (rc != TM_OK) return fail(rc);
 			if (t < cur) {
 				cur = t;
 				best = cands[c];
