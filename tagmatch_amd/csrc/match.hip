@@ -291,6 +291,25 @@ __device__ inline uint2 sld2(const uint32_t * p) {
 }
 __device__ inline uint32_t sld1(const uint32_t * p) { return *(cu32_t *)(p); }
 
+// (f0 & q0) | ... | (f5 & q5) over registers: one v_and_b32 and five
+// v_and_or_b32.  Left to itself the compiler balances the OR tree into
+// v_or3_b32 and plain ANDs, eight instructions for the same value, in the
+// subset test that runs for every candidate query of every visited block.
+__device__ inline uint32_t vand_or(uint32_t f, uint32_t q, uint32_t acc) {
+	uint32_t r;
+	asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(f), "v"(q), "v"(acc));
+	return r;
+}
+__device__ inline uint32_t test6(uint32_t f0, uint32_t f1, uint32_t f2, uint32_t f3, uint32_t f4, uint32_t f5,
+								 uint32_t q0, uint32_t q1, uint32_t q2, uint32_t q3, uint32_t q4, uint32_t q5) {
+	uint32_t acc = f0 & q0;
+	acc = vand_or(f1, q1, acc);
+	acc = vand_or(f2, q2, acc);
+	acc = vand_or(f3, q3, acc);
+	acc = vand_or(f4, q4, acc);
+	return vand_or(f5, q5, acc);
+}
+
 // True when a bitmap row in LDS holds every key.  Each lane starts at word
 // lane % kw: rows are kw words apart, so lanes that started at word 0 would
 // all read one LDS bank.
@@ -563,7 +582,14 @@ k_match_q(mq_args a) {
 			const uint32_t ql = q0 + jl;
 			const uint32_t gq = (MODE == 1 && a.qmap) ? sld1(a.qmap + ql) : ql;
 			if (a.log) {   // uniform: a scalar branch around the per-lane count
-				if ((h >> lane) & 1) key_acc += key_count(kv, a.f.ovf, ovf_n);
+				if ((h >> lane) & 1) {
+					// the empty asm keeps the count's arithmetic inside the
+					// branch: hoisted, it cost every block visit about 17
+					// VALU instructions in steps that log nothing
+					uint64_t kx = kv;
+					asm volatile("" : "+v"(kx));
+					key_acc += key_count(kx, a.f.ovf, ovf_n);
+				}
 			}
 			sink.add(h, jl, gq, b, kv, lane);
 		};
@@ -638,10 +664,8 @@ k_match_q(mq_args a) {
 							const uint2 c0 = *reinterpret_cast<const uint2 *>(qq0 + 4);
 							const uint4 a1 = *reinterpret_cast<const uint4 *>(qq1);
 							const uint2 c1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
-							const uint32_t acc0 =
-								(f0 & a0.x) | (f1 & a0.y) | (f2 & a0.z) | (f3 & a0.w) | (f4 & c0.x) | (f5 & c0.y);
-							const uint32_t acc1 =
-								(f0 & a1.x) | (f1 & a1.y) | (f2 & a1.z) | (f3 & a1.w) | (f4 & c1.x) | (f5 & c1.y);
+							const uint32_t acc0 = test6(f0, f1, f2, f3, f4, f5, a0.x, a0.y, a0.z, a0.w, c0.x, c0.y);
+							const uint32_t acc1 = test6(f0, f1, f2, f3, f4, f5, a1.x, a1.y, a1.z, a1.w, c1.x, c1.y);
 							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
 							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
 							if (h0) on_hit(h0, t * 64 + j0, b, kv);
