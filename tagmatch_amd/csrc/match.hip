@@ -291,23 +291,37 @@ __device__ inline uint2 sld2(const uint32_t * p) {
 }
 __device__ inline uint32_t sld1(const uint32_t * p) { return *(cu32_t *)(p); }
 
-// (f0 & q0) | ... | (f5 & q5) over registers: one v_and_b32 and five
-// v_and_or_b32.  Left to itself the compiler balances the OR tree into
+// (f0 & q0) | ... | (f5 & q5): one v_and_b32 and five v_and_or_b32 in one
+// asm block.  Left to itself the compiler balances the OR tree into
 // v_or3_b32 and plain ANDs, eight instructions for the same value, in the
-// subset test that runs for every candidate query of every visited block.
-__device__ inline uint32_t vand_or(uint32_t f, uint32_t q, uint32_t acc) {
-	uint32_t r;
-	asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(f), "v"(q), "v"(acc));
-	return r;
-}
+// subset tests that run for every candidate query of every visited block;
+// separate asm statements would each be followed by a wait state.
 __device__ inline uint32_t test6(uint32_t f0, uint32_t f1, uint32_t f2, uint32_t f3, uint32_t f4, uint32_t f5,
 								 uint32_t q0, uint32_t q1, uint32_t q2, uint32_t q3, uint32_t q4, uint32_t q5) {
-	uint32_t acc = f0 & q0;
-	acc = vand_or(f1, q1, acc);
-	acc = vand_or(f2, q2, acc);
-	acc = vand_or(f3, q3, acc);
-	acc = vand_or(f4, q4, acc);
-	return vand_or(f5, q5, acc);
+	uint32_t acc;
+	asm("v_and_b32 %0, %1, %7\n\t"
+		"v_and_or_b32 %0, %2, %8, %0\n\t"
+		"v_and_or_b32 %0, %3, %9, %0\n\t"
+		"v_and_or_b32 %0, %4, %10, %0\n\t"
+		"v_and_or_b32 %0, %5, %11, %0\n\t"
+		"v_and_or_b32 %0, %6, %12, %0"
+		: "=&v"(acc)
+		: "v"(f0), "v"(f1), "v"(f2), "v"(f3), "v"(f4), "v"(f5), "v"(q0), "v"(q1), "v"(q2), "v"(q3), "v"(q4), "v"(q5));
+	return acc;
+}
+// The same with uniform masks (scalar registers) against per-lane words.
+__device__ inline uint32_t test6s(const uint4 & ma, const uint4 & mb, const uint32_t (&q)[6]) {
+	uint32_t acc;
+	asm("v_and_b32 %0, %1, %7\n\t"
+		"v_and_or_b32 %0, %2, %8, %0\n\t"
+		"v_and_or_b32 %0, %3, %9, %0\n\t"
+		"v_and_or_b32 %0, %4, %10, %0\n\t"
+		"v_and_or_b32 %0, %5, %11, %0\n\t"
+		"v_and_or_b32 %0, %6, %12, %0"
+		: "=&v"(acc)
+		: "s"(ma.x), "s"(ma.y), "s"(ma.z), "s"(ma.w), "s"(mb.x), "s"(mb.y), "v"(q[0]), "v"(q[1]), "v"(q[2]),
+		  "v"(q[3]), "v"(q[4]), "v"(q[5]));
+	return acc;
 }
 
 // True when a bitmap row in LDS holds every key.  Each lane starts at word
@@ -415,8 +429,11 @@ struct hit_sink {
 // atomics); MODE 1: hit pairs.  VAR 0: candidate query words from LDS, two
 // tests per iteration; VAR 1: from pre-inverted query words in HBM through
 // scalar loads, four independent tests per iteration.
+// The default one-tile instance is held to 8 waves per SIMD (64 VGPRs, a few
+// spilled outside the test loop): at 10M filters its compacted launch took
+// 2.25 instead of 2.32 ms and the every-match step 642 instead of 675 ms.
 template <int MODE, bool LDS_BM, int TILES, int NW, int VAR>
-__global__ void __launch_bounds__(NW * 64)
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(TILES == 1 && VAR == 2 ? 8 : 1)))
 k_match_q(mq_args a) {
 	constexpr int NQ = TILES * 64;
 	constexpr int NT = NW * 64;
@@ -607,8 +624,7 @@ k_match_q(mq_args a) {
 			uint32_t nsel = 0;
 #pragma unroll
 			for (int t = 0; t < TILES; ++t) {
-				const uint32_t x = (pa.x & qv[t][0]) | (pa.y & qv[t][1]) | (pa.z & qv[t][2]) |
-								   (pa.w & qv[t][3]) | (pb.x & qv[t][4]) | (pb.y & qv[t][5]);
+				const uint32_t x = test6s(pa, pb, qv[t]);
 				qsel[t] = __ballot(x == 0) & qok[t];
 				anyq |= qsel[t];
 				nsel += (uint32_t)__popcll(qsel[t]);
@@ -743,8 +759,7 @@ k_match_q(mq_args a) {
 #pragma unroll
 				for (int t = 0; t < TILES; ++t) {
 					if (qsel[t] == 0) { cand[t] = 0; continue; }
-					const uint32_t x = (ma.x & qv[t][0]) | (ma.y & qv[t][1]) | (ma.z & qv[t][2]) |
-									   (ma.w & qv[t][3]) | (mb.x & qv[t][4]) | (mb.y & qv[t][5]);
+					const uint32_t x = test6s(ma, mb, qv[t]);
 					cand[t] = __ballot(x == 0) & qsel[t];
 					any |= cand[t];
 				}
