@@ -2137,6 +2137,15 @@ int run_plain(tm_ring * r, uint32_t nq, const uint32_t * list, std::vector<punit
 	return TM_OK;
 }
 
+// Copies of a ring's buffers go through its own non-blocking stream, which is
+// then waited for: while another thread of the process captures a graph, the
+// runtime refuses operations on the legacy stream ("would make the legacy
+// stream depend on a capturing blocking stream").
+hipError_t ring_copy(tm_ring * r, void * dst, const void * src, size_t bytes, hipMemcpyKind kind) {
+	hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, r->s);
+	return e == hipSuccess ? hipStreamSynchronize(r->s) : e;
+}
+
 int ensure_capacity(tm_ring * r) {
 	const uint32_t P = r->P;
 	const uint32_t nblk = r->fib->n_blocks;
@@ -2170,7 +2179,8 @@ int ensure_capacity(tm_ring * r) {
 	if (keyrows(r->cfg) && !r->d_gbm) {
 		size_t n = (size_t)r->max_q * r->kw;
 		TM_CHECK(dalloc(&r->d_gbm, n));
-		TM_HIP(hipMemset(r->d_gbm, 0, std::max<size_t>(n, 1) * 4));
+		TM_HIP(hipMemsetAsync(r->d_gbm, 0, std::max<size_t>(n, 1) * 4, r->s));
+		TM_HIP(hipStreamSynchronize(r->s));
 	}
 	if (keyrows(r->cfg) && !r->d_qfull) {
 		TM_CHECK(dalloc(&r->d_qfull, std::max<uint32_t>(r->max_q, 1)));
@@ -2847,7 +2857,7 @@ int tm_wait(tm_ring * r, tm_status * st) {
 			// packed rows beyond those copied with the step
 			const uint64_t n = *r->h_npack;
 			if (n > r->pack_sent) {
-				const hipError_t e = hipMemcpy(r->host_prow + r->pack_sent * r->kw, r->d_prow + r->pack_sent * r->kw,
+				const hipError_t e = ring_copy(r, r->host_prow + r->pack_sent * r->kw, r->d_prow + r->pack_sent * r->kw,
 											   (n - r->pack_sent) * r->kw * 4, hipMemcpyDeviceToHost);
 				if (e != hipSuccess) {
 					end_pending(r);
@@ -2891,13 +2901,13 @@ int tm_read_results(tm_ring * r, uint64_t * offsets, uint32_t * values, uint64_t
 		offsets[0] = 0;
 		return TM_OK;
 	}
-	TM_HIP(hipMemcpy(offsets, r->d_off, ((size_t)nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+	TM_HIP(ring_copy(r, offsets, r->d_off, ((size_t)nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
 	uint64_t total = offsets[nq];
 	if (!values) return TM_OK;
 	if (total > capacity)
 		return set_error(TM_ERR_OVERFLOW, "%llu result values, capacity %llu", (unsigned long long)total,
 						 (unsigned long long)capacity);
-	if (total) TM_HIP(hipMemcpy(values, r->d_vals, total * sizeof(uint32_t), hipMemcpyDeviceToHost));
+	if (total) TM_HIP(ring_copy(r, values, r->d_vals, total * sizeof(uint32_t), hipMemcpyDeviceToHost));
 	return TM_OK;
 }
 
@@ -2913,7 +2923,7 @@ int tm_read_key_bitmaps(tm_ring * r, uint32_t * rows, uint64_t capacity_words, u
 	if (n == 0) return TM_OK;
 	if (!rows) return set_error(TM_ERR_INVALID, "rows is NULL");
 	TM_CHECK(require_device());
-	TM_HIP(hipMemcpy(rows, r->d_gbm, n * 4, hipMemcpyDeviceToHost));
+	TM_HIP(ring_copy(r, rows, r->d_gbm, n * 4, hipMemcpyDeviceToHost));
 	return TM_OK;
 }
 
@@ -3075,10 +3085,10 @@ int tm_ring_unit_log(tm_ring * r, float * features, uint32_t * hits, uint32_t * 
 	if (n == 0) return TM_OK;
 	TM_CHECK(require_device());
 	if (features)
-		TM_HIP(hipMemcpy(features, r->d_ufeat, (size_t)n * N_FEAT * sizeof(float), hipMemcpyDeviceToHost));
+		TM_HIP(ring_copy(r, features, r->d_ufeat, (size_t)n * N_FEAT * sizeof(float), hipMemcpyDeviceToHost));
 	if (units) {
 		std::vector<uint4> u(n);
-		TM_HIP(hipMemcpy(u.data(), r->d_units, (size_t)n * sizeof(uint4), hipMemcpyDeviceToHost));
+		TM_HIP(ring_copy(r, u.data(), r->d_units, (size_t)n * sizeof(uint4), hipMemcpyDeviceToHost));
 		for (uint32_t i = 0; i < n; ++i) {
 			units[(size_t)i * 3] = u[i].x;
 			units[(size_t)i * 3 + 1] = u[i].y;
@@ -3087,7 +3097,7 @@ int tm_ring_unit_log(tm_ring * r, float * features, uint32_t * hits, uint32_t * 
 	}
 	if (hits || keys || ticks) {
 		std::vector<uint32_t> log((size_t)n * 3);
-		TM_HIP(hipMemcpy(log.data(), r->d_log, (size_t)n * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+		TM_HIP(ring_copy(r, log.data(), r->d_log, (size_t)n * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost));
 		for (uint32_t i = 0; i < n; ++i) {
 			if (hits) hits[i] = log[(size_t)i * 3];
 			if (keys) keys[i] = log[(size_t)i * 3 + 1];
@@ -3113,7 +3123,7 @@ int tm_ring_lookup_counts(tm_ring * r, uint32_t * counts, uint32_t n_parts) {
 	if (n_parts != r->P) return set_error(TM_ERR_INVALID, "n_parts %u != %u", n_parts, r->P);
 	if (r->pending) return set_error(TM_ERR_BUSY, "call tm_wait() first");
 	TM_CHECK(require_device());
-	if (n_parts) TM_HIP(hipMemcpy(counts, r->d_pcount, (size_t)n_parts * 4, hipMemcpyDeviceToHost));
+	if (n_parts) TM_HIP(ring_copy(r, counts, r->d_pcount, (size_t)n_parts * 4, hipMemcpyDeviceToHost));
 	return TM_OK;
 }
 
