@@ -671,13 +671,16 @@ k_match_q(mq_args a) {
 					uint64_t c = cand[t];
 					n_test += (uint32_t)__popcll(c);
 					if ((VAR & 5) == 0) {
-						// candidate query words broadcast from LDS, two per iteration
+						// candidate query words broadcast from LDS, two per iteration.
+						// Scalar work per pair kept small (the scalar unit is as busy
+						// as the vector one here): bits cleared by index, and one
+						// branch over both results for the common pair without hits.
 						while (c) {
 							const uint32_t j0 = (uint32_t)__builtin_ctzll(c);
-							c &= c - 1;
-							const uint64_t keep1 = c ? ~0ULL : 0ULL;
-							const uint32_t j1 = c ? (uint32_t)__builtin_ctzll(c) : j0;
-							c &= c - 1;
+							c &= ~(1ULL << j0);
+							const bool has1 = c != 0;
+							const uint32_t j1 = has1 ? (uint32_t)__builtin_ctzll(c) : j0;
+							c &= ~(1ULL << j1);
 							const uint32_t * qq0 = &s_q[(t * 64 + j0) * 8];
 							const uint32_t * qq1 = &s_q[(t * 64 + j1) * 8];
 							const uint4 a0 = *reinterpret_cast<const uint4 *>(qq0);
@@ -686,8 +689,10 @@ k_match_q(mq_args a) {
 							const uint2 c1 = *reinterpret_cast<const uint2 *>(qq1 + 4);
 							const uint32_t acc0 = test6(f0, f1, f2, f3, f4, f5, a0.x, a0.y, a0.z, a0.w, c0.x, c0.y);
 							const uint32_t acc1 = test6(f0, f1, f2, f3, f4, f5, a1.x, a1.y, a1.z, a1.w, c1.x, c1.y);
-							const uint64_t h0 = __ballot(acc0 == 0) & lmask;
-							const uint64_t h1 = __ballot(acc1 == 0) & lmask & keep1;
+							const uint64_t r0 = __ballot(acc0 == 0), r1 = __ballot(acc1 == 0);
+							if (((r0 | r1) & lmask) == 0) continue;   // (r1 = r0 when j1 = j0)
+							const uint64_t h0 = r0 & lmask;
+							const uint64_t h1 = has1 ? r1 & lmask : 0ULL;
 							if (h0) on_hit(h0, t * 64 + j0, b, kv);
 							if (h1) on_hit(h1, t * 64 + j1, b, kv);
 						}
