@@ -671,16 +671,17 @@ k_match_q(mq_args a) {
 					uint64_t c = cand[t];
 					n_test += (uint32_t)__popcll(c);
 					if ((VAR & 5) == 0) {
-						// candidate query words broadcast from LDS, two per iteration.
-						// Scalar work per pair kept small (the scalar unit is as busy
-						// as the vector one here): bits cleared by index, and one
-						// branch over both results for the common pair without hits.
+						// candidate query words broadcast from LDS, two per iteration:
+						// the lowest and the highest candidate left (one and the same
+						// for the last of an odd count).  Scalar work per pair kept
+						// small (the scalar unit is as busy as the vector one here):
+						// bits cleared by index, and one branch over both results for
+						// the common pair without hits.
 						while (c) {
 							const uint32_t j0 = (uint32_t)__builtin_ctzll(c);
-							c &= ~(1ULL << j0);
-							const bool has1 = c != 0;
-							const uint32_t j1 = has1 ? (uint32_t)__builtin_ctzll(c) : j0;
-							c &= ~(1ULL << j1);
+							const uint32_t j1 = 63u - (uint32_t)__builtin_clzll(c);
+							c &= ~((1ULL << j0) | (1ULL << j1));
+							const bool has1 = j1 != j0;
 							const uint32_t * qq0 = &s_q[(t * 64 + j0) * 8];
 							const uint32_t * qq1 = &s_q[(t * 64 + j1) * 8];
 							const uint4 a0 = *reinterpret_cast<const uint4 *>(qq0);
