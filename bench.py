@@ -106,6 +106,8 @@ def parse():
                     help="queries the plain form logs units over (drawn apart from the timed ones)")
     ap.add_argument("--c4-log-seconds", type=float, default=660.0, help="time limit of the logging runs")
     ap.add_argument("--c5-filters", type=int, default=200_000_000)
+    ap.add_argument("--c5-step-queries", type=int, default=1_000_000,
+                    help="c5 leg: queries of the packed-delivery step timed on the big table (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run an elementwise device copy of MIB MiB (read + write) three times, "
                          "to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE, and exit")
@@ -624,7 +626,8 @@ def c5_leg(args):
     Poisson arrivals at 30/70/95% of the sustained peak with 10k-filter
     inserts every 100 ms; a request's results alone are checked against the
     oracle on a sample first."""
-    cw, cko, ck, q, fib, perm, off, masks, setup_s = big_table(args.c5_filters, args, 4096)
+    cw, cko, ck, q, fib, perm, off, masks, setup_s = big_table(args.c5_filters, args,
+                                                               max(4096, args.c5_step_queries))
     res = {"leg": "c5", "filters": args.c5_filters, "distinct_filters": int(cw.shape[0]),
            "partitions": int(len(off) - 1), "setup_seconds": setup_s, "device_bytes": int(fib.stats()["device_bytes"])}
     n_or = 32
@@ -639,6 +642,21 @@ def c5_leg(args):
     res["oracle_sample_queries"] = n_or
     log(f"c5 oracle sample: {res['oracle_equal_sample']}")
     del cw, cko, ck
+    if args.c5_step_queries:
+        # the headline step on this table: every query's key set in host
+        # memory, packed, two rings alternating, steps replayed from graphs
+        nq = args.c5_step_queries
+        kwords = (int(fib.stats()["n_distinct_keys"]) + 31) // 32
+        ms, mm, packed, _, prings, pbufs = packed_steps(fib, q[:nq], {"use_graph": 1}, 5, 2, kwords)
+        res["step"] = {"queries": nq, "ms_per_step": ms, "queries_per_s": nq / (ms / 1e3),
+                       "match_ms_median": float(np.median(mm)) if mm else None,
+                       "packed_rows_per_step": packed[-1],
+                       "note": "packed key-set delivery to pinned host memory, as the headline step; "
+                               "match_ms is 0 for steps replayed from captured graphs"}
+        log(f"c5 step: {res['step']}")
+        for r in prings:
+            r.close()
+        del pbufs
     res["serving"] = serving_leg(fib, args, args.c5_filters)
     fib.close()
     return res
